@@ -1,0 +1,184 @@
+"""Headline benchmark: SLOS strong simulation, n=11 photons in m=32 modes, complex64, one GPU.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--workload c5|c2]
+
+A step is one full FockSLOSKernel.probs(U) pass (11 photon layers, the last one fused with the
+|amp|^2 * prod t! readout, float32 output) over one seeded Haar unitary already resident in HBM.
+`value` = output probabilities per second (S_11 / step time). The dominant kernel is the last
+layer; its launches are bracketed by HIP events on the stream they run on, and `roofline.achieved`
+= its algorithmic bytes (read layer 10 once: 8 S_10, write the probabilities: 4 S_11) / its mean
+duration. Accuracy of the same output: max|dp| and max relative error over 10,000 seeded outputs
+against FP64 Ryser permanents computed on the GPU, and the FP64 total mass.
+`cpu_baseline`: the oracle's C++/OpenMP pull-form restatement on this host's cores (a bounded
+sample: layers 1..10 complete plus the first 2^27 states of layer 11, the full-run time
+extrapolated linearly in layer-11 states).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SLOS probs/s + % HBM roofline (n=11,m=32,c64) at max|Δp|≤1e-6 vs FP64"
+PEAK_HBM_GBS = 8000.0
+WORKLOADS = {
+    "c5": dict(n=11, m=32, seed=11032),
+    "c2": dict(n=8, m=24, seed=8024),
+}
+
+
+def bytes_alg(n: int, m: int) -> int:
+    """Compulsory HBM bytes of one forward: every layer read once and written once (complex64),
+    the last written as float32: sum_{k=1..n} 8 S_{k-1} + sum_{k=1..n-1} 8 S_k + 4 S_n."""
+    S = [math.comb(m + k - 1, k) for k in range(n + 1)]
+    return sum(8 * S[k - 1] for k in range(1, n + 1)) + sum(8 * S[k] for k in range(1, n)) + 4 * S[n]
+
+
+def load_traffic(tag: str):
+    path = os.path.join(ROOT, "profiles", f"layer_block_{tag}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-last", type=int, default=1 << 27, help="layer-n states in the CPU sample")
+    ap.add_argument("--no-compensated", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the SLOS engine runs on a single MI355X in this build "
+              "(multi-GPU sharding is designed, not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from photonic_engines_amd import _native as nat
+    from photonic_engines_amd.slos import FockSLOSKernel, haar_unitary
+
+    wl = WORKLOADS[args.workload]
+    n, m = wl["n"], wl["m"]
+    occ = [1] * n + [0] * (m - n)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    kern = FockSLOSKernel(n, m, occ, device=str(dev))
+    plan, modes = kern._plan, kern.input_modes
+    U128 = haar_unitary(m, wl["seed"])
+    U = U128.to(torch.complex64).to(dev).unsqueeze(0).contiguous()
+    S = plan.sizes
+    ones = torch.ones((1, 1), dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step(flags: int, events=None):
+        psi = ones
+        for k in range(1, n):
+            psi = plan.layer_step(U, k, modes[k - 1], psi, nat.OUT_RAW, flags=flags)
+        if events is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+        p = plan.layer_step(U, n, modes[n - 1], psi, nat.OUT_PROBS, scale=1.0, flags=flags)
+        if events is not None:
+            e1.record(stream)
+            events.append((e0, e1))
+        return p
+
+    def timed(flags: int):
+        for _ in range(args.warmup):
+            p = step(flags)
+        torch.cuda.synchronize()
+        events = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            p = step(flags, events)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        last_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
+        return p, dt, last_ms
+
+    p, dt, last_ms = timed(0)
+    S_n = S[n]
+    # accuracy of this output: 10,000 seeded outputs against FP64 Ryser on the GPU, FP64 mass
+    rng = np.random.default_rng(wl["seed"])
+    idx = np.sort(rng.choice(S_n, size=min(10_000, S_n), replace=False)).astype(np.int64)
+    idx_t = torch.as_tensor(idx, device=dev)
+    ref = kern.spot_probs(U128, idx_t).cpu().numpy()
+    got = p[0, idx_t].double().cpu().numpy()
+    max_abs = float(np.abs(got - ref).max())
+    max_rel = float((np.abs(got - ref) / np.maximum(ref, 1e-300)).max())
+    mass = float(kern.total_mass(p)[0])
+
+    last_bytes = 8 * S[n - 1] + 4 * S[n]
+    achieved = last_bytes / (last_ms * 1e-3) / 1e9
+    traffic = load_traffic(args.workload)
+    out = {
+        "metric": METRIC,
+        "value": S_n / dt,
+        "unit": "probs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"SLOS fock n={n} m={m}, input [1]*{n}+[0]*{m - n}, one seeded Haar U "
+                               f"(seed {wl['seed']}), lossless, float32 probabilities",
+                   "n": n, "m": m, "states": S_n, "global_batch": 1, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "kernel": f"k_layer_block (layer {n})", "kernel_ms": last_ms,
+                     "alg_bytes_per_launch": last_bytes},
+        "whole_forward": {"alg_bytes": bytes_alg(n, m), "alg_GBps": bytes_alg(n, m) / dt / 1e9,
+                          "alg_frac": bytes_alg(n, m) / dt / 1e9 / PEAK_HBM_GBS},
+        "accuracy": {"max_abs_dp": max_abs, "max_rel": max_rel, "spot_outputs": int(idx.size),
+                     "oracle": "FP64 Ryser on the GPU", "mass_fp64": mass},
+    }
+    del p
+    if not args.no_compensated:
+        pc, dtc, last_c = timed(1)
+        gc = pc[0, idx_t].double().cpu().numpy()
+        out["variants"] = {"compensated": {"ms_per_step": dtc * 1e3, "value": S_n / dtc, "kernel_ms": last_c,
+                                           "max_abs_dp": float(np.abs(gc - ref).max()),
+                                           "max_rel": float((np.abs(gc - ref) / np.maximum(ref, 1e-300)).max())}}
+        del pc
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        from oracle import pull
+        threads = len(os.sched_getaffinity(0))
+        t_pre, t_last, done, _ = pull.forward_sample(n, m, occ, U128.numpy(), args.cpu_last, prec=64, threads=threads)
+        full = t_pre + t_last * S_n / done
+        out["cpu_baseline"] = {"value": S_n / full, "unit": "probs/s", "cores": threads, "kind": "port",
+                               "sample": f"oracle C++/OpenMP pull form, complex64, layers 1..{n - 1} complete "
+                                         f"({t_pre:.2f} s) + first {done} of {S_n} layer-{n} states "
+                                         f"({t_last:.2f} s), extrapolated to {full:.1f} s; CPU {cpu_model()}"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

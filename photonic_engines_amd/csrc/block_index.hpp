@@ -1,0 +1,229 @@
+// Index arithmetic of the block-structured layer kernel, shared by the device kernel and host code
+// (the plan builds the suffix-tuple table with it; the CPU tests execute it unchanged).
+//
+// A child state of layer K is c = (Q, tau): Q = its first d = K - SUF sorted positions, tau = the
+// last SUF positions, all >= lo = lo[last(Q)] (lo = 0 when d = 0). The children of one Q are
+// contiguous (the "block" of Q, size cg[SUF][lo]), and tau runs over the SUF-tuples >= lo in
+// lex order, which is the tail of the lex order of all SUF-tuples of the space. So with u the
+// position of c inside its block:
+//  * dropping the prefix photon q_l (first copy of its mode) gives the parent (Q - q_l, tau); for
+//    a fixed Q these parents form an aligned contiguous run of layer K-1: parent = base_l + u;
+//  * dropping the suffix photon x_j gives (Q, tau - x_j) inside the block of Q in layer K-1
+//    (suffix length SUF-1): parent = LBp + lr_j, lr_j = global (SUF-1)-tuple index of tau - x_j.
+// Ranks use the counting table of space.hpp: with lo_0 = 0, lo_{i+1} = lo[q_i],
+//   T_i = cg[K-i][lo_i] - cg[K-i][q_i]        (child terms, photon count K)
+//   A_i = cg[K-1-i][lo_i] - cg[K-1-i][q_i]    (same position, one photon fewer)
+//   M_l = cg[K-1-l][lo_l] - cg[K-1-l][q_{l+1}] (q_{l+1} moved into position l)
+//   block start CB = sum_{i<d} T_i, local parent base LB = sum_{i<d} A_i,
+//   prefix parent block of Q - q_l: sum_{i<l} A_i + M_l + sum_{l+2<=i<d} T_i (l < d-1),
+//                                  sum_{i<d-1} A_i (l = d-1).
+#pragma once
+#include <stdint.h>
+
+#include "space.hpp"
+
+namespace slos {
+
+constexpr int kSuf = 3;
+
+// suffix-tuple table entry for a SUF-tuple tau (lex order over all SUF-tuples of the space)
+//   w0: x0 | x1 << 8 | x2 << 16 | valid << 24 (3 bits: first copy of each mode inside tau)
+//       | lead << 27 (2 bits: copies of x0 in tau) | fcode << 29 (3 bits: prod of run
+//       factorials of tau, 1 -> 0, 2 -> 1, 6 -> 2)
+//   lr_j: global (SUF-1)-tuple index of tau without x_j
+struct TupleEntry {
+  uint32_t w0, lr0, lr1, lr2;
+};
+
+struct BlockTables {
+  const uint64_t* cg;        // (n + 2) x (me + 1)
+  const int* lo;             // me
+  const TupleEntry* tuples;  // cg[SUF][0] entries
+  int me, stride, multiset;
+};
+
+struct BlockInfo {
+  uint64_t CB;      // index of the block's first child in layer K
+  uint64_t LBp;     // local parent base minus the (SUF-1)-tuple tail offset of lo
+  uint32_t L;       // block size
+  uint32_t toff;    // tuple-table index of the block's first tau
+  int lo;           // smallest admissible suffix position
+  int skip0;        // multiset and d >= 1: a tau starting at lo continues last(Q)'s run
+  int lastrun;      // copies of last(Q) in Q (multiset)
+  float fQ;         // prod of run factorials of Q
+};
+
+template <int MAXD>
+struct Streams {
+  uint64_t base[MAXD];
+  int mode[MAXD];
+  int count;
+};
+
+template <int MAXD>
+SLOS_HD void block_info(const BlockTables& t, int K, int d, const int* q, BlockInfo& bi, Streams<MAXD>& st) {
+  const uint64_t* cg = t.cg;
+  const int W = t.stride;
+#define SLOS_C(r, x) cg[(r) * W + (x)]
+  int lov[MAXD + 1];
+  lov[0] = 0;
+#pragma unroll
+  for (int i = 0; i < MAXD; ++i)
+    if (i < d) lov[i + 1] = t.lo[q[i]];
+  const int lo = lov[d];
+  uint64_t CB = 0, LB = 0;
+#pragma unroll
+  for (int i = 0; i < MAXD; ++i)
+    if (i < d) {
+      CB += SLOS_C(K - i, lov[i]) - SLOS_C(K - i, q[i]);
+      LB += SLOS_C(K - 1 - i, lov[i]) - SLOS_C(K - 1 - i, q[i]);
+    }
+  bi.CB = CB;
+  bi.L = (uint32_t)SLOS_C(kSuf, lo);
+  bi.toff = (uint32_t)(SLOS_C(kSuf, 0) - SLOS_C(kSuf, lo));
+  bi.LBp = LB - (SLOS_C(kSuf - 1, 0) - SLOS_C(kSuf - 1, lo));
+  bi.lo = lo;
+  bi.skip0 = t.multiset && d >= 1;
+  float fQ = 1.f, run = 1.f;
+  if (t.multiset) {
+#pragma unroll
+    for (int i = 1; i < MAXD; ++i)
+      if (i < d) {
+        run = (q[i] == q[i - 1]) ? run + 1.f : 1.f;
+        fQ *= run;
+      }
+  }
+  bi.fQ = fQ;
+  bi.lastrun = d >= 1 ? (int)run : 0;
+  uint64_t sufT[MAXD + 2];
+  sufT[MAXD] = 0;
+  sufT[MAXD + 1] = 0;
+#pragma unroll
+  for (int i = MAXD - 1; i >= 0; --i)
+    sufT[i] = sufT[i + 1] + ((i < d) ? (SLOS_C(K - i, lov[i]) - SLOS_C(K - i, q[i])) : 0);
+  uint64_t preA = 0;
+  int ns = 0;
+  const uint64_t CS_lo = SLOS_C(kSuf, lo);
+#pragma unroll
+  for (int l = 0; l < MAXD; ++l) {
+    if (l < d) {
+      const bool first = !t.multiset || l == 0 || q[l] != q[l - 1];
+      if (first) {
+        uint64_t pr;
+        int lop;
+        if (l == d - 1) {
+          pr = preA;
+          lop = lov[d - 1];
+        } else {
+          pr = preA + (SLOS_C(K - 1 - l, lov[l]) - SLOS_C(K - 1 - l, q[l + 1])) + sufT[l + 2];
+          lop = lo;
+        }
+        st.base[ns] = pr + (SLOS_C(kSuf, lop) - CS_lo);
+        st.mode[ns] = q[l];
+        ++ns;
+      }
+      preA += SLOS_C(K - 1 - l, lov[l]) - SLOS_C(K - 1 - l, q[l]);
+    }
+  }
+  st.count = ns;
+#undef SLOS_C
+}
+
+// lex-next prefix Q (length d) whose block in layer K is non-empty; false after the last one
+SLOS_HD bool next_prefix(const BlockTables& t, int K, int d, int* q) {
+  const int W = t.stride;
+  for (int i = d - 1; i >= 0; --i) {
+    for (int x = q[i] + 1; x < t.me; ++x) {
+      if (t.cg[(K - 1 - i) * W + t.lo[x]] == 0) continue;
+      q[i] = x;
+      int l = t.lo[x];
+      for (int j = i + 1; j < d; ++j) {
+        int y = l;
+        while (t.cg[(K - 1 - j) * W + t.lo[y]] == 0) ++y;
+        q[j] = y;
+        l = t.lo[y];
+      }
+      return true;
+    }
+  }
+  return false;
+}
+
+// XCD-aware workgroup order. The dispatcher hands consecutive workgroups to the 8 XCDs in turn
+// (wg = 8 slot + xcd); this bijection of [0, nwg) gives each XCD one contiguous share of the work
+// (q + 1 workgroups for the first r XCDs, q for the rest, nwg = 8 q + r), so neighbouring child
+// ranges, which read overlapping parent runs, share that XCD's L2.
+SLOS_HD uint32_t xcd_remap(uint32_t wg, uint32_t nwg) {
+  const uint32_t q = nwg >> 3, r = nwg & 7u;
+  const uint32_t xcd = wg & 7u, slot = wg >> 3;
+  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// prefix Q of the child with lex index g in layer K; returns g's position u inside Q's block
+template <int MAXD>
+SLOS_HD uint64_t unrank_prefix(const BlockTables& t, int K, int d, uint64_t g, int* q) {
+  const int W = t.stride;
+  int l = 0;
+#pragma unroll
+  for (int i = 0; i < MAXD; ++i)
+    if (i < d) {
+      const int r = K - i;
+      const uint64_t base = t.cg[r * W + l];
+      int x = l;
+      while (base - t.cg[r * W + x + 1] <= g) ++x;
+      g -= base - t.cg[r * W + x];
+      q[i] = x;
+      l = t.lo[x];
+    }
+  return g;
+}
+
+// the table entry of one SUF-tuple (host side, at plan creation)
+inline TupleEntry make_tuple_entry(const SpaceView& sv, int multiset, uint64_t u) {
+  int x[kSuf], r[kSuf - 1];
+  sv.unrank(u, kSuf, x);
+  uint32_t valid = 1u;
+  if (!multiset || x[1] != x[0]) valid |= 2u;
+  if (!multiset || x[2] != x[1]) valid |= 4u;
+  const int lead = multiset ? 1 + (x[1] == x[0]) + (x[1] == x[0] && x[2] == x[0]) : 1;
+  int fct = 1;
+  if (multiset) {
+    if (x[0] == x[1] && x[1] == x[2]) fct = 6;
+    else if (x[0] == x[1] || x[1] == x[2]) fct = 2;
+  }
+  const uint32_t fcode = fct == 1 ? 0u : (fct == 2 ? 1u : 2u);
+  TupleEntry e;
+  e.w0 = (uint32_t)x[0] | ((uint32_t)x[1] << 8) | ((uint32_t)x[2] << 16) | (valid << 24) | ((uint32_t)lead << 27) |
+         (fcode << 29);
+  uint32_t lr[kSuf];
+  for (int j = 0; j < kSuf; ++j) {
+    int c = 0;
+    for (int i = 0; i < kSuf; ++i)
+      if (i != j) r[c++] = x[i];
+    lr[j] = (uint32_t)sv.rank(r, kSuf - 1);
+  }
+  e.lr0 = lr[0];
+  e.lr1 = lr[1];
+  e.lr2 = lr[2];
+  return e;
+}
+
+// prod t_j! of the child (Q, tau) from the block's run data and the tuple entry
+SLOS_HD float child_norm(const BlockInfo& bi, uint32_t w0, int multiset) {
+  if (!multiset) return 1.f;
+  const uint32_t fcode = w0 >> 29;
+  float f = bi.fQ * (fcode == 0 ? 1.f : (fcode == 1 ? 2.f : 6.f));
+  const int x0 = (int)(w0 & 0xff);
+  if (bi.skip0 && x0 == bi.lo) {
+    // tau's leading run continues the last run of Q: (r + c)! = r! c! binom(r + c, c)
+    const int lead = (int)((w0 >> 27) & 3u);
+    const float r = (float)bi.lastrun;
+    float b = r + 1.f;
+    if (lead >= 2) b = b * (r + 2.f) * 0.5f;
+    if (lead >= 3) b = b * (r + 3.f) * (1.f / 3.f);
+    f *= b;
+  }
+  return f;
+}
+
+}  // namespace slos

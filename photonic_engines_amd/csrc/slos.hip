@@ -1,0 +1,834 @@
+// SLOS strong simulation on MI355X (gfx950): plan, C ABI and kernels.
+//
+// Forward path (replaces FockSLOSKernel._amplitudes / probs, photonic_engines/slos/kernel.py:201-283):
+// one launch per photon layer, pull form — each child state sums its parents, whose lex ranks
+// are computed from the child's own sorted positions with the counting table (space.hpp), so no
+// state or edge table exists anywhere. Every child is written exactly once; there are no atomics.
+// The last layer's launch squares, normalises and writes float32 probabilities (or scaled
+// complex64 amplitudes) directly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/slos_mi355x.h"
+#include "space.hpp"
+#include "layer_block.hpp"
+#include "pull_index.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return fail(SLOS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kMaxPhotons = 32;
+
+}  // namespace
+
+struct DevTables {
+  int device = -1;
+  uint64_t* cg = nullptr;
+  int* lo = nullptr;
+  int* modes = nullptr;                    // the plan's input modes (spot permanents)
+  slos::TupleEntry* tuples = nullptr;      // block-kernel suffix table (may be absent)
+};
+
+struct slos_plan {
+  int n = 0, m = 0, mask = 0, me = 0, multiset = 1, shift = 0;
+  std::vector<int> modes;
+  std::vector<uint64_t> cg;  // (n + 2) x (me + 1)
+  std::vector<int> lo;
+  double s_fact = 1.0;       // prod_i s_i!
+  std::vector<slos::TupleEntry> tuples;  // suffix-tuple table of the block kernel (empty: unused)
+  std::vector<DevTables> dev;
+  std::mutex mu;
+  slos::SpaceView host_view() const { return slos::SpaceView{cg.data(), lo.data(), me, me + 1}; }
+  uint64_t size(int k) const { return cg[(size_t)k * (me + 1)]; }
+};
+
+namespace {
+
+int device_tables(const slos_plan* cp, DevTables* out) {
+  slos_plan* p = const_cast<slos_plan*>(cp);
+  int d = 0;
+  HIP_TRY(hipGetDevice(&d));
+  std::lock_guard<std::mutex> g(p->mu);
+  for (auto& t : p->dev)
+    if (t.device == d) {
+      *out = t;
+      return SLOS_OK;
+    }
+  DevTables t;
+  t.device = d;
+  HIP_TRY(hipMalloc(&t.cg, p->cg.size() * sizeof(uint64_t)));
+  HIP_TRY(hipMalloc(&t.lo, std::max<size_t>(1, p->lo.size()) * sizeof(int)));
+  HIP_TRY(hipMemcpy(t.cg, p->cg.data(), p->cg.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t.lo, p->lo.data(), p->lo.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&t.modes, p->modes.size() * sizeof(int)));
+  HIP_TRY(hipMemcpy(t.modes, p->modes.data(), p->modes.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!p->tuples.empty()) {
+    HIP_TRY(hipMalloc(&t.tuples, p->tuples.size() * sizeof(slos::TupleEntry)));
+    HIP_TRY(hipMemcpy(t.tuples, p->tuples.data(), p->tuples.size() * sizeof(slos::TupleEntry),
+                      hipMemcpyHostToDevice));
+  }
+  p->dev.push_back(t);
+  *out = t;
+  return SLOS_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// thread-per-state layer kernel: unrank, parent ranks by prefix/suffix sums, gather-MAC
+// ------------------------------------------------------------------------------------------
+
+struct Acc {
+  float re = 0.f, im = 0.f, cre = 0.f, cim = 0.f;
+};
+
+__device__ __forceinline__ void two_sum(float& s, float& c, float x) {
+  const float t = s + x;
+  const float bp = t - s;
+  c += (s - (t - bp)) + (x - bp);
+  s = t;
+}
+
+// acc += u * v, plain or compensated (product errors by fma, sums by two-sum)
+template <bool COMP>
+__device__ __forceinline__ void cmac(Acc& a, float2 u, float2 v) {
+  if constexpr (!COMP) {
+    a.re = fmaf(u.x, v.x, a.re);
+    a.re = fmaf(-u.y, v.y, a.re);
+    a.im = fmaf(u.x, v.y, a.im);
+    a.im = fmaf(u.y, v.x, a.im);
+  } else {
+    float p, e;
+    p = u.x * v.x; e = fmaf(u.x, v.x, -p); two_sum(a.re, a.cre, p); a.cre += e;
+    p = -u.y * v.y; e = fmaf(-u.y, v.y, -p); two_sum(a.re, a.cre, p); a.cre += e;
+    p = u.x * v.y; e = fmaf(u.x, v.y, -p); two_sum(a.im, a.cim, p); a.cim += e;
+    p = u.y * v.x; e = fmaf(u.y, v.x, -p); two_sum(a.im, a.cim, p); a.cim += e;
+  }
+}
+
+struct LayerArgs {
+  const uint64_t* cg;
+  const int* lo;
+  int me, multiset, K, m, col;
+  const float2* U;        // [B][m][m]
+  const int32_t* cols;    // unused by the simple kernel
+  const float2* par;      // [B][S_par]
+  uint64_t S_par;
+  void* out;              // [B][S]
+  uint64_t S;
+  int64_t B;
+  float scale;
+  unsigned flags;
+};
+
+// KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes
+template <int KIND, bool COMP>
+__global__ __launch_bounds__(256) void k_layer_simple(LayerArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int stride = a.me + 1;
+  const int ncg = (a.K + 1) * stride;
+  uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
+  int* lo = reinterpret_cast<int*>(cg + ncg);
+  float2* ucol = reinterpret_cast<float2*>(smem + ((ncg * 8 + a.me * 4 + 15) & ~15));
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = a.cg[i];
+  for (int i = threadIdx.x; i < a.me; i += blockDim.x) lo[i] = a.lo[i];
+  const slos::SpaceView sv{cg, lo, a.me, stride};
+  const int K = a.K;
+  for (int64_t b = blockIdx.y; b < a.B; b += gridDim.y) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < a.m; j += blockDim.x)
+      ucol[j] = a.U[(size_t)b * a.m * a.m + (size_t)j * a.m + a.col];
+    __syncthreads();
+    const float2* pb = a.par + (size_t)b * a.S_par;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.S;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+      Acc acc;
+      slos::for_each_parent(sv, a.multiset, K, g,
+                            [&](uint64_t pr, int mode, int) { cmac<COMP>(acc, ucol[mode], pb[pr]); });
+      float re = acc.re + acc.cre, im = acc.im + acc.cim;
+      if constexpr (KIND == 0) {
+        reinterpret_cast<float2*>(a.out)[(size_t)b * a.S + g] = make_float2(re, im);
+      } else {
+        int p[kMaxPhotons];
+        sv.unrank(g, K, p);
+        const float f = slos::run_factorial(p, K, a.multiset);
+        re *= a.scale;
+        im *= a.scale;
+        if constexpr (KIND == 1) {
+          reinterpret_cast<float*>(a.out)[(size_t)b * a.S + g] = (re * re + im * im) * f;
+        } else {
+          const float s = (a.flags & SLOS_FLAG_IMMEDIATE) ? sqrtf(f) : 1.f;
+          reinterpret_cast<float2*>(a.out)[(size_t)b * a.S + g] = make_float2(re * s, im * s);
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_fill_ones(float2* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_float2(1.f, 0.f);
+}
+
+// ------------------------------------------------------------------------------------------
+// FP64 mass: per-block partial sums, then one block per batch row
+// ------------------------------------------------------------------------------------------
+
+constexpr int kMassBlocks = 1024;
+
+__device__ double block_sum(double v) {
+  __shared__ double sh[16];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_mass_partial(const float* p, uint64_t count, double* part) {
+  const int64_t b = blockIdx.y;
+  const float* row = p + (size_t)b * count;
+  double s = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    s += (double)row[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[(size_t)b * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_mass_final(const double* part, int nb, double* out) {
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[(size_t)blockIdx.x * nb + i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP64 Ryser permanents for spot checks (one thread per output state)
+// ------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void k_spot_ryser(const uint64_t* cg_g, const int* lo_g, int me, int n, int m,
+                                                   const double2* U, const int* modes, double s_fact,
+                                                   int multiset, const uint64_t* idx, int64_t count,
+                                                   double* out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  const slos::SpaceView sv{cg_g, lo_g, me, me + 1};
+  int p[kMaxPhotons];
+  sv.unrank(idx[q], n, p);
+  double tf = 1.0, run = 1.0;
+  if (multiset)
+    for (int i = 1; i < n; ++i) {
+      run = (p[i] == p[i - 1]) ? run + 1.0 : 1.0;
+      tf *= run;
+    }
+  double rre[kMaxPhotons], rim[kMaxPhotons];
+  for (int i = 0; i < n; ++i) rre[i] = rim[i] = 0.0;
+  double tre = 0.0, tim = 0.0;
+  uint64_t prev = 0;
+  for (uint64_t k = 1; k < (1ull << n); ++k) {
+    const uint64_t gray = k ^ (k >> 1);
+    const uint64_t diff = gray ^ prev;
+    prev = gray;
+    const int c = __builtin_ctzll(diff);
+    const double sg = (gray & diff) ? 1.0 : -1.0;
+    double pre = 1.0, pim = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double2 u = U[(size_t)p[i] * m + modes[c]];
+      rre[i] += sg * u.x;
+      rim[i] += sg * u.y;
+      const double nr = pre * rre[i] - pim * rim[i];
+      pim = pre * rim[i] + pim * rre[i];
+      pre = nr;
+    }
+    const int pc = __builtin_popcountll(gray);
+    if ((n - pc) & 1) { tre -= pre; tim -= pim; } else { tre += pre; tim += pim; }
+  }
+  out[q] = (tre * tre + tim * tim) / (s_fact * tf);
+}
+
+
+// ------------------------------------------------------------------------------------------
+// uniform loss: q_{k-1}(t) = sum_j q_k(t + e_j) (t_j + 1) / k  (pull over children, no atomics)
+// ------------------------------------------------------------------------------------------
+// (child ranks: slos::for_each_child in pull_index.hpp)
+__global__ __launch_bounds__(256) void k_descend(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k,
+                                                 const float* __restrict__ qk, uint64_t Sk,
+                                                 float* __restrict__ qout, uint64_t Sp, int64_t B) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int stride = me + 1;
+  const int ncg = (k + 1) * stride;
+  uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
+  int* lo = reinterpret_cast<int*>(cg + ncg);
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = cg_g[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo[i] = lo_g[i];
+  __syncthreads();
+  const slos::SpaceView sv{cg, lo, me, stride};
+  const float invk = 1.f / (float)k;
+  for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
+    const float* qb = qk + (size_t)b * Sk;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < Sp; t += (uint64_t)gridDim.x * blockDim.x) {
+      float acc = 0.f;
+      slos::for_each_child(sv, multiset, k, t, [&](uint64_t c, int, int cnt) { acc += qb[c] * (float)cnt; });
+      qout[(size_t)b * Sp + t] = acc * invk;
+    }
+  }
+}
+
+// adjoint of the descent (gradient of q_{k-1} w.r.t. q_k): g_k[c] = sum_{j distinct in c}
+// g_{k-1}[c - e_j] * count_j(c) / k  — the forward pull structure with real weights
+__global__ __launch_bounds__(256) void k_ascend(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k,
+                                                const float* __restrict__ gp, uint64_t Sp,
+                                                float* __restrict__ gk, uint64_t Sk, int64_t B) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int stride = me + 1;
+  const int ncg = (k + 1) * stride;
+  uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
+  int* lo = reinterpret_cast<int*>(cg + ncg);
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = cg_g[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo[i] = lo_g[i];
+  __syncthreads();
+  const slos::SpaceView sv{cg, lo, me, stride};
+  const int K = k;
+  const float invk = 1.f / (float)k;
+  for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
+    const float* pb = gp + (size_t)b * Sp;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < Sk; g += (uint64_t)gridDim.x * blockDim.x) {
+      float acc = 0.f;
+      slos::for_each_parent(sv, multiset, K, g, [&](uint64_t pr, int, int cnt) { acc += pb[pr] * (float)cnt; });
+      gk[(size_t)b * Sk + g] = acc * invk;
+    }
+  }
+}
+
+
+// ------------------------------------------------------------------------------------------
+// adjoint (reverse-mode) kernels; torch's convention for complex leaves: grad z = dL/dRe z + i dL/dIm z
+// ------------------------------------------------------------------------------------------
+
+// readout: p = |a s|^2 f  =>  g_a = 2 s^2 f grad_p a
+__global__ __launch_bounds__(256) void k_readout_adjoint(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int K,
+                                                         const float2* __restrict__ a, const float* __restrict__ gp,
+                                                         float2* __restrict__ g, uint64_t S, int64_t B, float s2) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int stride = me + 1;
+  const int ncg = (K + 1) * stride;
+  uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
+  int* lo = reinterpret_cast<int*>(cg + ncg);
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = cg_g[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo[i] = lo_g[i];
+  __syncthreads();
+  const slos::SpaceView sv{cg, lo, me, stride};
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < S; c += (uint64_t)gridDim.x * blockDim.x) {
+    int p[kMaxPhotons];
+    sv.unrank(c, K, p);
+    const float f = slos::run_factorial(p, K, multiset);
+    for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
+      const float2 v = a[(size_t)b * S + c];
+      const float w = 2.f * s2 * f * gp[(size_t)b * S + c];
+      g[(size_t)b * S + c] = make_float2(w * v.x, w * v.y);
+    }
+  }
+}
+
+// g_{k-1}[t] = sum_{children c = t + e_j} conj(U[b, j, col]) g_k[c]   (pull over children)
+// and, per thread, gU_j += conj(psi_{k-1}[t]) g_k[t + e_j] in a private LDS row (no atomics);
+// each block then reduces its rows to partial[b][block][j] (FP64 stage in k_reduce_gu)
+constexpr int kAdjThreads = 64;
+__global__ __launch_bounds__(kAdjThreads) void k_adjoint_layer(const uint64_t* cg_g, const int* lo_g, int me, int multiset,
+                                                             int k, int m, int col, const float2* __restrict__ U,
+                                                             const float2* __restrict__ psi, const float2* __restrict__ gk,
+                                                             uint64_t Sk, float2* __restrict__ gout, uint64_t Sp,
+                                                             double2* __restrict__ partial, int want_psi_grad) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int stride = me + 1;
+  const int ncg = (k + 1) * stride;
+  uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
+  int* lo = reinterpret_cast<int*>(cg + ncg);
+  float2* ucol = reinterpret_cast<float2*>(smem + ((ncg * 8 + me * 4 + 15) & ~15));
+  float2* slots = ucol + m;  // [kAdjThreads][me]
+  const int64_t b = blockIdx.y;
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = cg_g[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo[i] = lo_g[i];
+  for (int j = threadIdx.x; j < m; j += blockDim.x) ucol[j] = U[(size_t)b * m * m + (size_t)j * m + col];
+  float2* my = slots + (size_t)threadIdx.x * me;
+  for (int j = 0; j < me; ++j) my[j] = make_float2(0.f, 0.f);
+  __syncthreads();
+  const slos::SpaceView sv{cg, lo, me, stride};
+  const float2* gb = gk + (size_t)b * Sk;
+  const float2* pb = psi + (size_t)b * Sp;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < Sp; t += (uint64_t)gridDim.x * blockDim.x) {
+    const float2 ps = pb[t];
+    float gr = 0.f, gi = 0.f;
+    slos::for_each_child(sv, multiset, k, t, [&](uint64_t c, int j, int) {
+      const float2 gv = gb[c];
+      const float2 u = ucol[j];
+      gr += u.x * gv.x + u.y * gv.y;   // conj(u) * gv
+      gi += u.x * gv.y - u.y * gv.x;
+      my[j].x += ps.x * gv.x + ps.y * gv.y;   // conj(psi) * gv
+      my[j].y += ps.x * gv.y - ps.y * gv.x;
+    });
+    if (want_psi_grad) gout[(size_t)b * Sp + t] = make_float2(gr, gi);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < me; j += blockDim.x) {
+    double sr = 0, si = 0;
+    for (int r = 0; r < kAdjThreads; ++r) {
+      sr += slots[(size_t)r * me + j].x;
+      si += slots[(size_t)r * me + j].y;
+    }
+    partial[((size_t)b * gridDim.x + blockIdx.x) * me + j] = make_double2(sr, si);
+  }
+}
+
+// gU[b][j][col] += sum over blocks of partial[b][block][j]
+__global__ __launch_bounds__(64) void k_reduce_gu(const double2* __restrict__ partial, int nblk, int me, int m, int col,
+                                                  float2* __restrict__ gU, int accumulate) {
+  const int64_t b = blockIdx.y;
+  const int j = blockIdx.x;
+  double sr = 0, si = 0;
+  for (int r = threadIdx.x; r < nblk; r += blockDim.x) {
+    const double2 v = partial[((size_t)b * nblk + r) * me + j];
+    sr += v.x;
+    si += v.y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sr += __shfl_down(sr, o, 64);
+    si += __shfl_down(si, o, 64);
+  }
+  if (threadIdx.x == 0) {
+    float2* dst = gU + (size_t)b * m * m + (size_t)j * m + col;
+    if (accumulate) *dst = make_float2(dst->x + (float)sr, dst->y + (float)si);
+    else *dst = make_float2((float)sr, (float)si);
+  }
+}
+
+template <int KIND, bool COMP>
+void launch_block_kind(int maxd, dim3 grid, hipStream_t st, const slos::BlockTables& t, const slos::BlockLayer& L) {
+  if (maxd <= 8) hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, 8>), grid, dim3(slos::kBlockThreads), 0, st, t, L);
+  else if (maxd <= 16) hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, 16>), grid, dim3(slos::kBlockThreads), 0, st, t, L);
+  else hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, 32>), grid, dim3(slos::kBlockThreads), 0, st, t, L);
+}
+
+int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, int kind, bool comp, hipStream_t st) {
+  slos::BlockTables t;
+  t.cg = dt.cg; t.lo = dt.lo; t.tuples = dt.tuples; t.me = p->me; t.stride = p->me + 1; t.multiset = p->multiset;
+  slos::BlockLayer L;
+  L.K = a.K; L.m = a.m; L.col = a.col; L.U = a.U; L.par = a.par; L.S_par = a.S_par; L.out = a.out; L.S = a.S;
+  L.scale = a.scale; L.flags = a.flags;
+  // children per wave: enough waves to fill the chip several times, but long ranges at scale
+  uint64_t pw = (a.S / 16384 + 63) & ~63ull;
+  L.per_wave = std::min<uint64_t>(std::max<uint64_t>(pw, 256), 4096);
+  const uint64_t waves = (a.S + L.per_wave - 1) / L.per_wave;
+  const uint64_t wgs = (waves + 3) / 4;
+  if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
+  if (a.B > 65535) return fail(SLOS_ERANGE, "batch > 65535");
+  dim3 grid((unsigned)wgs, (unsigned)a.B);
+  const int maxd = a.K - slos::kSuf;
+  if (kind == 0) comp ? launch_block_kind<0, true>(maxd, grid, st, t, L) : launch_block_kind<0, false>(maxd, grid, st, t, L);
+  else if (kind == 1) comp ? launch_block_kind<1, true>(maxd, grid, st, t, L) : launch_block_kind<1, false>(maxd, grid, st, t, L);
+  else comp ? launch_block_kind<2, true>(maxd, grid, st, t, L) : launch_block_kind<2, false>(maxd, grid, st, t, L);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int launch_layer(const slos_plan* p, const DevTables& dt, int K, int col, const float2* U, int64_t B,
+                 const float2* par, uint64_t S_par, void* out, uint64_t S, int kind, float scale,
+                 unsigned flags, hipStream_t st) {
+  LayerArgs a;
+  a.cg = dt.cg; a.lo = dt.lo; a.me = p->me; a.multiset = p->multiset; a.K = K; a.m = p->m; a.col = col;
+  a.U = U; a.cols = nullptr; a.par = par; a.S_par = S_par; a.out = out; a.S = S; a.B = B;
+  a.scale = scale; a.flags = flags;
+  const bool comp = flags & SLOS_FLAG_COMPENSATED;
+  if (!(flags & SLOS_FLAG_SIMPLE) && dt.tuples && K > slos::kSuf)
+    return launch_block(p, dt, a, kind, comp, st);
+  const size_t sh = (((size_t)(K + 1) * (p->me + 1) * 8 + p->me * 4 + 15) & ~(size_t)15) + (size_t)p->m * 8;
+  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  const uint64_t blocks = std::min<uint64_t>((S + 255) / 256, 1u << 20);
+  dim3 grid((unsigned)std::max<uint64_t>(blocks, 1), (unsigned)std::min<int64_t>(B, 65535));
+  if (kind == 0 && !comp) hipLaunchKernelGGL((k_layer_simple<0, false>), grid, dim3(256), sh, st, a);
+  else if (kind == 0) hipLaunchKernelGGL((k_layer_simple<0, true>), grid, dim3(256), sh, st, a);
+  else if (kind == 1 && !comp) hipLaunchKernelGGL((k_layer_simple<1, false>), grid, dim3(256), sh, st, a);
+  else if (kind == 1) hipLaunchKernelGGL((k_layer_simple<1, true>), grid, dim3(256), sh, st, a);
+  else if (!comp) hipLaunchKernelGGL((k_layer_simple<2, false>), grid, dim3(256), sh, st, a);
+  else hipLaunchKernelGGL((k_layer_simple<2, true>), grid, dim3(256), sh, st, a);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+bool mul_overflow(uint64_t a, uint64_t b, uint64_t* r) { return __builtin_mul_overflow(a, b, r); }
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct WsLayout {
+  size_t ones = 0, odd = 0, even = 0, total = 0;
+};
+
+int forward_layout(const slos_plan* p, int64_t B, int layer, WsLayout* w) {
+  uint64_t maxodd = 0, maxeven = 0;
+  for (int k = 1; k < layer; ++k) (k & 1 ? maxodd : maxeven) = std::max(k & 1 ? maxodd : maxeven, p->size(k));
+  uint64_t a = 0, b = 0;
+  if (mul_overflow(maxodd, (uint64_t)B * 8, &a) || mul_overflow(maxeven, (uint64_t)B * 8, &b))
+    return fail(SLOS_ERANGE, "workspace size overflows");
+  w->ones = 0;
+  w->odd = align256((size_t)B * 8);
+  w->even = w->odd + align256(a);
+  w->total = w->even + align256(b);
+  return SLOS_OK;
+}
+
+}  // namespace
+
+void build_tuples(slos_plan* p) {
+  const auto sv = p->host_view();
+  const uint64_t N = p->size(slos::kSuf);
+  p->tuples.resize(N);
+  for (uint64_t u = 0; u < N; ++u) p->tuples[u] = slos::make_tuple_entry(sv, p->multiset, u);
+}
+
+extern "C" {
+
+const char* slos_version(void) { return "slos_mi355x 0.1 (gfx950)"; }
+
+const char* slos_last_error(void) { return g_err.c_str(); }
+
+int slos_plan_create(int n, int m, int mask, const int32_t* input_modes, slos_plan** out) {
+  if (!out) return fail(SLOS_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (n < 1) return fail(SLOS_EINVAL, "n must be >= 1");
+  if (n > kMaxPhotons) return fail(SLOS_ERANGE, "at most 32 photons");
+  if (m < 1 || m > 4096) return fail(SLOS_EINVAL, "m must be in [1, 4096]");
+  if (!input_modes) return fail(SLOS_EINVAL, "input_modes is NULL");
+  auto* p = new slos_plan();
+  p->n = n; p->m = m; p->mask = mask;
+  switch (mask) {
+    case SLOS_MASK_FOCK: p->me = m; p->multiset = 1; break;
+    case SLOS_MASK_UNBUNCHED: p->me = m; p->multiset = 0; p->shift = 0; break;
+    case SLOS_MASK_DUAL_RAIL: p->me = m; p->multiset = 0; p->shift = 1; break;
+    case SLOS_MASK_SPARSE_1IN4: p->me = m; p->multiset = 0; p->shift = 2; break;
+    case SLOS_MASK_TAIL_EMPTY: p->me = m - 2; p->multiset = 1; break;
+    default: delete p; return fail(SLOS_EINVAL, "unknown mask id " + std::to_string(mask));
+  }
+  if (p->me < 1) { delete p; return fail(SLOS_EINVAL, "empty output space (too few modes for the mask)"); }
+  std::vector<int> cnt(m, 0);
+  for (int k = 0; k < n; ++k) {
+    if (input_modes[k] < 0 || input_modes[k] >= m) { delete p; return fail(SLOS_EINVAL, "input mode out of range"); }
+    p->modes.push_back(input_modes[k]);
+    cnt[input_modes[k]]++;
+  }
+  for (int c : cnt) for (int i = 2; i <= c; ++i) p->s_fact *= i;
+  p->lo.resize(p->me);
+  for (int x = 0; x < p->me; ++x) {
+    const int v = p->multiset ? x : (((x >> p->shift) + 1) << p->shift);
+    p->lo[x] = std::min(v, p->me);
+  }
+  const int st = p->me + 1;
+  p->cg.assign((size_t)(n + 2) * st, 0);
+  for (int x = 0; x <= p->me; ++x) p->cg[x] = 1;
+  for (int r = 1; r <= n + 1; ++r)
+    for (int x = p->me - 1; x >= 0; --x) {
+      uint64_t v;
+      if (__builtin_add_overflow(p->cg[(size_t)r * st + x + 1], p->cg[(size_t)(r - 1) * st + p->lo[x]], &v)) {
+        if (r <= n) { delete p; return fail(SLOS_ERANGE, "layer size exceeds 64 bits"); }
+        v = UINT64_MAX;  // row n+1 is only a guard
+      }
+      p->cg[(size_t)r * st + x] = v;
+    }
+  if (n > slos::kSuf && p->m <= 256 && p->size(slos::kSuf) <= (4u << 20)) build_tuples(p);
+  if (p->size(n) == 0) {
+    delete p;
+    return fail(SLOS_EINVAL, "empty output space for n=" + std::to_string(n) + " m=" + std::to_string(m));
+  }
+  *out = p;
+  return SLOS_OK;
+}
+
+int slos_plan_destroy(slos_plan* p) {
+  if (!p) return SLOS_OK;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto& t : p->dev) {
+    (void)hipSetDevice(t.device);
+    (void)hipFree(t.cg);
+    (void)hipFree(t.lo);
+    (void)hipFree(t.modes);
+    if (t.tuples) (void)hipFree(t.tuples);
+  }
+  (void)hipSetDevice(cur);
+  delete p;
+  return SLOS_OK;
+}
+
+int slos_layer_size(const slos_plan* p, int k, uint64_t* size) {
+  if (!p || !size) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 0 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  *size = p->size(k);
+  return SLOS_OK;
+}
+
+int slos_rank(const slos_plan* p, int k, const int32_t* pos, int64_t count, uint64_t* idx) {
+  if (!p || (!pos && count) || (!idx && count)) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 0 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  const auto sv = p->host_view();
+  for (int64_t q = 0; q < count; ++q) {
+    const int32_t* t = pos + q * k;
+    int l = 0, run = 0;
+    for (int i = 0; i < k; ++i) {
+      if (t[i] < l || t[i] >= p->me) return fail(SLOS_EINVAL, "positions do not describe an admissible sorted state");
+      run = (i > 0 && t[i] == t[i - 1]) ? run + 1 : 1;
+      if (sv.C(k - i - 1, p->lo[t[i]]) == 0) return fail(SLOS_EINVAL, "state has no admissible completion");
+      l = p->lo[t[i]];
+    }
+    (void)run;
+    idx[q] = sv.rank(reinterpret_cast<const int*>(t), k);
+  }
+  return SLOS_OK;
+}
+
+int slos_unrank(const slos_plan* p, int k, const uint64_t* idx, int64_t count, int32_t* pos) {
+  if (!p || (!pos && count) || (!idx && count)) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 0 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  const auto sv = p->host_view();
+  const uint64_t S = p->size(k);
+  for (int64_t q = 0; q < count; ++q) {
+    if (idx[q] >= S) return fail(SLOS_ERANGE, "index >= layer size");
+    sv.unrank(idx[q], k, reinterpret_cast<int*>(pos + q * k));
+  }
+  return SLOS_OK;
+}
+
+int slos_prob_norms(const slos_plan* p, int k, uint64_t first, int64_t count, double* out) {
+  if (!p || (!out && count)) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 0 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  const uint64_t S = p->size(k);
+  if (count < 0 || first > S || (uint64_t)count > S - first) return fail(SLOS_ERANGE, "range outside the layer");
+  const auto sv = p->host_view();
+  int t[kMaxPhotons];
+  for (int64_t q = 0; q < count; ++q) {
+    sv.unrank(first + q, k, t);
+    double f = 1, run = 1;
+    if (p->multiset)
+      for (int i = 1; i < k; ++i) {
+        run = (t[i] == t[i - 1]) ? run + 1 : 1;
+        f *= run;
+      }
+    out[q] = f;
+  }
+  return SLOS_OK;
+}
+
+int slos_forward_workspace(const slos_plan* p, int64_t batch, int layer, size_t* bytes) {
+  if (!p || !bytes) return fail(SLOS_EINVAL, "NULL argument");
+  if (batch < 1) return fail(SLOS_EINVAL, "batch must be >= 1");
+  if (layer < 1 || layer > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  WsLayout w;
+  int rc = forward_layout(p, batch, layer, &w);
+  if (rc) return rc;
+  *bytes = w.total;
+  return SLOS_OK;
+}
+
+int slos_forward(const slos_plan* p, const void* U, int64_t batch, const int32_t* modes, int layer, int out_kind,
+                 void* out, double scale, unsigned flags, void* ws, size_t ws_bytes, slos_stream_t stream) {
+  if (!p || !U || !out) return fail(SLOS_EINVAL, "NULL argument");
+  if (batch < 1) return fail(SLOS_EINVAL, "batch must be >= 1");
+  if (layer < 1 || layer > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (out_kind != SLOS_OUT_PROBS && out_kind != SLOS_OUT_AMPS) return fail(SLOS_EINVAL, "unknown output kind");
+  std::vector<int> cols(layer);
+  for (int k = 0; k < layer; ++k) {
+    cols[k] = modes ? modes[k] : p->modes[k];
+    if (cols[k] < 0 || cols[k] >= p->m) return fail(SLOS_EINVAL, "input mode out of range");
+  }
+  WsLayout w;
+  int rc = forward_layout(p, batch, layer, &w);
+  if (rc) return rc;
+  if (!ws || ws_bytes < w.total)
+    return fail(SLOS_ENOMEM, "workspace too small: need " + std::to_string(w.total) + " bytes");
+  DevTables dt;
+  rc = device_tables(p, &dt);
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  auto* base = static_cast<unsigned char*>(ws);
+  float2* ones = reinterpret_cast<float2*>(base + w.ones);
+  float2* bufs[2] = {reinterpret_cast<float2*>(base + w.even), reinterpret_cast<float2*>(base + w.odd)};
+  hipLaunchKernelGGL(k_fill_ones, dim3((unsigned)std::min<int64_t>((batch + 255) / 256, 4096)), dim3(256), 0, st,
+                     ones, batch);
+  HIP_TRY(hipGetLastError());
+  const float2* Uf = static_cast<const float2*>(U);
+  const float2* par = ones;
+  uint64_t S_par = 1;
+  for (int k = 1; k <= layer; ++k) {
+    const uint64_t S = p->size(k);
+    const bool last = (k == layer);
+    void* dst = last ? out : bufs[k & 1];
+    const int kind = last ? (out_kind == SLOS_OUT_PROBS ? 1 : 2) : 0;
+    rc = launch_layer(p, dt, k, cols[k - 1], Uf, batch, par, S_par, dst, S, kind, (float)scale, flags, st);
+    if (rc) return rc;
+    par = static_cast<const float2*>(dst);
+    S_par = S;
+  }
+  return SLOS_OK;
+}
+
+int slos_layer_step(const slos_plan* p, const void* U, int64_t batch, int k, int col, const void* psi_prev,
+                    int out_kind, void* out, double scale, unsigned flags, slos_stream_t stream) {
+  if (!p || !U || !psi_prev || !out) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 1 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (col < 0 || col >= p->m) return fail(SLOS_EINVAL, "input mode out of range");
+  if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
+  if (out_kind < 0 || out_kind > 2) return fail(SLOS_EINVAL, "unknown output kind");
+  DevTables dt;
+  int rc = device_tables(p, &dt);
+  if (rc) return rc;
+  const int kind = out_kind == 2 ? 0 : (out_kind == SLOS_OUT_PROBS ? 1 : 2);
+  return launch_layer(p, dt, k, col, static_cast<const float2*>(U), batch, static_cast<const float2*>(psi_prev),
+                      p->size(k - 1), out, p->size(k), kind, (float)scale, flags, reinterpret_cast<hipStream_t>(stream));
+}
+
+int slos_readout_adjoint(const slos_plan* p, int64_t batch, const void* amps, const float* grad_p, double scale,
+                         void* grad_amps, slos_stream_t stream) {
+  if (!p || !amps || !grad_p || !grad_amps) return fail(SLOS_EINVAL, "NULL argument");
+  if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
+  DevTables dt;
+  int rc = device_tables(p, &dt);
+  if (rc) return rc;
+  const uint64_t S = p->size(p->n);
+  const size_t sh = (size_t)(p->n + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
+  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((S + 255) / 256, 1u << 20)),
+            (unsigned)std::min<int64_t>(batch, 16));
+  hipLaunchKernelGGL(k_readout_adjoint, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo,
+                     p->me, p->multiset, p->n, static_cast<const float2*>(amps), grad_p,
+                     static_cast<float2*>(grad_amps), S, batch, (float)(scale * scale));
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+static int adjoint_blocks(uint64_t Sp) { return (int)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 63) / 64, 1024)); }
+
+int slos_adjoint_workspace(const slos_plan* p, int64_t batch, int k, size_t* bytes) {
+  if (!p || !bytes) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 1 || k > p->n || batch < 1) return fail(SLOS_EINVAL, "bad argument");
+  *bytes = (size_t)batch * adjoint_blocks(p->size(k - 1)) * p->me * sizeof(double2);
+  return SLOS_OK;
+}
+
+int slos_adjoint_layer(const slos_plan* p, const void* U, int64_t batch, int k, int col, const void* psi_prev,
+                       const void* grad_k, void* grad_prev, void* grad_U, int accumulate, void* ws, size_t ws_bytes,
+                       slos_stream_t stream) {
+  if (!p || !U || !psi_prev || !grad_k || !grad_U) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 1 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (col < 0 || col >= p->m) return fail(SLOS_EINVAL, "input mode out of range");
+  if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
+  const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
+  const int nblk = adjoint_blocks(Sp);
+  if (!ws || ws_bytes < (size_t)batch * nblk * p->me * sizeof(double2)) return fail(SLOS_ENOMEM, "workspace too small");
+  const size_t sh = (((size_t)(k + 1) * (p->me + 1) * 8 + p->me * 4 + 15) & ~(size_t)15) + (size_t)p->m * 8 +
+                    (size_t)kAdjThreads * p->me * 8;
+  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "adjoint LDS exceeds 64 KB (too many modes)");
+  DevTables dt;
+  int rc = device_tables(p, &dt);
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_adjoint_layer, dim3((unsigned)nblk, (unsigned)batch), dim3(kAdjThreads), sh, st, dt.cg, dt.lo,
+                     p->me, p->multiset, k, p->m, col, static_cast<const float2*>(U), static_cast<const float2*>(psi_prev),
+                     static_cast<const float2*>(grad_k), Sk, static_cast<float2*>(grad_prev), Sp,
+                     static_cast<double2*>(ws), grad_prev != nullptr);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_gu, dim3((unsigned)p->me, (unsigned)batch), dim3(64), 0, st,
+                     static_cast<const double2*>(ws), nblk, p->me, p->m, col, static_cast<float2*>(grad_U), accumulate);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1, int64_t batch, slos_stream_t stream) {
+  if (!p || !q_k || !q_km1) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 1 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
+  DevTables dt;
+  int rc = device_tables(p, &dt);
+  if (rc) return rc;
+  const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
+  const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
+  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 255) / 256, 1u << 20)), (unsigned)batch);
+  hipLaunchKernelGGL(k_descend, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
+                     p->multiset, k, q_k, Sk, q_km1, Sp, batch);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_loss_ascend(const slos_plan* p, int k, const float* g_km1, float* g_k, int64_t batch, slos_stream_t stream) {
+  if (!p || !g_km1 || !g_k) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 1 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
+  DevTables dt;
+  int rc = device_tables(p, &dt);
+  if (rc) return rc;
+  const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
+  const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
+  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sk + 255) / 256, 1u << 20)), (unsigned)batch);
+  hipLaunchKernelGGL(k_ascend, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
+                     p->multiset, k, g_km1, Sp, g_k, Sk, batch);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_mass_workspace(int64_t batch, uint64_t count, size_t* bytes) {
+  (void)count;
+  if (!bytes || batch < 1) return fail(SLOS_EINVAL, "bad argument");
+  *bytes = (size_t)batch * kMassBlocks * sizeof(double);
+  return SLOS_OK;
+}
+
+int slos_total_mass(const float* probs, int64_t batch, uint64_t count, double* out, void* ws, size_t ws_bytes,
+                    slos_stream_t stream) {
+  if (!probs || !out || batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "bad argument");
+  if (!ws || ws_bytes < (size_t)batch * kMassBlocks * sizeof(double)) return fail(SLOS_ENOMEM, "workspace too small");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(ws);
+  hipLaunchKernelGGL(k_mass_partial, dim3(kMassBlocks, (unsigned)batch), dim3(256), 0, st, probs, count, part);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_mass_final, dim3((unsigned)batch), dim3(256), 0, st, part, kMassBlocks, out);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_spot_probs(const slos_plan* p, const void* U, const uint64_t* idx, int64_t count, double* out,
+                    slos_stream_t stream) {
+  if (!p || !U || (!idx && count) || (!out && count)) return fail(SLOS_EINVAL, "NULL argument");
+  if (p->n > 24) return fail(SLOS_ERANGE, "spot permanents support n <= 24");
+  if (count == 0) return SLOS_OK;
+  DevTables dt;
+  int rc = device_tables(p, &dt);
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_spot_ryser, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, st, dt.cg, dt.lo, p->me, p->n,
+                     p->m, static_cast<const double2*>(U), dt.modes, p->s_fact, p->multiset, idx, count, out);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,190 @@
+"""Uniform output loss on the GPU: drop-in for the `loss=` path of the reference
+(photonic_engines/imperfections.py:89-337: `_eta`, `binomial_pmf`, `LossyDistribution`,
+`draw_indices`, `thin_distribution`) and `spec` (imperfections.py:508-526) for the `indist=`
+argument check.
+
+Removing one photon uniformly at random from a k-photon pattern t + e_j lands on t with
+probability (t_j + 1) / k, so
+
+    q_n = p,   q_{k-1}(t) = sum_j q_k(t + e_j) (t_j + 1) / k,   P_loss(layer k) = C(n,k) eta^k (1-eta)^(n-k) q_k.
+
+Each descent step is one pull-form HIP kernel launch (every (k-1)-photon state gathers its m
+children; no atomics, no edge tables); the binomial weights are torch ops so that eta may be a
+tensor with a gradient, and the descent has its own adjoint kernel for gradients through p.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+Eta = float | torch.Tensor
+
+
+def _eta(eta) -> float | torch.Tensor | None:
+    """None / >= 1 -> no loss; a float outside [0, 1] raises (imperfections.py:94-103)."""
+    if eta is None:
+        return None
+    if isinstance(eta, torch.Tensor):
+        return None if float(eta.detach()) >= 1.0 else eta
+    e = float(eta)
+    if not 0.0 <= e <= 1.0:
+        raise ValueError(f"loss (transmission) must be in [0, 1], got {e}")
+    return None if e >= 1.0 else e
+
+
+def binomial_pmf(n: int, eta: Eta, dtype=torch.float64, device=None) -> torch.Tensor:
+    """P(k of n photons survive), k = 0..n; differentiable in a tensor eta (imperfections.py:106-111)."""
+    e = torch.as_tensor(eta, dtype=dtype, device=device)
+    k = torch.arange(n + 1, dtype=dtype, device=e.device)
+    logc = (torch.lgamma(torch.tensor(n + 1.0, dtype=dtype, device=e.device)) - torch.lgamma(k + 1)
+            - torch.lgamma(n - k + 1))
+    return torch.exp(logc) * e ** k * (1 - e) ** (n - k)
+
+
+class Distinguishability:
+    """Minimal x-model marker for the `indist=` argument check (imperfections.py:440-526)."""
+
+    def __init__(self, x: float = 1.0):
+        self.x = float(x)
+
+    @property
+    def is_ideal(self) -> bool:
+        return self.x >= 1.0
+
+
+def spec(indist):
+    """None / 1.0 / an ideal spec -> None (the lossless path); otherwise a non-ideal marker.
+    Overlaps outside [0, 1] raise ValueError (imperfections.py:508-526)."""
+    if indist is None:
+        return None
+    if isinstance(indist, (list, tuple)) or (isinstance(indist, torch.Tensor) and indist.dim() == 1):
+        xs = [float(v) for v in indist]
+        if any(not 0.0 <= v <= 1.0 for v in xs):
+            raise ValueError("per-photon overlaps must be in [0, 1]")
+        return None if all(v >= 1.0 for v in xs) else Distinguishability(min(xs))
+    d = indist if isinstance(indist, Distinguishability) else Distinguishability(
+        float(indist.detach()) if isinstance(indist, torch.Tensor) else float(indist))
+    if not 0.0 <= d.x <= 1.0:
+        raise ValueError("indistinguishability must be in [0, 1]")
+    return None if d.is_ideal else d
+
+
+MULTINOMIAL_MAX = 1 << 24
+
+
+def draw_indices(p: torch.Tensor, shots: int, generator: torch.Generator | None = None) -> torch.Tensor:
+    """`shots` indices from the unnormalised weights p [S], with replacement (imperfections.py:224-237):
+    torch.multinomial up to 2^24 categories, the inverse CDF beyond."""
+    if p.numel() <= MULTINOMIAL_MAX:
+        return torch.multinomial(p, shots, replacement=True, generator=generator)
+    cdf = torch.cumsum(p, dim=0)
+    u = torch.rand(shots, dtype=cdf.dtype, device=cdf.device, generator=generator) * cdf[-1]
+    return torch.searchsorted(cdf, u).clamp_(max=p.numel() - 1)
+
+
+def thin_to_clicks(occupations, probs: torch.Tensor) -> tuple[np.ndarray, torch.Tensor]:
+    """Group a pattern distribution by click pattern (imperfections.py:429-438)."""
+    occ = np.asarray(occupations)
+    cl = (occ > 0).astype(np.int64)
+    keys = (cl * (2 ** np.arange(occ.shape[1], dtype=np.int64))).sum(axis=1)
+    uk, inv = np.unique(keys, return_inverse=True)
+    first = np.zeros(uk.shape[0], dtype=np.int64)
+    first[inv[::-1]] = np.arange(occ.shape[0])[::-1]
+    out = torch.zeros((*probs.shape[:-1], uk.shape[0]), dtype=probs.dtype, device=probs.device)
+    return cl[first], out.index_add(-1, torch.as_tensor(inv, device=probs.device), probs)
+
+
+@dataclass
+class LossyDistribution:
+    """A distribution over 0..n photons: probs[k] is [S_k] (or [B, S_k]) over occupations(k);
+    the layers sum to the binomial weights (imperfections.py:164-218)."""
+    layers: object
+    probs: list
+    _occ: list | None = None
+
+    @property
+    def n(self) -> int:
+        return len(self.probs) - 1
+
+    def occupations(self, k: int) -> np.ndarray:
+        if self._occ is not None:
+            return self._occ[k]
+        return self.layers.occupations(k)
+
+    def mass(self, k: int) -> torch.Tensor:
+        """P(exactly k photons detected): scalar or [B]."""
+        return self.probs[k].sum(dim=-1)
+
+    def conditional(self, k: int) -> torch.Tensor:
+        """The k-photon patterns given k detected (post-selection)."""
+        return self.probs[k] / self.mass(k).unsqueeze(-1).clamp_min(1e-300)
+
+    def flat(self) -> tuple[np.ndarray, torch.Tensor]:
+        occ = np.concatenate([self.occupations(k) for k in range(self.n + 1)], axis=0)
+        return occ, torch.cat(self.probs, dim=-1)
+
+    def as_dict(self, tol: float = 0.0, batch: int | None = None) -> dict:
+        occ, p = self.flat()
+        if batch is not None:
+            p = p[batch]
+        p = p.detach().cpu().to(torch.float64).numpy()
+        return {tuple(int(v) for v in o): float(q) for o, q in zip(occ, p) if q > tol}
+
+    def clicks(self) -> tuple[np.ndarray, torch.Tensor]:
+        occ, p = self.flat()
+        return thin_to_clicks(occ, p)
+
+    def sample(self, shots: int, generator: torch.Generator | None = None) -> torch.Tensor:
+        occ, p = self.flat()
+        if p.dim() != 1:
+            raise ValueError("one distribution at a time")
+        return torch.as_tensor(occ)[draw_indices(p.detach().cpu().to(torch.float64), shots, generator)]
+
+
+class _Descend(torch.autograd.Function):
+    """q_{k-1} from q_k with the descent kernel; backward with its adjoint kernel."""
+
+    @staticmethod
+    def forward(ctx, q: torch.Tensor, plan, k: int) -> torch.Tensor:
+        ctx.plan, ctx.k = plan, k
+        return plan.loss_descend(k, q)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        return ctx.plan.loss_ascend(ctx.k, g.contiguous().float()), None, None
+
+
+def thin_distribution(layers, probs: torch.Tensor, eta: Eta, chunk: int | None = None) -> LossyDistribution:
+    """Uniform output loss on the n-photon distribution `probs` ([S] or [B, S] over
+    layers.final_states), returned as a LossyDistribution over 0..n photons
+    (imperfections.py:309-337). Differentiable in probs and eta. `chunk` is accepted for
+    signature compatibility (the pull kernels need no edge chunks)."""
+    n = layers.n
+    e = _eta(eta)
+    user_dev = probs.device
+    batched = probs.dim() == 2
+    dev = nat.require_gpu(probs.device if probs.device.type == "cuda" else None)
+    q_top = (probs if batched else probs.unsqueeze(0)).to(device=dev, dtype=torch.float32).contiguous()
+    plan = layers._plan
+    q = [None] * (n + 1)
+    q[n] = q_top
+    grad = torch.is_grad_enabled() and q_top.requires_grad
+    for k in range(n, 0, -1):
+        q[k - 1] = _Descend.apply(q[k], plan, k) if grad else plan.loss_descend(k, q[k])
+    if e is None:
+        weights = torch.zeros(n + 1, dtype=torch.float32, device=dev)
+        weights[n] = 1.0
+    else:
+        weights = binomial_pmf(n, e.to(dev) if isinstance(e, torch.Tensor) else e, dtype=torch.float64,
+                               device=dev).to(torch.float32)
+    blocks = []
+    for k in range(n + 1):
+        b = q[k] * weights[k]
+        if not batched:
+            b = b.squeeze(0)
+        blocks.append(b.to(user_dev) if user_dev.type != "cuda" else b)
+    return LossyDistribution(layers, blocks)

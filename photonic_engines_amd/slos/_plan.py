@@ -1,0 +1,159 @@
+"""Python handle of one native plan (slos_plan_* in include/slos_mi355x.h).
+
+A plan fixes (n, m, mask, input modes) and owns only the counting tables; every large buffer
+(workspace, outputs) is a torch tensor from the caching allocator, passed by pointer, and every
+launch goes to torch's current stream on the compute device.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+
+
+class Plan:
+    def __init__(self, n: int, m: int, mask: str, input_modes):
+        if mask not in nat.MASK_IDS:
+            raise ValueError(f"unknown mask {mask!r}")
+        self.n, self.m, self.mask = int(n), int(m), mask
+        self.modes = [int(i) for i in input_modes]
+        if len(self.modes) != self.n:
+            raise ValueError(f"input has {len(self.modes)} photons, expected n={n}")
+        arr = (ctypes.c_int32 * self.n)(*self.modes)
+        h = ctypes.c_void_p()
+        self._lib = nat.lib()
+        nat.check(self._lib.slos_plan_create(self.n, self.m, nat.MASK_IDS[mask], arr, ctypes.byref(h)),
+                  "slos_plan_create")
+        self._h = h
+        self.sizes = [self._layer_size(k) for k in range(self.n + 1)]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.slos_plan_destroy(h)
+            self._h = None
+
+    # ---- host queries ---------------------------------------------------------------------
+    def _layer_size(self, k: int) -> int:
+        out = ctypes.c_uint64()
+        nat.check(self._lib.slos_layer_size(self._h, k, ctypes.byref(out)), "slos_layer_size")
+        return int(out.value)
+
+    def rank(self, k: int, positions) -> np.ndarray:
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.int32).reshape(-1, k) if k else
+                                   np.zeros((len(positions), 0), dtype=np.int32))
+        out = np.zeros(pos.shape[0], dtype=np.uint64)
+        nat.check(self._lib.slos_rank(self._h, k, pos.ctypes.data_as(nat._i32p), pos.shape[0],
+                                      out.ctypes.data_as(nat._u64p)), "slos_rank")
+        return out
+
+    def unrank(self, k: int, idx) -> np.ndarray:
+        idx = np.ascontiguousarray(np.asarray(idx, dtype=np.uint64).reshape(-1))
+        out = np.zeros((idx.shape[0], k), dtype=np.int32)
+        nat.check(self._lib.slos_unrank(self._h, k, idx.ctypes.data_as(nat._u64p), idx.shape[0],
+                                        out.ctypes.data_as(nat._i32p)), "slos_unrank")
+        return out
+
+    def prob_norms(self, k: int, first: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.sizes[k] - first if count is None else count
+        out = np.zeros(count, dtype=np.float64)
+        nat.check(self._lib.slos_prob_norms(self._h, k, first, count, out.ctypes.data_as(nat._f64p)),
+                  "slos_prob_norms")
+        return out
+
+    # ---- device work --------------------------------------------------------------------------
+    def workspace_bytes(self, batch: int, layer: int) -> int:
+        out = ctypes.c_size_t()
+        nat.check(self._lib.slos_forward_workspace(self._h, batch, layer, ctypes.byref(out)),
+                  "slos_forward_workspace")
+        return int(out.value)
+
+    def forward(self, U: torch.Tensor, *, modes=None, layer: int | None = None, out_kind: int = nat.OUT_PROBS,
+                scale: float = 1.0, flags: int = 0, out: torch.Tensor | None = None) -> torch.Tensor:
+        """U: complex64 [B, m, m] on the compute device -> [B, S_layer] float32 / complex64."""
+        layer = self.n if layer is None else int(layer)
+        dev = U.device
+        B = U.shape[0]
+        if U.dtype != torch.complex64 or not U.is_contiguous() or U.shape[1:] != (self.m, self.m):
+            raise ValueError("U must be a contiguous complex64 [B, m, m] tensor")
+        S = self.sizes[layer]
+        if out is None:
+            out = torch.empty((B, S), dtype=torch.float32 if out_kind == nat.OUT_PROBS else torch.complex64,
+                              device=dev)
+        ws = torch.empty(max(self.workspace_bytes(B, layer), 1), dtype=torch.uint8, device=dev)
+        mp = None
+        if modes is not None:
+            mp = (ctypes.c_int32 * layer)(*[int(i) for i in modes])
+        nat.check(self._lib.slos_forward(self._h, nat.ptr(U), B, mp, layer, out_kind, nat.ptr(out),
+                                         float(scale), int(flags), nat.ptr(ws), ws.numel(),
+                                         nat.stream_handle(dev)), "slos_forward")
+        return out
+
+    def layer_step(self, U: torch.Tensor, k: int, col: int, psi_prev: torch.Tensor, out_kind: int = nat.OUT_RAW,
+                   scale: float = 1.0, flags: int = 0) -> torch.Tensor:
+        """Layer k from layer k-1 (input column col): raw complex64, probs or scaled amplitudes."""
+        B = U.shape[0]
+        S = self.sizes[k]
+        out = torch.empty((B, S), dtype=torch.float32 if out_kind == nat.OUT_PROBS else torch.complex64,
+                          device=U.device)
+        nat.check(self._lib.slos_layer_step(self._h, nat.ptr(U), B, k, int(col), nat.ptr(psi_prev), out_kind,
+                                            nat.ptr(out), float(scale), int(flags), nat.stream_handle(U.device)),
+                  "slos_layer_step")
+        return out
+
+    def readout_adjoint(self, amps: torch.Tensor, grad_p: torch.Tensor, scale: float) -> torch.Tensor:
+        g = torch.empty_like(amps)
+        nat.check(self._lib.slos_readout_adjoint(self._h, amps.shape[0], nat.ptr(amps), nat.ptr(grad_p), float(scale),
+                                                 nat.ptr(g), nat.stream_handle(amps.device)), "slos_readout_adjoint")
+        return g
+
+    def adjoint_layer(self, U: torch.Tensor, k: int, col: int, psi_prev: torch.Tensor, grad_k: torch.Tensor,
+                      grad_U: torch.Tensor, want_prev: bool = True) -> torch.Tensor | None:
+        """grad of layer k-1 (if want_prev) and grad_U[:, :, col] += the layer's contribution."""
+        B = U.shape[0]
+        wsb = ctypes.c_size_t()
+        nat.check(self._lib.slos_adjoint_workspace(self._h, B, k, ctypes.byref(wsb)), "slos_adjoint_workspace")
+        ws = torch.empty(max(int(wsb.value), 1), dtype=torch.uint8, device=U.device)
+        gprev = torch.empty((B, self.sizes[k - 1]), dtype=torch.complex64, device=U.device) if want_prev else None
+        nat.check(self._lib.slos_adjoint_layer(self._h, nat.ptr(U), B, k, int(col), nat.ptr(psi_prev), nat.ptr(grad_k),
+                                               nat.ptr(gprev) if want_prev else None, nat.ptr(grad_U), 1,
+                                               nat.ptr(ws), ws.numel(), nat.stream_handle(U.device)),
+                  "slos_adjoint_layer")
+        return gprev
+
+    def loss_descend(self, k: int, q: torch.Tensor) -> torch.Tensor:
+        """q_{k-1} = descent of q_k ([B, S_k] float32 contiguous on the GPU)."""
+        out = torch.empty((q.shape[0], self.sizes[k - 1]), dtype=torch.float32, device=q.device)
+        nat.check(self._lib.slos_loss_descend(self._h, k, nat.ptr(q), nat.ptr(out), q.shape[0],
+                                              nat.stream_handle(q.device)), "slos_loss_descend")
+        return out
+
+    def loss_ascend(self, k: int, g: torch.Tensor) -> torch.Tensor:
+        """Adjoint of loss_descend: [B, S_{k-1}] -> [B, S_k]."""
+        out = torch.empty((g.shape[0], self.sizes[k]), dtype=torch.float32, device=g.device)
+        nat.check(self._lib.slos_loss_ascend(self._h, k, nat.ptr(g), nat.ptr(out), g.shape[0],
+                                             nat.stream_handle(g.device)), "slos_loss_ascend")
+        return out
+
+    def total_mass(self, probs: torch.Tensor) -> torch.Tensor:
+        """FP64 sum over the last axis of float32 probabilities [B, S] -> float64 [B]."""
+        B, S = probs.shape
+        out = torch.empty(B, dtype=torch.float64, device=probs.device)
+        wsb = ctypes.c_size_t()
+        nat.check(self._lib.slos_mass_workspace(B, S, ctypes.byref(wsb)), "slos_mass_workspace")
+        ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=probs.device)
+        nat.check(self._lib.slos_total_mass(nat.ptr(probs), B, S, nat.ptr(out), nat.ptr(ws), ws.numel(),
+                                            nat.stream_handle(probs.device)), "slos_total_mass")
+        return out
+
+    def spot_probs(self, U128: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+        """FP64 Ryser probabilities of the layer-n states with lex indices idx (device uint64/int64)."""
+        U128 = U128.to(torch.complex128).contiguous()
+        idx = idx.to(device=U128.device, dtype=torch.int64).contiguous()
+        out = torch.empty(idx.numel(), dtype=torch.float64, device=U128.device)
+        nat.check(self._lib.slos_spot_probs(self._h, nat.ptr(U128), nat.ptr(idx), idx.numel(), nat.ptr(out),
+                                            nat.stream_handle(U128.device)), "slos_spot_probs")
+        return out

@@ -1,0 +1,201 @@
+"""FockSLOSKernel on MI355X: drop-in for photonic_engines/slos/kernel.py:83-293.
+
+Same constructor arguments, the same output order (layers.final_states) and the same output
+tensors. The recurrence (deferred normalisation, kernel.py:127-141)
+
+    psi_{k+1}[c] = sum_{j distinct in c} U[b, j, i_{k+1}] * psi_k[c - e_j]
+
+runs as one hand-written gfx950 HIP kernel launch per photon layer (pull form: every child sums
+its parents, whose lex ranks are computed on the fly; nothing is stored but the two live layers),
+and the last launch writes |amp * input_norm|^2 * prod_j t_j! directly as float32.
+
+Device model: the engine always computes on an AMD GPU. `device` is where U is taken from and
+where results are returned; a CPU `device` (the reference's default) therefore means host-resident
+inputs and outputs around the same GPU computation. Without a visible GPU the engine raises.
+
+Precision: complex64 end to end; `compensated=True` (keyword) accumulates every child sum with
+fma-split products and two-sum (a compensated dot product), as an accuracy option. complex128 is
+not computed by this engine (ValueError), see the design notes.
+"""
+from __future__ import annotations
+
+import math
+from collections.abc import Sequence
+
+import torch
+
+from .. import _native as nat
+from ..imperfections import LossyDistribution  # noqa: F401  (re-exported like the reference)
+from ..masks import Mask
+from ._plan import Plan
+from .basis import RankedLayers, get_layers
+
+
+def _maybe_thin(layers, p: torch.Tensor, loss):
+    """loss None / >= 1: p itself; else its uniform-loss thinning (kernel.py:32-37)."""
+    if loss is None or (not isinstance(loss, torch.Tensor) and float(loss) >= 1.0):
+        return p
+    from ..imperfections import thin_distribution
+    return thin_distribution(layers, p, loss)
+
+
+class FockSLOSKernel:
+    """Precompiled SLOS evaluator for a fixed (n, m, mask, input); `probs(U)` is the timed op.
+
+    U is [m, m] or [B, m, m]; probabilities come back as [S] or [B, S] float32 over the final
+    (masked) basis in layers.final_states order.
+    """
+
+    def __init__(self, n: int, m: int, input_occupation: Sequence[int],
+                 mask: str = Mask.FOCK.value, device: str = "cpu",
+                 dtype: torch.dtype = torch.complex64,
+                 normalization: str = "deferred",
+                 gradient_checkpointing: bool = False,
+                 layout: str = "mode", layers=None, *, compensated: bool = False):
+        # argument checks and their order follow kernel.py:96-126
+        if layout not in ("mode", "edge"):
+            raise ValueError("layout must be 'mode' or 'edge'")
+        if layout == "mode" and (gradient_checkpointing or normalization != "deferred"):
+            layout = "edge"
+        self.layout = layout          # both layouts run the same pull kernel (no edge tables exist)
+        if normalization not in ("deferred", "immediate"):
+            raise ValueError(f"normalization must be 'deferred' or 'immediate', got {normalization!r}")
+        if gradient_checkpointing and normalization != "deferred":
+            raise ValueError("gradient_checkpointing requires deferred normalization")
+        if dtype not in (torch.complex64,):
+            raise ValueError(f"dtype {dtype}: the MI355X engine computes in complex64 "
+                             "(optionally compensated); complex128 is not supported")
+        self.gradient_checkpointing = gradient_checkpointing
+        self.normalization = normalization
+        self.compensated = bool(compensated)
+        self.n, self.m, self.mask = int(n), int(m), Mask(mask).value
+        self.device = torch.device(device)
+        self.cdtype = dtype
+        self.rdtype = torch.float32
+        self.layers: RankedLayers = layers if layers is not None else get_layers(n, m, self.mask, device=str(self.device))
+        if self.layers.output_size() == 0:
+            raise ValueError(f"empty output space for n={n} m={m} mask={mask}")
+        self.input_modes: list[int] = [j for j, c in enumerate(input_occupation) for _ in range(int(c))]
+        if len(self.input_modes) != n:
+            raise ValueError(f"input has {len(self.input_modes)} photons, expected n={n}")
+        norm = 1.0
+        for c in input_occupation:
+            norm *= math.factorial(int(c))
+        self._input_norm = 1.0 / math.sqrt(norm)
+        self._input_norm_is_one = norm == 1
+        # prod t_j! == 1 for every state exactly when no state can hold two photons in a mode
+        self._output_norm_is_one = self.mask in ("unbunched", "dual_rail", "sparse_1in4") or self.n < 2
+        self._plan = Plan(self.n, self.m, self.mask, self.input_modes)
+        self._output_prob_norm = None
+
+    # ---- reference attributes ------------------------------------------------------------------
+    @property
+    def output_prob_norm(self) -> torch.Tensor:
+        """prod_j t_j! per final state as an [S] float32 tensor on `device` (kernel.py:139),
+        built on first access; the engine itself never reads it (the norm is fused)."""
+        if self._output_prob_norm is None:
+            self._output_prob_norm = torch.as_tensor(self.layers.output_prob_norms()).to(
+                dtype=self.rdtype, device=self.device)
+        return self._output_prob_norm
+
+    def output_size(self) -> int:
+        """Size S of the output distribution (kernel.py:169-171)."""
+        return self.layers.output_size()
+
+    def _flags(self) -> int:
+        f = nat.FLAG_COMPENSATED if self.compensated else 0
+        if self.normalization == "immediate":
+            f |= nat.FLAG_IMMEDIATE
+        return f
+
+    def _add_oom_hint(self, e: BaseException) -> None:
+        """OOM guidance, a hint and never a retry (kernel.py:173-189)."""
+        if not self.gradient_checkpointing and torch.is_grad_enabled():
+            hint = ("hint: FockSLOSKernel(gradient_checkpointing=True) reduces peak GPU memory "
+                    "in training, at the cost of a slower backward pass.")
+            if hasattr(e, "add_note"):
+                e.add_note(hint)
+            elif len(e.args) == 1 and isinstance(e.args[0], str):
+                e.args = (f"{e.args[0]}\n{hint}",)
+            else:
+                e.args = (*e.args, hint)
+
+    def _prepare(self, U: torch.Tensor):
+        """U [m,m] | [B,m,m] (any device) -> (contiguous complex64 [B,m,m] on the GPU, batched)."""
+        batched = U.dim() == 3
+        if not batched:
+            U = U.unsqueeze(0)
+        if U.shape[-2:] != (self.m, self.m):
+            raise ValueError(f"U must be [m, m] or [B, m, m] with m={self.m}, got {tuple(U.shape)}")
+        dev = nat.require_gpu(U.device if U.device.type == "cuda" else self.device)
+        return U.to(device=dev, dtype=torch.complex64).contiguous(), batched
+
+    def _out(self, t: torch.Tensor, batched: bool) -> torch.Tensor:
+        t = t if batched else t.squeeze(0)
+        return t.to(self.device) if self.device.type != "cuda" else t
+
+    # ---- amplitudes ------------------------------------------------------------------------------
+    def amplitudes(self, U: torch.Tensor) -> torch.Tensor:
+        """Output amplitudes over the final basis, [S] / [B, S] complex64 (kernel.py:191-199):
+        the raw (deferred-normalisation) amplitude times the input norm."""
+        try:
+            return self._amplitudes(U)
+        except torch.cuda.OutOfMemoryError as e:
+            self._add_oom_hint(e)
+            raise
+
+    def _amplitudes(self, U: torch.Tensor, input_modes: Sequence[int] | None = None) -> torch.Tensor:
+        """With `input_modes` (g <= n columns) the recurrence stops on layer g of the same basis,
+        raw, without the input norm (kernel.py:201-206, used by slos/species.py:94)."""
+        if U.requires_grad and torch.is_grad_enabled():
+            raise NotImplementedError("gradients flow through probs(U); amplitudes() is forward-only")
+        Ug, batched = self._prepare(U)
+        if input_modes is None:
+            out = self._plan.forward(Ug, out_kind=nat.OUT_AMPS, scale=self._input_norm, flags=self._flags())
+        else:
+            modes = [int(i) for i in input_modes]
+            if not modes:
+                out = torch.ones((Ug.shape[0], 1), dtype=torch.complex64, device=Ug.device)
+            else:
+                out = self._plan.forward(Ug, modes=modes, layer=len(modes), out_kind=nat.OUT_AMPS,
+                                         scale=1.0, flags=self._flags() & ~nat.FLAG_IMMEDIATE)
+        return self._out(out, batched)
+
+    # ---- probabilities ------------------------------------------------------------------------------
+    def probs(self, U: torch.Tensor, loss=None, indist=None, order: int | None = None):
+        """Output probabilities over the final basis (kernel.py:262-283). With `loss=eta` the
+        uniform-loss LossyDistribution over 0..n photons is returned instead."""
+        if indist is not None:
+            from ..imperfections import spec
+            if spec(indist) is not None:
+                raise NotImplementedError("indist= (partial distinguishability) is not implemented by the "
+                                          "MI355X engine")
+        try:
+            if U.requires_grad and torch.is_grad_enabled():
+                from .autograd import SLOSProbs
+                Ug, batched = self._prepare(U)
+                p = SLOSProbs.apply(U, Ug, self)
+                p = p if batched else p.squeeze(0)
+                p = p.to(self.device) if self.device.type != "cuda" else p
+            else:
+                Ug, batched = self._prepare(U)
+                p = self._out(self._plan.forward(Ug, out_kind=nat.OUT_PROBS, scale=self._input_norm,
+                                                 flags=self._flags()), batched)
+        except torch.cuda.OutOfMemoryError as e:
+            self._add_oom_hint(e)
+            raise
+        return _maybe_thin(self.layers, p, loss)
+
+    def total_mass(self, p: torch.Tensor) -> torch.Tensor:
+        """FP64 sum of a probability vector [S] / [B, S] on the GPU (deterministic, no atomics)."""
+        q = p if p.dim() == 2 else p.unsqueeze(0)
+        dev = nat.require_gpu(q.device if q.device.type == "cuda" else self.device)
+        out = self._plan.total_mass(q.to(device=dev, dtype=torch.float32).contiguous())
+        return out if p.dim() == 2 else out.squeeze(0)
+
+    def spot_probs(self, U: torch.Tensor, indices) -> torch.Tensor:
+        """FP64 Ryser probabilities of the final states `indices` for one unitary U (complex128
+        inputs kept exact): the accuracy oracle at sizes with no full FP64 run."""
+        dev = nat.require_gpu(U.device if U.device.type == "cuda" else self.device)
+        idx = torch.as_tensor(indices, dtype=torch.int64, device=dev)
+        return self._plan.spot_probs(U.to(device=dev, dtype=torch.complex128), idx)

@@ -1,0 +1,209 @@
+// Host execution of the block layer kernel's index code (photonic_engines_amd/csrc/block_index.hpp)
+// with the kernel's own wave / block / 64-lane loop structure, in complex128. Test infrastructure:
+// it lets the CPU test suite check the GPU kernel's decomposition (block starts, prefix-stream
+// bases, suffix-tuple table, norms, wave boundaries) against the oracle without a GPU.
+#include <complex>
+#include <cstdint>
+#include <vector>
+
+#include "../../photonic_engines_amd/csrc/block_index.hpp"
+#include "../../photonic_engines_amd/csrc/pull_index.hpp"
+
+using cd = std::complex<double>;
+
+namespace {
+
+struct Space {
+  int me = 0, multiset = 1;
+  std::vector<uint64_t> cg;
+  std::vector<int> lo;
+  std::vector<slos::TupleEntry> tuples;
+};
+
+bool build(int n, int m, int mask, Space& s) {
+  int shift = 0;
+  switch (mask) {
+    case 0: s.me = m; s.multiset = 1; break;
+    case 1: s.me = m; s.multiset = 0; break;
+    case 2: s.me = m; s.multiset = 0; shift = 1; break;
+    case 3: s.me = m; s.multiset = 0; shift = 2; break;
+    case 4: s.me = m - 2; s.multiset = 1; break;
+    default: return false;
+  }
+  s.lo.resize(s.me);
+  for (int x = 0; x < s.me; ++x) {
+    int v = s.multiset ? x : (((x >> shift) + 1) << shift);
+    s.lo[x] = v > s.me ? s.me : v;
+  }
+  const int W = s.me + 1;
+  s.cg.assign((size_t)(n + 2) * W, 0);
+  for (int x = 0; x <= s.me; ++x) s.cg[x] = 1;
+  for (int r = 1; r <= n + 1; ++r)
+    for (int x = s.me - 1; x >= 0; --x) s.cg[(size_t)r * W + x] = s.cg[(size_t)r * W + x + 1] + s.cg[(size_t)(r - 1) * W + s.lo[x]];
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, W};
+  if (n > slos::kSuf) {
+    const uint64_t N = s.cg[(size_t)slos::kSuf * W];
+    s.tuples.resize(N);
+    for (uint64_t u = 0; u < N; ++u) s.tuples[u] = slos::make_tuple_entry(sv, s.multiset, u);
+  }
+  return true;
+}
+
+// thread-per-state formula (the simple kernel's slos::for_each_parent) for layers K <= SUF
+void layer_simple(const Space& s, int K, const cd* ucol, const std::vector<cd>& par, std::vector<cd>& out) {
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, s.me + 1};
+  const uint64_t S = sv.C(K, 0);
+  out.assign(S, 0);
+  for (uint64_t g = 0; g < S; ++g) {
+    cd acc = 0;
+    slos::for_each_parent(sv, s.multiset, K, g, [&](uint64_t pr, int mode, int) { acc += ucol[mode] * par[pr]; });
+    out[g] = acc;
+  }
+}
+
+// the block kernel's wave loop; writes[c] counts how often child c is written
+template <int MAXD>
+void layer_block(const Space& s, int K, const cd* ucol, const std::vector<cd>& par, std::vector<cd>& out,
+                 std::vector<double>* norms, uint64_t per_wave, std::vector<int>& writes) {
+  slos::BlockTables t{s.cg.data(), s.lo.data(), s.tuples.data(), s.me, s.me + 1, s.multiset};
+  const int W = s.me + 1;
+  const uint64_t S = s.cg[(size_t)K * W];
+  out.assign(S, 0);
+  writes.assign(S, 0);
+  if (norms) norms->assign(S, 0.0);
+  const int d = K - slos::kSuf;
+  const uint64_t waves = (S + per_wave - 1) / per_wave;
+  for (uint64_t w = 0; w < waves; ++w) {
+    uint64_t g = w * per_wave;
+    const uint64_t g_end = (g + per_wave < S) ? g + per_wave : S;
+    int q[MAXD];
+    uint64_t u0 = slos::unrank_prefix<MAXD>(t, K, d, g, q);
+    slos::BlockInfo bi;
+    slos::Streams<MAXD> st;
+    while (true) {
+      slos::block_info<MAXD>(t, K, d, q, bi, st);
+      const uint64_t avail = g_end - g;
+      const uint32_t u_stop = (uint32_t)(((uint64_t)bi.L - u0 < avail) ? (uint64_t)bi.L : u0 + avail);
+      for (uint32_t uc = (uint32_t)u0; uc < u_stop; uc += 64)
+        for (int lane = 0; lane < 64; ++lane) {
+          const uint32_t u = uc + lane;
+          if (u >= u_stop) continue;
+          cd acc = 0;
+          for (int si = 0; si < st.count; ++si) acc += ucol[st.mode[si]] * par.at(st.base[si] + u);
+          const slos::TupleEntry e = t.tuples[bi.toff + u];
+          const int x0 = e.w0 & 0xff, x1 = (e.w0 >> 8) & 0xff, x2 = (e.w0 >> 16) & 0xff;
+          const uint32_t valid = (e.w0 >> 24) & 7u;
+          if ((valid & 1u) && !(bi.skip0 && x0 == bi.lo)) acc += ucol[x0] * par.at(bi.LBp + e.lr0);
+          if (valid & 2u) acc += ucol[x1] * par.at(bi.LBp + e.lr1);
+          if (valid & 4u) acc += ucol[x2] * par.at(bi.LBp + e.lr2);
+          const uint64_t c = bi.CB + u;
+          out.at(c) = acc;
+          writes.at(c)++;
+          if (norms) (*norms)[c] = slos::child_norm(bi, e.w0, t.multiset);
+        }
+      g += u_stop - u0;
+      if (g >= g_end) break;
+      u0 = 0;
+      if (!slos::next_prefix(t, K, d, q)) break;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Forward pass from the vacuum to layer n; probs[S_n] = |amp * in_norm|^2 * prod t_j!.
+// Returns 0 on success, 2 if some child was not written exactly once, 1 on bad input.
+int emu_forward(int n, int m, int mask, const int* modes, const double* U, double in_norm, uint64_t per_wave,
+                double* probs) {
+  Space s;
+  if (!build(n, m, mask, s) || n > 32) return 1;
+  const cd* Uc = reinterpret_cast<const cd*>(U);
+  std::vector<cd> par(1, cd(1, 0)), out, ucol(m);
+  std::vector<double> norms;
+  std::vector<int> writes;
+  for (int K = 1; K <= n; ++K) {
+    for (int j = 0; j < m; ++j) ucol[j] = Uc[(size_t)j * m + modes[K - 1]];
+    const bool last = K == n;
+    if (K <= slos::kSuf) {
+      layer_simple(s, K, ucol.data(), par, out);
+      if (last) {
+        slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, s.me + 1};
+        norms.assign(out.size(), 1.0);
+        int p[64];
+        for (uint64_t g = 0; g < out.size(); ++g) {
+          sv.unrank(g, K, p);
+          norms[g] = slos::run_factorial(p, K, s.multiset);
+        }
+      }
+    } else {
+      const int d = K - slos::kSuf;
+      if (d <= 8) layer_block<8>(s, K, ucol.data(), par, out, last ? &norms : nullptr, per_wave, writes);
+      else layer_block<32>(s, K, ucol.data(), par, out, last ? &norms : nullptr, per_wave, writes);
+      for (int w : writes)
+        if (w != 1) return 2;
+    }
+    par.swap(out);
+  }
+  for (uint64_t g = 0; g < par.size(); ++g) probs[g] = std::norm(par[g] * in_norm) * norms[g];
+  return 0;
+}
+
+// q_{k-1}(t) = sum_j q_k(t + e_j) count / k via slos::for_each_child (the descent kernel's code)
+int emu_descend(int n, int m, int mask, int k, const double* qk, double* out) {
+  Space s;
+  if (!build(n, m, mask, s) || k < 1 || k > n) return 1;
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, s.me + 1};
+  const uint64_t Sp = sv.C(k - 1, 0);
+  for (uint64_t t = 0; t < Sp; ++t) {
+    double acc = 0;
+    slos::for_each_child(sv, s.multiset, k, t, [&](uint64_t c, int, int cnt) { acc += qk[c] * cnt; });
+    out[t] = acc / k;
+  }
+  return 0;
+}
+
+// g_k(c) = sum_{j distinct in c} g_{k-1}(c - e_j) count / k via slos::for_each_parent
+int emu_ascend(int n, int m, int mask, int k, const double* gp, double* gk) {
+  Space s;
+  if (!build(n, m, mask, s) || k < 1 || k > n) return 1;
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, s.me + 1};
+  const uint64_t Sk = sv.C(k, 0);
+  for (uint64_t c = 0; c < Sk; ++c) {
+    double acc = 0;
+    slos::for_each_parent(sv, s.multiset, k, c, [&](uint64_t p, int, int cnt) { acc += gp[p] * cnt; });
+    gk[c] = acc / k;
+  }
+  return 0;
+}
+
+// edges k-1 -> k enumerated from the parent side: (parent, child, mode, count) rows; returns count
+int64_t emu_child_edges(int n, int m, int mask, int k, int64_t cap, int64_t* out) {
+  Space s;
+  if (!build(n, m, mask, s) || k < 1 || k > n) return -1;
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, s.me + 1};
+  const uint64_t Sp = sv.C(k - 1, 0);
+  int64_t e = 0;
+  for (uint64_t t = 0; t < Sp; ++t)
+    slos::for_each_child(sv, s.multiset, k, t, [&](uint64_t c, int j, int cnt) {
+      if (e < cap) { out[4 * e] = (int64_t)t; out[4 * e + 1] = (int64_t)c; out[4 * e + 2] = j; out[4 * e + 3] = cnt; }
+      ++e;
+    });
+  return e;
+}
+
+// 1 if xcd_remap is a bijection of [0, nwg) for every nwg in [1, max_nwg]
+int emu_xcd_remap_ok(uint32_t max_nwg) {
+  std::vector<int> seen;
+  for (uint32_t nwg = 1; nwg <= max_nwg; ++nwg) {
+    seen.assign(nwg, 0);
+    for (uint32_t wg = 0; wg < nwg; ++wg) {
+      const uint32_t v = slos::xcd_remap(wg, nwg);
+      if (v >= nwg || seen[v]++) return 0;
+    }
+  }
+  return 1;
+}
+
+}  // extern "C"

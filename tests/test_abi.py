@@ -1,0 +1,72 @@
+"""The C ABI: the library loads, exports every symbol include/slos_mi355x.h declares, and its argument
+checks map to the reference's exceptions — all without a GPU (no compute call is made)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from photonic_engines_amd import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "slos_mi355x.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(slos_\w+)\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = nat.lib()
+    syms = header_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(L, s), s
+        assert s in nat.SIGNATURES, f"{s} has no ctypes signature"
+    assert set(nat.SIGNATURES) == set(syms)
+    assert nat.lib().slos_version().decode().startswith("slos_mi355x")
+
+
+def _create(n, m, mask, modes):
+    h = ctypes.c_void_p()
+    arr = (ctypes.c_int32 * len(modes))(*modes)
+    rc = nat.lib().slos_plan_create(n, m, mask, arr, ctypes.byref(h))
+    return rc, h
+
+
+def test_plan_errors():
+    L = nat.lib()
+    assert _create(3, 6, 9, [0, 1, 2])[0] == nat.SLOS_EINVAL
+    assert b"unknown mask" in L.slos_last_error()
+    assert _create(3, 6, 0, [0, 1, 7])[0] == nat.SLOS_EINVAL
+    assert _create(0, 6, 0, [])[0] == nat.SLOS_EINVAL
+    assert _create(40, 60, 0, [0] * 40)[0] == nat.SLOS_ERANGE
+    assert _create(4, 3, 1, [0, 1, 2, 2])[0] == nat.SLOS_EINVAL               # unbunched C(3,4) = 0
+    assert _create(30, 200, 0, [0] * 30)[0] == nat.SLOS_ERANGE               # C(229, 30) > 2^64
+    rc, h = _create(3, 6, 0, [0, 2, 4])
+    assert rc == nat.SLOS_OK
+    out = ctypes.c_uint64()
+    assert L.slos_layer_size(h, 3, ctypes.byref(out)) == 0 and out.value == 56
+    assert L.slos_layer_size(h, 4, ctypes.byref(out)) == nat.SLOS_EINVAL
+    # compute entry points validate before touching a device
+    assert L.slos_forward(h, None, 1, None, 3, 0, None, 1.0, 0, None, 0, None) == nat.SLOS_EINVAL
+    wsb = ctypes.c_size_t()
+    assert L.slos_forward_workspace(h, 2, 3, ctypes.byref(wsb)) == 0 and wsb.value >= 2 * 8 * (6 + 21)
+    idx = (ctypes.c_uint64 * 1)(56)
+    pos = (ctypes.c_int32 * 3)()
+    assert L.slos_unrank(h, 3, idx, 1, pos) == nat.SLOS_ERANGE
+    bad = (ctypes.c_int32 * 3)(2, 1, 0)                                         # not sorted
+    assert L.slos_rank(h, 3, bad, 1, idx) == nat.SLOS_EINVAL
+    assert L.slos_plan_destroy(h) == 0
+
+
+def test_error_mapping():
+    with pytest.raises(ValueError):
+        nat.check(nat.SLOS_EINVAL, "x")
+    with pytest.raises(OverflowError):
+        nat.check(nat.SLOS_ERANGE, "x")
+    import torch
+    with pytest.raises(torch.cuda.OutOfMemoryError):
+        nat.check(nat.SLOS_ENOMEM, "x")
+    with pytest.raises(nat.NativeError):
+        nat.check(nat.SLOS_EHIP, "x")

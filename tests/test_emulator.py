@@ -1,0 +1,79 @@
+"""The GPU kernels' index code executed on the CPU: tests/native/block_emu.cpp compiles the product
+headers photonic_engines_amd/csrc/{block_index,pull_index,space}.hpp with g++ and runs the block
+kernel's wave / block / 64-lane loop, the descent and its adjoint, against the oracle."""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from helpers import haar_unitary, occupation_list
+
+from oracle import pull
+from oracle import slos_oracle as so
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    out = tmp_path_factory.mktemp("emu") / "libemu.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wno-unknown-pragmas", "-o", str(out),
+                    os.path.join(ROOT, "tests", "native", "block_emu.cpp")], check=True)
+    L = ctypes.CDLL(str(out))
+    L.emu_child_edges.restype = ctypes.c_int64
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+FWD = [(4, 6, "fock", None), (5, 7, "fock", None), (6, 6, "fock", [3, 0, 2, 0, 1, 0]), (5, 8, "unbunched", None),
+       (6, 12, "dual_rail", None), (4, 16, "sparse_1in4", None), (5, 9, "tail_empty", [0, 2, 1, 1, 1, 0, 0, 0, 0]),
+       (7, 9, "fock", [1, 0, 2, 1, 0, 3, 0, 0, 0]), (8, 10, "fock", None), (6, 8, "fock", [6, 0, 0, 0, 0, 0, 0, 0]),
+       (7, 12, "fock", None), (5, 20, "sparse_1in4", None)]
+
+
+@pytest.mark.parametrize("n,m,mask,occ", FWD)
+def test_block_kernel_decomposition(emu, n, m, mask, occ):
+    occ = occ or occupation_list(n, m, mask)
+    U = haar_unitary(m, n + 7 * m).numpy()
+    ref = pull.forward_probs(n, m, occ, U, mask)
+    modes = np.array([j for j, c in enumerate(occ) for _ in range(c)], dtype=np.int32)
+    Ud = np.ascontiguousarray(U).view(np.float64)
+    innorm = 1 / math.sqrt(math.prod(math.factorial(c) for c in occ))
+    for per_wave in (64, 100, 256, 1000, 4096):
+        out = np.zeros(ref.shape[0])
+        rc = emu.emu_forward(n, m, pull.MASK_IDS[mask], _p(modes), _p(Ud), ctypes.c_double(innorm),
+                             ctypes.c_uint64(per_wave), _p(out))
+        assert rc == 0, "some child was not written exactly once" if rc == 2 else rc
+        assert np.abs(out - ref).max() < 1e-14
+
+
+@pytest.mark.parametrize("n,m,mask", [(4, 6, "fock"), (5, 5, "fock"), (4, 8, "unbunched"), (4, 8, "dual_rail"),
+                                      (3, 12, "sparse_1in4"), (4, 7, "tail_empty"), (6, 4, "fock")])
+def test_descent_adjoint_and_child_edges(emu, n, m, mask):
+    rng = np.random.default_rng(n * m)
+    lay = so.enumerate_layers(n, m, mask)
+    for k in range(1, n + 1):
+        q = rng.random(len(lay[k]))
+        out = np.zeros(len(lay[k - 1]))
+        assert emu.emu_descend(n, m, pull.MASK_IDS[mask], k, _p(q), _p(out)) == 0
+        assert np.abs(out - so.descend(lay, q, k)[0]).max() < 1e-12
+        g = rng.random(len(lay[k - 1]))
+        gk = np.zeros(len(lay[k]))
+        assert emu.emu_ascend(n, m, pull.MASK_IDS[mask], k, _p(g), _p(gk)) == 0
+        assert abs(gk @ q - g @ out) < 1e-10 * abs(g @ out)
+        cap = len(lay[k]) * m
+        buf = np.zeros(4 * cap, dtype=np.int64)
+        E = emu.emu_child_edges(n, m, pull.MASK_IDS[mask], k, cap, _p(buf))
+        got = sorted(map(tuple, buf[:4 * E].reshape(-1, 4).tolist()))
+        par, chi, mode, fac = so.edges(lay[k - 1], lay[k])
+        exp = sorted(zip(par.tolist(), chi.tolist(), mode.tolist(), np.round(fac ** 2).astype(int).tolist()))
+        assert got == exp
+
+
+def test_xcd_remap_is_a_bijection(emu):
+    assert emu.emu_xcd_remap_ok(4096) == 1

@@ -1,0 +1,164 @@
+"""GPU parity of the forward path (FockSLOSKernel.probs / amplitudes through the C ABI) against the
+oracle. Tolerances: the engine computes in complex64; the bar is max|dp| <= 1e-6 against complex128
+(the accuracy target of the frontier workload), with the max relative error reported by assertion
+messages."""
+import math
+
+import numpy as np
+import pytest
+import torch
+from helpers import first_modes, golden, haar_unitary, input_norm, occupation_list
+
+from oracle import pull
+from oracle import slos_oracle as so
+from photonic_engines_amd import _native as nat
+from photonic_engines_amd.slos import FockSLOSKernel, get_layers
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+TOL = 1e-6
+
+
+def _check(p_gpu, p_ref, tol=TOL, what=""):
+    p = p_gpu.detach().double().cpu().numpy()
+    err = np.abs(p - p_ref).max()
+    assert err <= tol, f"{what}: max|dp| = {err:.3e}"
+    return err
+
+
+@pytest.mark.parametrize("mask", ["fock", "unbunched", "dual_rail", "tail_empty"])
+def test_matches_permanents_all_masks(mask):                                  # test_kernel.py:38-46
+    n, m = 3, 6
+    U = haar_unitary(m, 7)
+    occ = occupation_list(n, m, mask)
+    p = FockSLOSKernel(n, m, occ, mask=mask, device=DEV).probs(U.to(torch.complex64))
+    _check(p, so.reference_probs(U.numpy(), n, m, occ, mask), what=mask)
+
+
+CASES = [(3, 6, "fock", None), (3, 6, "unbunched", None), (3, 6, "dual_rail", None), (2, 8, "sparse_1in4", None),
+         (3, 7, "tail_empty", None), (3, 4, "fock", [2, 0, 1, 0]), (5, 7, "fock", None), (4, 8, "dual_rail", None),
+         (3, 12, "sparse_1in4", None), (4, 7, "tail_empty", [0, 2, 1, 1, 0, 0, 0]), (6, 6, "fock", [3, 0, 2, 0, 1, 0]),
+         (7, 9, "fock", [1, 0, 2, 1, 0, 3, 0, 0, 0]), (8, 10, "fock", None), (6, 12, "dual_rail", None),
+         (5, 20, "sparse_1in4", None), (6, 8, "fock", [6, 0, 0, 0, 0, 0, 0, 0]), (9, 11, "fock", None),
+         (2, 129, "fock", None), (1, 5, "fock", None), (4, 130, "fock", None), (5, 60, "unbunched", None)]
+
+
+@pytest.mark.parametrize("n,m,mask,occ", CASES)
+def test_matches_oracle(n, m, mask, occ):
+    occ = occ or occupation_list(n, m, mask)
+    if len(occ) < m:
+        occ = occ + [0] * (m - len(occ))
+    U = haar_unitary(m, n * 31 + m).numpy()
+    ref = pull.forward_probs(n, m, occ, U, mask)
+    k = FockSLOSKernel(n, m, occ, mask=mask, device=DEV)
+    p = k.probs(torch.as_tensor(U).to(torch.complex64))
+    _check(p, ref, what=f"{n},{m},{mask}")
+    # the thread-per-state kernel (ablation arm) agrees too
+    ps = k._plan.forward(torch.as_tensor(U).to(torch.complex64).to(DEV).unsqueeze(0), scale=input_norm(occ),
+                         flags=nat.FLAG_SIMPLE)
+    _check(ps[0], ref, what=f"simple {n},{m},{mask}")
+    amp = k.amplitudes(torch.as_tensor(U).to(torch.complex64))
+    ref_amp = pull.forward_amps(n, m, occ, U, mask)
+    assert np.abs(amp.cpu().numpy() - ref_amp).max() < 1e-5
+
+
+def test_c1_golden():
+    U = torch.as_tensor(golden("c1_u.npy"))
+    occ = [1, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0]
+    k = FockSLOSKernel(3, 12, occ, device=DEV)
+    p = k.probs(U.to(torch.complex64))
+    assert p.shape == (364,) and p.dtype == torch.float32
+    _check(p, golden("c1_probs.npy"), what="C1")
+    assert abs(float(k.total_mass(p)) - 1) < 1e-6
+
+
+def test_c2_full_and_spot():
+    """n=8, m=24: all 7,888,725 outputs against the complex128 pull oracle, 10,000 against Ryser."""
+    U = golden("c2_u.npy")
+    occ = first_modes(8, 24)
+    k = FockSLOSKernel(8, 24, occ, device=DEV)
+    p = k.probs(torch.as_tensor(U).to(torch.complex64).to(DEV))
+    ref = pull.forward_probs(8, 24, occ, U)
+    err = _check(p, ref, what="C2")
+    pc = p.double().cpu().numpy()
+    tvd = 0.5 * np.abs(pc - ref).sum()
+    assert tvd < 1e-4, tvd
+    idx = golden("c2_spot_idx.npy").astype(np.int64)
+    assert np.abs(pc[idx] - golden("c2_spot_probs.npy")).max() <= TOL
+    assert abs(float(k.total_mass(p)) - 1) < 1e-5
+    # compensated accumulation is at least as accurate
+    kc = FockSLOSKernel(8, 24, occ, device=DEV, compensated=True)
+    pc2 = kc.probs(torch.as_tensor(U).to(torch.complex64).to(DEV))
+    err_c = _check(pc2, ref, what="C2 compensated")
+    assert err_c <= err * 1.01 + 1e-12
+
+
+def test_batched_equals_looped_and_deterministic():                            # test_kernel.py:64-71
+    n, m = 5, 10
+    occ = occupation_list(n, m)
+    k = FockSLOSKernel(n, m, occ, device=DEV)
+    Us = torch.stack([haar_unitary(m, s).to(torch.complex64) for s in (1, 2, 3)]).to(DEV)
+    pb = k.probs(Us)
+    assert pb.shape == (3, k.output_size())
+    for b in range(3):
+        assert torch.equal(pb[b], k.probs(Us[b]))
+    assert torch.equal(pb, k.probs(Us))
+
+
+def test_host_device_roundtrip_and_amplitude_norms():                          # test_tutorial.py:45-59
+    U = haar_unitary(8, 0)
+    k = FockSLOSKernel(n=3, m=8, input_occupation=[1, 0, 1, 0, 1, 0, 0, 0], mask="fock", device="cpu")
+    p = k.probs(U.to(torch.complex64))
+    assert p.device.type == "cpu" and p.shape == (120,) and abs(float(p.sum()) - 1) < 1e-6
+    amp = k.amplitudes(U.to(torch.complex64)) * torch.sqrt(torch.as_tensor(k.layers.output_prob_norms(), dtype=torch.float32))
+    assert torch.allclose(amp.abs() ** 2, p, atol=1e-6)
+    ki = FockSLOSKernel(3, 8, [1, 0, 1, 0, 1, 0, 0, 0], normalization="immediate", device="cpu")
+    assert torch.allclose(ki.amplitudes(U.to(torch.complex64)), amp, atol=1e-6)
+    assert torch.allclose(ki.probs(U.to(torch.complex64)), p, atol=1e-7)
+
+
+def test_immediate_equals_deferred_bunched():                                  # test_kernel.py:96-113
+    for n, m, mask, occ in [(2, 4, "fock", [1, 0, 1, 0]), (3, 6, "fock", [1, 0, 1, 0, 1, 0]),
+                            (2, 4, "unbunched", [1, 0, 1, 0]), (4, 8, "dual_rail", [1, 0, 1, 0, 1, 0, 1, 0]),
+                            (3, 6, "tail_empty", [1, 1, 1, 0, 0, 0]), (3, 4, "fock", [2, 0, 1, 0])]:
+        U = haar_unitary(m, 13).to(torch.complex64)
+        pd = FockSLOSKernel(n, m, occ, mask=mask, device=DEV).probs(U)
+        pi = FockSLOSKernel(n, m, occ, mask=mask, normalization="immediate", device=DEV).probs(U)
+        assert torch.allclose(pd, pi, atol=1e-6)
+        ref = so.reference_probs(U.to(torch.complex128).numpy(), n, m, occ, mask)
+        _check(pd, ref, what=f"{n},{m},{mask},{occ}")
+
+
+def test_subset_amplitudes():
+    """_amplitudes(U, input_modes=...) stops at layer g of the same basis, raw (kernel.py:201-206)."""
+    n, m = 4, 7
+    occ = first_modes(n, m)
+    k = FockSLOSKernel(n, m, occ, device=DEV)
+    U = haar_unitary(m, 5)
+    sub = k._amplitudes(U.to(torch.complex64), input_modes=[1, 3])
+    ref = pull.forward_amps(2, m, [0, 1, 0, 1, 0, 0, 0], U.numpy())
+    assert sub.shape == (math.comb(m + 1, 2),)
+    assert np.abs(sub.cpu().numpy() - ref).max() < 1e-6
+
+
+def test_frontier_shape_mass_and_spot_checks():
+    """n=11, m=32 (4.28e9 outputs): FP64 total mass of the float32 output and 10,000 seeded outputs
+    against complex128 Ryser permanents (computed on the CPU when the fixtures were made)."""
+    props = torch.cuda.get_device_properties(0)
+    if props.total_memory < 40 * 2 ** 30:
+        pytest.skip("needs ~26 GB of device memory")
+    U = torch.as_tensor(golden("c5_u.npy"))
+    k = FockSLOSKernel(11, 32, first_modes(11, 32), device=DEV)
+    p = k.probs(U.to(torch.complex64).to(DEV))
+    assert p.shape == (4_280_561_376,)
+    mass = float(k.total_mass(p))
+    assert abs(mass - 1) < 1e-4, mass
+    idx = torch.as_tensor(golden("c5_spot_idx.npy").astype(np.int64), device=DEV)
+    got = p[idx].double().cpu().numpy()
+    ref = golden("c5_spot_probs.npy")
+    assert np.abs(got - ref).max() <= TOL
+    rel = np.abs(got - ref) / np.maximum(ref, 1e-30)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    # the device FP64 Ryser oracle agrees with the host one
+    gs = k.spot_probs(U, idx[:256]).cpu().numpy()
+    assert np.abs(gs - ref[:256]).max() < 1e-12 * max(1.0, ref[:256].max())
