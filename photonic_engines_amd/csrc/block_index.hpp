@@ -24,16 +24,19 @@
 
 namespace slos {
 
-constexpr int kSuf = 3;
+constexpr int kSuf = 4;
 
-// suffix-tuple table entry for a SUF-tuple tau (lex order over all SUF-tuples of the space)
-//   w0: x0 | x1 << 8 | x2 << 16 | valid << 24 (3 bits: first copy of each mode inside tau)
-//       | lead << 27 (2 bits: copies of x0 in tau) | fcode << 29 (3 bits: prod of run
-//       factorials of tau, 1 -> 0, 2 -> 1, 6 -> 2)
-//   lr_j: global (SUF-1)-tuple index of tau without x_j
+// suffix-tuple table entry (16 B) for a SUF-tuple tau, in lex order over all SUF-tuples:
+//   pos   x0 | x1 << 8 | x2 << 16 | x3 << 24   (positions < 256)
+//   flags valid (bits 0-3: x_j is the first copy of its mode inside tau) | lead << 4 (3 bits:
+//         copies of x0 in tau) | fct << 7 (5 bits: prod of the run factorials of tau, <= 24)
+//   lr[j] global (SUF-1)-tuple index of tau without x_j (< 65536, checked at plan creation)
 struct TupleEntry {
-  uint32_t w0, lr0, lr1, lr2;
+  uint32_t pos;
+  uint16_t flags, pad;
+  uint16_t lr[kSuf];
 };
+static_assert(sizeof(TupleEntry) == 16, "tuple entry is 16 bytes");
 
 struct BlockTables {
   const uint64_t* cg;        // (n + 2) x (me + 1)
@@ -53,11 +56,14 @@ struct BlockInfo {
   float fQ;         // prod of run factorials of Q
 };
 
+// Prefix streams, one slot per prefix position l (active when q_l is the first copy of its mode),
+// so that every array index is a compile-time constant after unrolling and the whole per-wave
+// state stays in registers.
 template <int MAXD>
 struct Streams {
   uint64_t base[MAXD];
   int mode[MAXD];
-  int count;
+  bool active[MAXD];
 };
 
 template <int MAXD>
@@ -67,10 +73,12 @@ SLOS_HD void block_info(const BlockTables& t, int K, int d, const int* q, BlockI
 #define SLOS_C(r, x) cg[(r) * W + (x)]
   int lov[MAXD + 1];
   lov[0] = 0;
+  int lo = 0;
 #pragma unroll
-  for (int i = 0; i < MAXD; ++i)
-    if (i < d) lov[i + 1] = t.lo[q[i]];
-  const int lo = lov[d];
+  for (int i = 0; i < MAXD; ++i) {
+    lov[i + 1] = (i < d) ? t.lo[q[i]] : 0;
+    if (i < d) lo = lov[i + 1];
+  }
   uint64_t CB = 0, LB = 0;
 #pragma unroll
   for (int i = 0; i < MAXD; ++i)
@@ -102,10 +110,12 @@ SLOS_HD void block_info(const BlockTables& t, int K, int d, const int* q, BlockI
   for (int i = MAXD - 1; i >= 0; --i)
     sufT[i] = sufT[i + 1] + ((i < d) ? (SLOS_C(K - i, lov[i]) - SLOS_C(K - i, q[i])) : 0);
   uint64_t preA = 0;
-  int ns = 0;
   const uint64_t CS_lo = SLOS_C(kSuf, lo);
 #pragma unroll
   for (int l = 0; l < MAXD; ++l) {
+    st.active[l] = false;
+    st.base[l] = 0;
+    st.mode[l] = 0;
     if (l < d) {
       const bool first = !t.multiset || l == 0 || q[l] != q[l - 1];
       if (first) {
@@ -113,40 +123,52 @@ SLOS_HD void block_info(const BlockTables& t, int K, int d, const int* q, BlockI
         int lop;
         if (l == d - 1) {
           pr = preA;
-          lop = lov[d - 1];
+          lop = lov[l];
         } else {
           pr = preA + (SLOS_C(K - 1 - l, lov[l]) - SLOS_C(K - 1 - l, q[l + 1])) + sufT[l + 2];
           lop = lo;
         }
-        st.base[ns] = pr + (SLOS_C(kSuf, lop) - CS_lo);
-        st.mode[ns] = q[l];
-        ++ns;
+        st.base[l] = pr + (SLOS_C(kSuf, lop) - CS_lo);
+        st.mode[l] = q[l];
+        st.active[l] = true;
       }
       preA += SLOS_C(K - 1 - l, lov[l]) - SLOS_C(K - 1 - l, q[l]);
     }
   }
-  st.count = ns;
 #undef SLOS_C
 }
 
-// lex-next prefix Q (length d) whose block in layer K is non-empty; false after the last one
+// lex-next prefix Q (length d) whose block in layer K is non-empty; false after the last one.
+// Unrolled over positions so that q stays in registers: find the last position c that can be
+// raised (to the smallest admissible x > q_c leaving a completion), then refill c+1.. minimally.
+template <int MAXD>
 SLOS_HD bool next_prefix(const BlockTables& t, int K, int d, int* q) {
   const int W = t.stride;
-  for (int i = d - 1; i >= 0; --i) {
-    for (int x = q[i] + 1; x < t.me; ++x) {
-      if (t.cg[(K - 1 - i) * W + t.lo[x]] == 0) continue;
-      q[i] = x;
-      int l = t.lo[x];
-      for (int j = i + 1; j < d; ++j) {
-        int y = l;
-        while (t.cg[(K - 1 - j) * W + t.lo[y]] == 0) ++y;
-        q[j] = y;
-        l = t.lo[y];
+  int c = -1, cx = 0;
+#pragma unroll
+  for (int i = 0; i < MAXD; ++i)
+    if (i < d) {
+      int x = q[i] + 1;
+      while (x < t.me && t.cg[(K - 1 - i) * W + t.lo[x]] == 0) ++x;
+      if (x < t.me) {
+        c = i;
+        cx = x;
       }
-      return true;
     }
-  }
-  return false;
+  if (c < 0) return false;
+  int l = 0;
+#pragma unroll
+  for (int i = 0; i < MAXD; ++i)
+    if (i < d && i >= c) {
+      int v = cx;
+      if (i > c) {
+        v = l;
+        while (t.cg[(K - 1 - i) * W + t.lo[v]] == 0) ++v;
+      }
+      q[i] = v;
+      l = t.lo[v];
+    }
+  return true;
 }
 
 // XCD-aware workgroup order. The dispatcher hands consecutive workgroups to the 8 XCDs in turn
@@ -182,45 +204,44 @@ SLOS_HD uint64_t unrank_prefix(const BlockTables& t, int K, int d, uint64_t g, i
 inline TupleEntry make_tuple_entry(const SpaceView& sv, int multiset, uint64_t u) {
   int x[kSuf], r[kSuf - 1];
   sv.unrank(u, kSuf, x);
-  uint32_t valid = 1u;
-  if (!multiset || x[1] != x[0]) valid |= 2u;
-  if (!multiset || x[2] != x[1]) valid |= 4u;
-  const int lead = multiset ? 1 + (x[1] == x[0]) + (x[1] == x[0] && x[2] == x[0]) : 1;
-  int fct = 1;
-  if (multiset) {
-    if (x[0] == x[1] && x[1] == x[2]) fct = 6;
-    else if (x[0] == x[1] || x[1] == x[2]) fct = 2;
-  }
-  const uint32_t fcode = fct == 1 ? 0u : (fct == 2 ? 1u : 2u);
   TupleEntry e;
-  e.w0 = (uint32_t)x[0] | ((uint32_t)x[1] << 8) | ((uint32_t)x[2] << 16) | (valid << 24) | ((uint32_t)lead << 27) |
-         (fcode << 29);
-  uint32_t lr[kSuf];
+  e.pos = 0;
+  uint32_t valid = 0;
+  for (int j = 0; j < kSuf; ++j) {
+    e.pos |= (uint32_t)x[j] << (8 * j);
+    if (!multiset || j == 0 || x[j] != x[j - 1]) valid |= 1u << j;
+  }
+  int lead = 1;
+  if (multiset)
+    while (lead < kSuf && x[lead] == x[0]) ++lead;
+  int fct = 1, run = 1;
+  if (multiset)
+    for (int j = 1; j < kSuf; ++j) {
+      run = (x[j] == x[j - 1]) ? run + 1 : 1;
+      fct *= run;
+    }
+  e.flags = (uint16_t)(valid | ((uint32_t)lead << 4) | ((uint32_t)fct << 7));
+  e.pad = 0;
   for (int j = 0; j < kSuf; ++j) {
     int c = 0;
     for (int i = 0; i < kSuf; ++i)
       if (i != j) r[c++] = x[i];
-    lr[j] = (uint32_t)sv.rank(r, kSuf - 1);
+    e.lr[j] = (uint16_t)sv.rank(r, kSuf - 1);
   }
-  e.lr0 = lr[0];
-  e.lr1 = lr[1];
-  e.lr2 = lr[2];
   return e;
 }
 
 // prod t_j! of the child (Q, tau) from the block's run data and the tuple entry
-SLOS_HD float child_norm(const BlockInfo& bi, uint32_t w0, int multiset) {
+SLOS_HD float child_norm(const BlockInfo& bi, const TupleEntry& e, int multiset) {
   if (!multiset) return 1.f;
-  const uint32_t fcode = w0 >> 29;
-  float f = bi.fQ * (fcode == 0 ? 1.f : (fcode == 1 ? 2.f : 6.f));
-  const int x0 = (int)(w0 & 0xff);
+  float f = bi.fQ * (float)(e.flags >> 7);
+  const int x0 = (int)(e.pos & 0xffu);
   if (bi.skip0 && x0 == bi.lo) {
     // tau's leading run continues the last run of Q: (r + c)! = r! c! binom(r + c, c)
-    const int lead = (int)((w0 >> 27) & 3u);
+    const int lead = (int)((e.flags >> 4) & 7u);
     const float r = (float)bi.lastrun;
-    float b = r + 1.f;
-    if (lead >= 2) b = b * (r + 2.f) * 0.5f;
-    if (lead >= 3) b = b * (r + 3.f) * (1.f / 3.f);
+    float b = 1.f;
+    for (int i = 1; i <= lead; ++i) b = b * (r + (float)i) / (float)i;
     f *= b;
   }
   return f;

@@ -426,9 +426,13 @@ __global__ __launch_bounds__(64) void k_reduce_gu(const double2* __restrict__ pa
 
 template <int KIND, bool COMP>
 void launch_block_kind(int maxd, dim3 grid, hipStream_t st, const slos::BlockTables& t, const slos::BlockLayer& L) {
-  if (maxd <= 8) hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, 8>), grid, dim3(slos::kBlockThreads), 0, st, t, L);
-  else if (maxd <= 16) hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, 16>), grid, dim3(slos::kBlockThreads), 0, st, t, L);
-  else hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, 32>), grid, dim3(slos::kBlockThreads), 0, st, t, L);
+#define SLOS_BLK(D)                                                                                         \
+  hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, D>), grid, dim3(slos::kBlockThreads), 0, st, t.cg, t.lo, \
+                     t.tuples, t.me, t.stride, t.multiset, L.par, L.out, L)
+  if (maxd <= 8) SLOS_BLK(8);
+  else if (maxd <= 16) SLOS_BLK(16);
+  else SLOS_BLK(32);
+#undef SLOS_BLK
 }
 
 int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, int kind, bool comp, hipStream_t st) {
@@ -555,7 +559,9 @@ int slos_plan_create(int n, int m, int mask, const int32_t* input_modes, slos_pl
       }
       p->cg[(size_t)r * st + x] = v;
     }
-  if (n > slos::kSuf && p->m <= 256 && p->size(slos::kSuf) <= (4u << 20)) build_tuples(p);
+  // the block kernel needs positions < 256 (8-bit fields) and (SUF-1)-tuple indices < 65536
+  if (n > slos::kSuf && p->m <= 256 && p->size(slos::kSuf - 1) < 65536 && p->size(slos::kSuf) <= (4u << 20))
+    build_tuples(p);
   if (p->size(n) == 0) {
     delete p;
     return fail(SLOS_EINVAL, "empty output space for n=" + std::to_string(n) + " m=" + std::to_string(m));

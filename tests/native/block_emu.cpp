@@ -89,22 +89,24 @@ void layer_block(const Space& s, int K, const cd* ucol, const std::vector<cd>& p
           const uint32_t u = uc + lane;
           if (u >= u_stop) continue;
           cd acc = 0;
-          for (int si = 0; si < st.count; ++si) acc += ucol[st.mode[si]] * par.at(st.base[si] + u);
+          for (int si = 0; si < MAXD; ++si)
+            if (st.active[si]) acc += ucol[st.mode[si]] * par.at(st.base[si] + u);
           const slos::TupleEntry e = t.tuples[bi.toff + u];
-          const int x0 = e.w0 & 0xff, x1 = (e.w0 >> 8) & 0xff, x2 = (e.w0 >> 16) & 0xff;
-          const uint32_t valid = (e.w0 >> 24) & 7u;
-          if ((valid & 1u) && !(bi.skip0 && x0 == bi.lo)) acc += ucol[x0] * par.at(bi.LBp + e.lr0);
-          if (valid & 2u) acc += ucol[x1] * par.at(bi.LBp + e.lr1);
-          if (valid & 4u) acc += ucol[x2] * par.at(bi.LBp + e.lr2);
+          for (int j = 0; j < slos::kSuf; ++j) {
+            const int xj = (int)((e.pos >> (8 * j)) & 0xffu);
+            bool on = (e.flags >> j) & 1u;
+            if (j == 0) on = on && !(bi.skip0 && xj == bi.lo);
+            if (on) acc += ucol[xj] * par.at(bi.LBp + e.lr[j]);
+          }
           const uint64_t c = bi.CB + u;
           out.at(c) = acc;
           writes.at(c)++;
-          if (norms) (*norms)[c] = slos::child_norm(bi, e.w0, t.multiset);
+          if (norms) (*norms)[c] = slos::child_norm(bi, e, t.multiset);
         }
       g += u_stop - u0;
       if (g >= g_end) break;
       u0 = 0;
-      if (!slos::next_prefix(t, K, d, q)) break;
+      if (!slos::next_prefix<MAXD>(t, K, d, q)) break;
     }
   }
 }

@@ -54,7 +54,7 @@ def test_matches_oracle(n, m, mask, occ):
     p = k.probs(torch.as_tensor(U).to(torch.complex64))
     _check(p, ref, what=f"{n},{m},{mask}")
     # the thread-per-state kernel (ablation arm) agrees too
-    ps = k._plan.forward(torch.as_tensor(U).to(torch.complex64).to(DEV).unsqueeze(0), scale=input_norm(occ),
+    ps = k._plan.forward(torch.as_tensor(U).to(torch.complex64).to(DEV).unsqueeze(0).contiguous(), scale=input_norm(occ),
                          flags=nat.FLAG_SIMPLE)
     _check(ps[0], ref, what=f"simple {n},{m},{mask}")
     amp = k.amplitudes(torch.as_tensor(U).to(torch.complex64))
