@@ -12,7 +12,8 @@ import os
 import torch  # noqa: F401  (binds the library to torch's HIP runtime: one runtime per process)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libslos_mi355x.so")
+# SLOS_MI355X_LIB selects another build of the same library (e.g. an instrumented one)
+LIB_PATH = os.environ.get("SLOS_MI355X_LIB") or os.path.join(_HERE, "_build", "libslos_mi355x.so")
 
 SLOS_OK, SLOS_EINVAL, SLOS_ENOMEM, SLOS_EHIP, SLOS_ERANGE, SLOS_ENODEV = range(6)
 MASK_IDS = {"fock": 0, "unbunched": 1, "dual_rail": 2, "sparse_1in4": 3, "tail_empty": 4}

@@ -25,9 +25,11 @@
 namespace slos {
 
 constexpr int kSuf = 4;
+constexpr int kNoEdge = 255;
 
 // suffix-tuple table entry (16 B) for a SUF-tuple tau, in lex order over all SUF-tuples:
-//   pos   x0 | x1 << 8 | x2 << 16 | x3 << 24   (positions < 256)
+//   pos   x0 | x1 << 8 | x2 << 16 | x3 << 24   (positions < 255; a repeated copy of a mode,
+//         which is not an edge, carries the position kNoEdge, whose U coefficient is zero)
 //   flags valid (bits 0-3: x_j is the first copy of its mode inside tau) | lead << 4 (3 bits:
 //         copies of x0 in tau) | fct << 7 (5 bits: prod of the run factorials of tau, <= 24)
 //   lr[j] global (SUF-1)-tuple index of tau without x_j (< 65536, checked at plan creation)
@@ -181,6 +183,21 @@ SLOS_HD uint32_t xcd_remap(uint32_t wg, uint32_t nwg) {
   return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
+// lex rank of the prefix Q among all d-tuples of the space (layer d)
+template <int MAXD>
+SLOS_HD uint64_t prefix_rank(const BlockTables& t, int d, const int* q) {
+  const int W = t.stride;
+  uint64_t r = 0;
+  int l = 0;
+#pragma unroll
+  for (int i = 0; i < MAXD; ++i)
+    if (i < d) {
+      r += t.cg[(d - i) * W + l] - t.cg[(d - i) * W + q[i]];
+      l = t.lo[q[i]];
+    }
+  return r;
+}
+
 // prefix Q of the child with lex index g in layer K; returns g's position u inside Q's block
 template <int MAXD>
 SLOS_HD uint64_t unrank_prefix(const BlockTables& t, int K, int d, uint64_t g, int* q) {
@@ -208,8 +225,9 @@ inline TupleEntry make_tuple_entry(const SpaceView& sv, int multiset, uint64_t u
   e.pos = 0;
   uint32_t valid = 0;
   for (int j = 0; j < kSuf; ++j) {
-    e.pos |= (uint32_t)x[j] << (8 * j);
-    if (!multiset || j == 0 || x[j] != x[j - 1]) valid |= 1u << j;
+    const bool first = !multiset || j == 0 || x[j] != x[j - 1];
+    if (first) valid |= 1u << j;
+    e.pos |= (uint32_t)(j == 0 || first ? x[j] : kNoEdge) << (8 * j);
   }
   int lead = 1;
   if (multiset)

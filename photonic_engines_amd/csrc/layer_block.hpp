@@ -2,10 +2,14 @@
 // pass (replaces the per-slab gather / scatter_add_ of photonic_engines/slos/kernel.py:216-227).
 // The index arithmetic (blocks, aligned prefix streams, suffix-tuple table) is in block_index.hpp.
 //
-// A wave owns a contiguous range of children and walks the blocks it meets in order. Block data
-// (start, bases of the prefix streams, their U coefficients) is wave-uniform and lives in scalar
-// registers; each step covers 64 consecutive children of one block, so the child writes and every
-// prefix stream are fully coalesced. Nothing is written twice and nothing is atomic.
+// Work split. A wave owns a contiguous range of children. It walks the blocks it meets in batches
+// of 64 consecutive prefixes: lane i unranks prefix r + i and computes that block's data (start,
+// size, stream bases, coefficients) with the counting tables in LDS, all lanes in parallel. The
+// wave then takes the blocks one at a time, broadcasting the block's data from its lane into
+// scalar registers (readlane), and sweeps the block's children 64 at a time. In that sweep every
+// address is a scalar base plus a 32-bit lane offset, the prefix streams and the child writes are
+// fully coalesced, all loads of a step are in flight together (no branches), and nothing is
+// written twice or atomically.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -15,29 +19,35 @@
 namespace slos {
 
 constexpr int kBlockThreads = 256;
+constexpr int kWaves = kBlockThreads / 64;
 
 struct BlockLayer {
   int K, m, col;
   const float2* U;       // [B][m][m]
-  const float2* par;     // [B][S_par]
-  uint64_t S_par;
-  void* out;             // [B][S]
-  uint64_t S;
+  uint64_t S_par;        // parent layer size
+  uint64_t S;            // child layer size
+  uint64_t S_pre;        // number of d-prefixes (layer d of the space)
   float scale;
   unsigned flags;
   uint64_t per_wave;     // children per wave
 };
 
-// wave-uniform copy of a value the compiler cannot prove uniform (keeps it in scalar registers)
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t rl(uint32_t v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ uint64_t rl64(uint64_t v, int lane) {
+  return ((uint64_t)rl((uint32_t)(v >> 32), lane) << 32) | rl((uint32_t)v, lane);
 }
-
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  return ((uint64_t)rfl((uint32_t)(v >> 32)) << 32) | rfl((uint32_t)v);
+}
 __device__ __forceinline__ float2 uniform_f2(float2 v) {
-  return make_float2(__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v.x))),
-                     __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v.y))));
+  return make_float2(__builtin_bit_cast(float, rfl(__builtin_bit_cast(uint32_t, v.x))),
+                     __builtin_bit_cast(float, rfl(__builtin_bit_cast(uint32_t, v.y))));
+}
+// element `off` bytes past a scalar base: lets the compiler use base-in-SGPR addressing
+template <class T>
+__device__ __forceinline__ const T& at_bytes(const T* base, uint32_t off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
 }
 
 struct BlockAcc {
@@ -68,10 +78,10 @@ __device__ __forceinline__ void bk_cmac(BlockAcc& a, float2 u, float2 v) {
   }
 }
 
+// LDS bytes: counting-table rows 0..K and lo[] (per-lane block data is computed from them)
+inline size_t block_lds_bytes(int K, int me) { return (size_t)(K + 1) * (me + 1) * 8 + (size_t)me * 4; }
+
 // KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes
-// The counting tables are read with wave-uniform indices from read-only (__restrict__) kernel
-// arguments, so the per-block arithmetic runs on the scalar unit (s_load / s_add) and the stream
-// bases sit in scalar registers; only the U column goes to LDS (per-lane indexed by suffix modes).
 template <int KIND, bool COMP, int MAXD>
 __global__ __launch_bounds__(kBlockThreads) void k_layer_block(const uint64_t* __restrict__ cg_t,
                                                                const int* __restrict__ lo_t,
@@ -79,89 +89,156 @@ __global__ __launch_bounds__(kBlockThreads) void k_layer_block(const uint64_t* _
                                                                int me, int stride, int multiset,
                                                                const float2* __restrict__ par_all,
                                                                void* __restrict__ out_all, BlockLayer a) {
-  __shared__ float2 ucol[256];
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ float2 ucol[256];            // U[b, :, col]; entries >= m are zero (kNoEdge = 255)
+  __shared__ float2 ucol_w[kWaves][256];  // per-wave copy with the block's own zero at lo (skip0)
   const int K = a.K;
+  const int ncg = (K + 1) * stride;
+  uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
+  int* lo_l = reinterpret_cast<int*>(cg_l + ncg);
   const int64_t b = blockIdx.y;
-  for (int j = threadIdx.x; j < a.m; j += blockDim.x)
-    ucol[j] = a.U[(size_t)b * a.m * a.m + (size_t)j * a.m + a.col];
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = cg_t[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo_l[i] = lo_t[i];
+  for (int j = threadIdx.x; j < 256; j += blockDim.x) {
+    const float2 v = j < a.m ? a.U[(size_t)b * a.m * a.m + (size_t)j * a.m + a.col] : make_float2(0.f, 0.f);
+    ucol[j] = v;
+    for (int w = 0; w < kWaves; ++w) ucol_w[w][j] = v;
+  }
   __syncthreads();
-  const BlockTables t{cg_t, lo_t, tuples, me, stride, multiset};
+  const BlockTables tg{cg_t, lo_t, tuples, me, stride, multiset};  // global: scalar loads
+  const BlockTables tl{cg_l, lo_l, tuples, me, stride, multiset};  // LDS: per-lane lookups
 
   const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wv = (int)rfl(threadIdx.x >> 6);
+  float2* uw = ucol_w[wv];
   const uint32_t wg_l = xcd_remap(blockIdx.x, gridDim.x);
-  uint64_t g = uniform_u64(((uint64_t)wg_l * (kBlockThreads / 64) + wv) * a.per_wave);
+  uint64_t g = uniform_u64(((uint64_t)wg_l * kWaves + wv) * a.per_wave);
   if (g >= a.S) return;
   const uint64_t g_end = (g + a.per_wave < a.S) ? g + a.per_wave : a.S;
   const int d = K - kSuf;
   const float2* par = par_all + (size_t)b * a.S_par;
 
-  int q[MAXD];
-  uint64_t u0 = unrank_prefix<MAXD>(t, K, d, g, q);
-  BlockInfo bi;
-  Streams<MAXD> st;
-  while (true) {
-    block_info<MAXD>(t, K, d, q, bi, st);
-    // stream slots: inactive ones (repeated modes, positions >= d) re-read an active stream's
-    // address with a zero coefficient, so the chunk body below has no branches and all of its
-    // loads are in flight together
-    float2 coef[MAXD];
-    const float2* sp[MAXD];
-    const float2* sp_any = par;
-#pragma unroll
-    for (int s = 0; s < MAXD; ++s)
-      if (st.active[s]) sp_any = par + st.base[s];
-#pragma unroll
-    for (int s = 0; s < MAXD; ++s) {
-      coef[s] = st.active[s] ? uniform_f2(ucol[st.mode[s]]) : make_float2(0.f, 0.f);
-      sp[s] = st.active[s] ? par + st.base[s] : sp_any;
-    }
-    const float2* lp = par + bi.LBp;
-    const TupleEntry* tb = t.tuples + bi.toff;
-    const uint64_t avail = g_end - g;
-    const uint32_t u_stop = (uint32_t)(((uint64_t)bi.L - u0 < avail) ? (uint64_t)bi.L : u0 + avail);
-    for (uint32_t uc = (uint32_t)u0; uc < u_stop; uc += 64) {
-      const uint32_t u_raw = uc + lane;
-      const bool in = u_raw < u_stop;
-      const uint32_t u = in ? u_raw : u_stop - 1;  // tail lanes re-read a valid child, store nothing
-      float2 sv[MAXD];
-#pragma unroll
-      for (int s = 0; s < MAXD; ++s) sv[s] = sp[s][u];
-      const TupleEntry e = tb[u];
-      float2 lv[kSuf];
-#pragma unroll
-      for (int j = 0; j < kSuf; ++j) lv[j] = lp[e.lr[j]];  // every lr_j indexes a real parent
-      BlockAcc acc{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int s = 0; s < MAXD; ++s) bk_cmac<COMP>(acc, coef[s], sv[s]);
-#pragma unroll
-      for (int j = 0; j < kSuf; ++j) {
-        const int xj = (int)((e.pos >> (8 * j)) & 0xffu);
-        bool on = (e.flags >> j) & 1u;
-        if (j == 0) on = on && !(bi.skip0 && xj == bi.lo);
-        const float2 cu = ucol[xj];
-        bk_cmac<COMP>(acc, on ? cu : make_float2(0.f, 0.f), lv[j]);
-      }
-      float re = acc.re + acc.cre, im = acc.im + acc.cim;
-      const uint64_t c = bi.CB + u;
-      if constexpr (KIND == 0) {
-        if (in) reinterpret_cast<float2*>(out_all)[(size_t)b * a.S + c] = make_float2(re, im);
+  // the block holding g: its prefix rank r and g's offset inside it (scalar path)
+  uint64_t r, u0;
+  {
+    int q[MAXD];
+    u0 = unrank_prefix<MAXD>(tg, K, d, g, q);
+    r = prefix_rank<MAXD>(tg, d, q);
+  }
+  int zeroed = kNoEdge;  // the entry of uw currently zeroed for skip0 (kNoEdge: none)
+  while (g < g_end) {
+    // ---- lane i: data of the block of prefix r + i ----
+    BlockInfo bi;
+    Streams<MAXD> st;
+    {
+      int q[MAXD];
+      const uint64_t ri = r + (uint64_t)lane;
+      const SpaceView sv{cg_l, lo_l, me, stride};
+      if (ri < a.S_pre) {
+        sv.unrank(ri, d, q);
+        block_info<MAXD>(tl, K, d, q, bi, st);
       } else {
-        const float f = child_norm(bi, e, t.multiset);
-        re *= a.scale;
-        im *= a.scale;
-        if constexpr (KIND == 1) {
-          if (in) reinterpret_cast<float*>(out_all)[(size_t)b * a.S + c] = (re * re + im * im) * f;
-        } else {
-          const float s = (a.flags & 2u) ? sqrtf(f) : 1.f;
-          if (in) reinterpret_cast<float2*>(out_all)[(size_t)b * a.S + c] = make_float2(re * s, im * s);
+        bi.L = 0;
+        bi.CB = bi.LBp = 0;
+        bi.toff = 0;
+        bi.lo = 0;
+        bi.skip0 = 0;
+        bi.lastrun = 0;
+        bi.fQ = 1.f;
+#pragma unroll
+        for (int s = 0; s < MAXD; ++s) {
+          st.base[s] = 0;
+          st.mode[s] = kNoEdge;
+          st.active[s] = false;
         }
       }
     }
-    g += u_stop - u0;
-    if (g >= g_end) break;
-    u0 = 0;
-    if (!next_prefix<MAXD>(t, K, d, q)) break;
+    // stream slots: inactive ones read the first slot's address with a zero coefficient
+    const uint64_t base0 = st.base[0];
+    uint32_t modes[MAXD];
+#pragma unroll
+    for (int s = 0; s < MAXD; ++s) modes[s] = st.active[s] ? (uint32_t)st.mode[s] : (uint32_t)kNoEdge;
+    if (r >= a.S_pre) break;  // cannot happen while g < S; keeps the loop bounded regardless
+    const int nb = (int)((a.S_pre - r < 64) ? a.S_pre - r : 64);
+    for (int j = 0; j < nb && g < g_end; ++j) {
+      const uint32_t L = rl(bi.L, j);
+      if (L == 0) continue;                       // empty block (set masks only)
+      const uint64_t CB = rl64(bi.CB, j), LBp = rl64(bi.LBp, j);
+      const uint32_t toff = rl(bi.toff, j);
+      const int blo = (int)rl((uint32_t)bi.lo, j);
+      const int skip0 = (int)rl((uint32_t)bi.skip0, j);
+      float2 coef[MAXD];
+      const float2* sp[MAXD];
+      const uint64_t b0 = rl64(base0, j);
+#pragma unroll
+      for (int s = 0; s < MAXD; ++s) {
+        const uint32_t md = rl(modes[s], j);
+        coef[s] = uniform_f2(ucol[md]);
+        sp[s] = par + (md == (uint32_t)kNoEdge ? b0 : rl64(st.base[s], j));
+      }
+      // per-wave coefficient table: zero at lo when tau's leading copy of lo is not an edge
+      const int want = skip0 ? blo : kNoEdge;
+      if (want != zeroed) {
+        if (lane == 0) {
+          uw[zeroed] = ucol[zeroed];
+          uw[want] = make_float2(0.f, 0.f);
+        }
+        zeroed = want;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      float fQ = 1.f;
+      int lastrun = 0;
+      if constexpr (KIND != 0) {
+        fQ = __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, bi.fQ), j));
+        lastrun = (int)rl((uint32_t)bi.lastrun, j);
+      }
+      const float2* lp = par + LBp;
+      const TupleEntry* tb = tuples + toff;
+      char* ob = reinterpret_cast<char*>(out_all) + ((size_t)b * a.S + CB) * (KIND == 1 ? 4 : 8);
+      const uint64_t avail = g_end - g;
+      const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
+      for (uint32_t uc = (uint32_t)u0; uc < u_stop; uc += 64) {
+        const uint32_t u_raw = uc + lane;
+        const bool in = u_raw < u_stop;
+        const uint32_t u = in ? u_raw : u_stop - 1;  // tail lanes re-read a valid child, store nothing
+        float2 sv[MAXD];
+#pragma unroll
+        for (int s = 0; s < MAXD; ++s) sv[s] = at_bytes(sp[s], u * 8u);
+        const TupleEntry e = at_bytes(tb, u * 16u);
+        float2 lv[kSuf];
+#pragma unroll
+        for (int jj = 0; jj < kSuf; ++jj) lv[jj] = at_bytes(lp, (uint32_t)e.lr[jj] * 8u);  // real parents
+        BlockAcc acc{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < MAXD; ++s) bk_cmac<COMP>(acc, coef[s], sv[s]);
+#pragma unroll
+        for (int jj = 0; jj < kSuf; ++jj) bk_cmac<COMP>(acc, uw[(e.pos >> (8 * jj)) & 0xffu], lv[jj]);
+        float re = acc.re + acc.cre, im = acc.im + acc.cim;
+        if constexpr (KIND == 0) {
+          if (in) *reinterpret_cast<float2*>(ob + u * 8u) = make_float2(re, im);
+        } else {
+          BlockInfo cb;
+          cb.fQ = fQ;
+          cb.lastrun = lastrun;
+          cb.skip0 = skip0;
+          cb.lo = blo;
+          const float f = child_norm(cb, e, multiset);
+          re *= a.scale;
+          im *= a.scale;
+          if constexpr (KIND == 1) {
+            if (in) *reinterpret_cast<float*>(ob + u * 4u) = (re * re + im * im) * f;
+          } else {
+            const float s = (a.flags & 2u) ? sqrtf(f) : 1.f;
+            if (in) *reinterpret_cast<float2*>(ob + u * 8u) = make_float2(re * s, im * s);
+          }
+        }
+      }
+      g += u_stop - u0;
+      u0 = 0;
+    }
+    r += 64;
   }
 }
 

@@ -425,34 +425,38 @@ __global__ __launch_bounds__(64) void k_reduce_gu(const double2* __restrict__ pa
 }
 
 template <int KIND, bool COMP>
-void launch_block_kind(int maxd, dim3 grid, hipStream_t st, const slos::BlockTables& t, const slos::BlockLayer& L) {
-#define SLOS_BLK(D)                                                                                         \
-  hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, D>), grid, dim3(slos::kBlockThreads), 0, st, t.cg, t.lo, \
-                     t.tuples, t.me, t.stride, t.multiset, L.par, L.out, L)
+void launch_block_kind(int maxd, dim3 grid, size_t sh, hipStream_t st, const DevTables& dt, const slos_plan* p,
+                       const float2* par, void* out, const slos::BlockLayer& L) {
+#define SLOS_BLK(D)                                                                                          \
+  hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, D>), grid, dim3(slos::kBlockThreads), sh, st, dt.cg, dt.lo, \
+                     dt.tuples, p->me, p->me + 1, p->multiset, par, out, L)
   if (maxd <= 8) SLOS_BLK(8);
-  else if (maxd <= 16) SLOS_BLK(16);
-  else SLOS_BLK(32);
+  else SLOS_BLK(16);
 #undef SLOS_BLK
 }
 
 int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, int kind, bool comp, hipStream_t st) {
-  slos::BlockTables t;
-  t.cg = dt.cg; t.lo = dt.lo; t.tuples = dt.tuples; t.me = p->me; t.stride = p->me + 1; t.multiset = p->multiset;
   slos::BlockLayer L;
-  L.K = a.K; L.m = a.m; L.col = a.col; L.U = a.U; L.par = a.par; L.S_par = a.S_par; L.out = a.out; L.S = a.S;
+  L.K = a.K; L.m = a.m; L.col = a.col; L.U = a.U; L.S_par = a.S_par; L.S = a.S;
+  L.S_pre = p->size(a.K - slos::kSuf);
   L.scale = a.scale; L.flags = a.flags;
   // children per wave: enough waves to fill the chip several times, but long ranges at scale
   uint64_t pw = (a.S / 16384 + 63) & ~63ull;
   L.per_wave = std::min<uint64_t>(std::max<uint64_t>(pw, 256), 4096);
   const uint64_t waves = (a.S + L.per_wave - 1) / L.per_wave;
-  const uint64_t wgs = (waves + 3) / 4;
+  const uint64_t wgs = (waves + slos::kWaves - 1) / slos::kWaves;
   if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
   if (a.B > 65535) return fail(SLOS_ERANGE, "batch > 65535");
+  const size_t sh = slos::block_lds_bytes(a.K, p->me);
+  if (sh + 10 * 1024 > 64 * 1024) return fail(SLOS_ERANGE, "block kernel LDS exceeds 64 KB");
   dim3 grid((unsigned)wgs, (unsigned)a.B);
   const int maxd = a.K - slos::kSuf;
-  if (kind == 0) comp ? launch_block_kind<0, true>(maxd, grid, st, t, L) : launch_block_kind<0, false>(maxd, grid, st, t, L);
-  else if (kind == 1) comp ? launch_block_kind<1, true>(maxd, grid, st, t, L) : launch_block_kind<1, false>(maxd, grid, st, t, L);
-  else comp ? launch_block_kind<2, true>(maxd, grid, st, t, L) : launch_block_kind<2, false>(maxd, grid, st, t, L);
+  if (kind == 0) comp ? launch_block_kind<0, true>(maxd, grid, sh, st, dt, p, a.par, a.out, L)
+                      : launch_block_kind<0, false>(maxd, grid, sh, st, dt, p, a.par, a.out, L);
+  else if (kind == 1) comp ? launch_block_kind<1, true>(maxd, grid, sh, st, dt, p, a.par, a.out, L)
+                           : launch_block_kind<1, false>(maxd, grid, sh, st, dt, p, a.par, a.out, L);
+  else comp ? launch_block_kind<2, true>(maxd, grid, sh, st, dt, p, a.par, a.out, L)
+            : launch_block_kind<2, false>(maxd, grid, sh, st, dt, p, a.par, a.out, L);
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -465,7 +469,7 @@ int launch_layer(const slos_plan* p, const DevTables& dt, int K, int col, const 
   a.U = U; a.cols = nullptr; a.par = par; a.S_par = S_par; a.out = out; a.S = S; a.B = B;
   a.scale = scale; a.flags = flags;
   const bool comp = flags & SLOS_FLAG_COMPENSATED;
-  if (!(flags & SLOS_FLAG_SIMPLE) && dt.tuples && K > slos::kSuf)
+  if (!(flags & SLOS_FLAG_SIMPLE) && dt.tuples && K > slos::kSuf && K - slos::kSuf <= 16)
     return launch_block(p, dt, a, kind, comp, st);
   const size_t sh = (((size_t)(K + 1) * (p->me + 1) * 8 + p->me * 4 + 15) & ~(size_t)15) + (size_t)p->m * 8;
   if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
@@ -559,8 +563,8 @@ int slos_plan_create(int n, int m, int mask, const int32_t* input_modes, slos_pl
       }
       p->cg[(size_t)r * st + x] = v;
     }
-  // the block kernel needs positions < 256 (8-bit fields) and (SUF-1)-tuple indices < 65536
-  if (n > slos::kSuf && p->m <= 256 && p->size(slos::kSuf - 1) < 65536 && p->size(slos::kSuf) <= (4u << 20))
+  // the block kernel needs modes < 255 (8-bit fields, 255 = no edge) and (SUF-1)-tuple indices < 65536
+  if (n > slos::kSuf && p->m <= 255 && p->size(slos::kSuf - 1) < 65536 && p->size(slos::kSuf) <= (4u << 20))
     build_tuples(p);
   if (p->size(n) == 0) {
     delete p;

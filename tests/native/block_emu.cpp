@@ -61,52 +61,62 @@ void layer_simple(const Space& s, int K, const cd* ucol, const std::vector<cd>& 
   }
 }
 
-// the block kernel's wave loop; writes[c] counts how often child c is written
+// the block kernel's wave loop: batches of 64 consecutive prefixes (one per lane), then the blocks
+// one at a time, 64 children per step; writes[c] counts how often child c is written
 template <int MAXD>
-void layer_block(const Space& s, int K, const cd* ucol, const std::vector<cd>& par, std::vector<cd>& out,
+void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const std::vector<cd>& par, std::vector<cd>& out,
                  std::vector<double>* norms, uint64_t per_wave, std::vector<int>& writes) {
   slos::BlockTables t{s.cg.data(), s.lo.data(), s.tuples.data(), s.me, s.me + 1, s.multiset};
   const int W = s.me + 1;
   const uint64_t S = s.cg[(size_t)K * W];
+  const int d = K - slos::kSuf;
+  const uint64_t S_pre = s.cg[(size_t)d * W];
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, W};
+  std::vector<cd> ucol(256, cd(0, 0));
+  for (size_t j = 0; j < ucol_in.size() && j < 255; ++j) ucol[j] = ucol_in[j];
   out.assign(S, 0);
   writes.assign(S, 0);
   if (norms) norms->assign(S, 0.0);
-  const int d = K - slos::kSuf;
   const uint64_t waves = (S + per_wave - 1) / per_wave;
   for (uint64_t w = 0; w < waves; ++w) {
     uint64_t g = w * per_wave;
     const uint64_t g_end = (g + per_wave < S) ? g + per_wave : S;
-    int q[MAXD];
-    uint64_t u0 = slos::unrank_prefix<MAXD>(t, K, d, g, q);
-    slos::BlockInfo bi;
-    slos::Streams<MAXD> st;
-    while (true) {
-      slos::block_info<MAXD>(t, K, d, q, bi, st);
-      const uint64_t avail = g_end - g;
-      const uint32_t u_stop = (uint32_t)(((uint64_t)bi.L - u0 < avail) ? (uint64_t)bi.L : u0 + avail);
-      for (uint32_t uc = (uint32_t)u0; uc < u_stop; uc += 64)
-        for (int lane = 0; lane < 64; ++lane) {
-          const uint32_t u = uc + lane;
-          if (u >= u_stop) continue;
-          cd acc = 0;
-          for (int si = 0; si < MAXD; ++si)
-            if (st.active[si]) acc += ucol[st.mode[si]] * par.at(st.base[si] + u);
-          const slos::TupleEntry e = t.tuples[bi.toff + u];
-          for (int j = 0; j < slos::kSuf; ++j) {
-            const int xj = (int)((e.pos >> (8 * j)) & 0xffu);
-            bool on = (e.flags >> j) & 1u;
-            if (j == 0) on = on && !(bi.skip0 && xj == bi.lo);
-            if (on) acc += ucol[xj] * par.at(bi.LBp + e.lr[j]);
+    int q0[MAXD];
+    uint64_t u0 = slos::unrank_prefix<MAXD>(t, K, d, g, q0);
+    uint64_t r = slos::prefix_rank<MAXD>(t, d, q0);
+    while (g < g_end) {
+      if (r >= S_pre) break;
+      const int nb = (int)((S_pre - r < 64) ? S_pre - r : 64);
+      for (int j = 0; j < nb && g < g_end; ++j) {
+        int q[MAXD];
+        sv.unrank(r + j, d, q);
+        slos::BlockInfo bi;
+        slos::Streams<MAXD> st;
+        slos::block_info<MAXD>(t, K, d, q, bi, st);
+        if (bi.L == 0) continue;
+        std::vector<cd> uw = ucol;
+        if (bi.skip0) uw[bi.lo] = 0;
+        const uint64_t avail = g_end - g;
+        const uint32_t u_stop = (uint32_t)(((uint64_t)bi.L - u0 < avail) ? (uint64_t)bi.L : u0 + avail);
+        for (uint32_t uc = (uint32_t)u0; uc < u_stop; uc += 64)
+          for (int lane = 0; lane < 64; ++lane) {
+            const uint32_t u = uc + lane;
+            if (u >= u_stop) continue;
+            cd acc = 0;
+            for (int si = 0; si < MAXD; ++si)
+              if (st.active[si]) acc += ucol[st.mode[si]] * par.at(st.base[si] + u);
+            const slos::TupleEntry e = t.tuples[bi.toff + u];
+            for (int jj = 0; jj < slos::kSuf; ++jj)
+              acc += uw[(e.pos >> (8 * jj)) & 0xffu] * par.at(bi.LBp + e.lr[jj]);
+            const uint64_t c = bi.CB + u;
+            out.at(c) = acc;
+            writes.at(c)++;
+            if (norms) (*norms)[c] = slos::child_norm(bi, e, t.multiset);
           }
-          const uint64_t c = bi.CB + u;
-          out.at(c) = acc;
-          writes.at(c)++;
-          if (norms) (*norms)[c] = slos::child_norm(bi, e, t.multiset);
-        }
-      g += u_stop - u0;
-      if (g >= g_end) break;
-      u0 = 0;
-      if (!slos::next_prefix<MAXD>(t, K, d, q)) break;
+        g += u_stop - u0;
+        u0 = 0;
+      }
+      r += 64;
     }
   }
 }
@@ -141,8 +151,8 @@ int emu_forward(int n, int m, int mask, const int* modes, const double* U, doubl
       }
     } else {
       const int d = K - slos::kSuf;
-      if (d <= 8) layer_block<8>(s, K, ucol.data(), par, out, last ? &norms : nullptr, per_wave, writes);
-      else layer_block<32>(s, K, ucol.data(), par, out, last ? &norms : nullptr, per_wave, writes);
+      if (d <= 8) layer_block<8>(s, K, ucol, par, out, last ? &norms : nullptr, per_wave, writes);
+      else layer_block<32>(s, K, ucol, par, out, last ? &norms : nullptr, per_wave, writes);
       for (int w : writes)
         if (w != 1) return 2;
     }
