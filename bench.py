@@ -16,6 +16,7 @@ extrapolated linearly in layer-11 states).
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import math
 import os
@@ -44,11 +45,16 @@ def bytes_alg(n: int, m: int) -> int:
 
 
 def load_traffic(tag: str):
+    """HBM bytes per launch of the last-layer kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    and scaled by a plain-copy calibration; used only if the kernel sources are the ones measured."""
     path = os.path.join(ROOT, "profiles", f"layer_block_{tag}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh)
+        rec = json.load(fh)
+    src = b"".join(open(os.path.join(ROOT, "photonic_engines_amd", "csrc", p), "rb").read()
+                   for p in ("layer_block.hpp", "block_index.hpp"))
+    return rec if hashlib.sha256(src).hexdigest() == rec.get("source_sha256") else None
 
 
 def cpu_model() -> str:
