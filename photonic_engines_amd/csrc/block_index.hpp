@@ -249,19 +249,24 @@ inline TupleEntry make_tuple_entry(const SpaceView& sv, int multiset, uint64_t u
   return e;
 }
 
+static_assert(kSuf <= 4, "lead_binom covers runs of up to 4 copies");
+
+// binom(r + lead, lead) for lead <= kSuf, exact in integers (each step divides exactly)
+SLOS_HD float lead_binom(int r, int lead) {
+  uint32_t p = (uint32_t)r + 1u;
+  const uint32_t p2 = p * ((uint32_t)r + 2u) / 2u;
+  const uint32_t p3 = p2 * ((uint32_t)r + 3u) / 3u;
+  const uint32_t p4 = p3 * ((uint32_t)r + 4u) / 4u;
+  p = lead >= 4 ? p4 : lead == 3 ? p3 : lead == 2 ? p2 : lead == 1 ? p : 1u;
+  return (float)p;
+}
+
 // prod t_j! of the child (Q, tau) from the block's run data and the tuple entry
 SLOS_HD float child_norm(const BlockInfo& bi, const TupleEntry& e, int multiset) {
   if (!multiset) return 1.f;
   float f = bi.fQ * (float)(e.flags >> 7);
-  const int x0 = (int)(e.pos & 0xffu);
-  if (bi.skip0 && x0 == bi.lo) {
-    // tau's leading run continues the last run of Q: (r + c)! = r! c! binom(r + c, c)
-    const int lead = (int)((e.flags >> 4) & 7u);
-    const float r = (float)bi.lastrun;
-    float b = 1.f;
-    for (int i = 1; i <= lead; ++i) b = b * (r + (float)i) / (float)i;
-    f *= b;
-  }
+  // tau's leading run continues the last run of Q: (r + c)! = r! c! binom(r + c, c)
+  if (bi.skip0 && (int)(e.pos & 0xffu) == bi.lo) f *= lead_binom(bi.lastrun, (int)((e.flags >> 4) & 7u));
   return f;
 }
 

@@ -78,6 +78,82 @@ __device__ __forceinline__ void bk_cmac(BlockAcc& a, float2 u, float2 v) {
   }
 }
 
+// per-block state of the chunk sweep (all wave-uniform except `lane`)
+template <int MAXD>
+struct ChunkCtx {
+  const float2* const (&sp)[MAXD];  // prefix-stream bases (parent of child u at sp[s] + u)
+  const float2 (&coef)[MAXD];       // their U coefficients (zero for inactive slots)
+  const float2* lp;                 // local parents: lp + lr_j
+  const TupleEntry* tb;             // tuple entries of the block's children
+  char* ob;                         // output of the block's first child
+  const float2* uw;                 // U column with this block's skip0 zero
+  uint32_t u_stop;
+  int lane;
+  float fQ;
+  int lastrun, skip0, lo, multiset;
+  float scale;
+  unsigned flags;
+};
+
+// NC consecutive 64-child chunks starting at uc: every load of the NC chunks is issued before any
+// result is needed. Lanes past u_stop re-read the last valid child and store nothing.
+template <int KIND, bool COMP, int MAXD, int NC>
+__device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
+  uint32_t u[NC];
+  bool in[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const uint32_t ur = uc + 64u * c + (uint32_t)cx.lane;
+    in[c] = ur < cx.u_stop;
+    u[c] = in[c] ? ur : cx.u_stop - 1u;
+  }
+  uint4 te[NC];  // TupleEntry as {pos, flags | pad << 16, lr0 | lr1 << 16, lr2 | lr3 << 16}
+#pragma unroll
+  for (int c = 0; c < NC; ++c) te[c] = at_bytes(reinterpret_cast<const uint4*>(cx.tb), u[c] * 16u);
+  float2 sv[NC][MAXD];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int s = 0; s < MAXD; ++s) sv[c][s] = at_bytes(cx.sp[s], u[c] * 8u);
+  float2 lv[NC][kSuf];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    lv[c][0] = at_bytes(cx.lp, (te[c].z & 0xffffu) * 8u);
+    lv[c][1] = at_bytes(cx.lp, (te[c].z >> 16) * 8u);
+    lv[c][2] = at_bytes(cx.lp, (te[c].w & 0xffffu) * 8u);
+    lv[c][3] = at_bytes(cx.lp, (te[c].w >> 16) * 8u);
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    BlockAcc acc{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < MAXD; ++s) bk_cmac<COMP>(acc, cx.coef[s], sv[c][s]);
+#pragma unroll
+    for (int jj = 0; jj < kSuf; ++jj) bk_cmac<COMP>(acc, cx.uw[(te[c].x >> (8 * jj)) & 0xffu], lv[c][jj]);
+    float re = acc.re + acc.cre, im = acc.im + acc.cim;
+    if constexpr (KIND == 0) {
+      if (in[c]) *reinterpret_cast<float2*>(cx.ob + u[c] * 8u) = make_float2(re, im);
+    } else {
+      // prod t_j! of the child: prefix runs, tau's runs, and the join of tau's leading run with
+      // the prefix's last run ((r + c)! = r! c! binom(r + c, c))
+      float f = 1.f;
+      if (cx.multiset) {
+        const uint32_t flags = te[c].y & 0xffffu;
+        f = cx.fQ * (float)(flags >> 7);
+        if (cx.skip0 && (int)(te[c].x & 0xffu) == cx.lo) f *= lead_binom(cx.lastrun, (int)((flags >> 4) & 7u));
+      }
+      re *= cx.scale;
+      im *= cx.scale;
+      if constexpr (KIND == 1) {
+        if (in[c]) *reinterpret_cast<float*>(cx.ob + u[c] * 4u) = (re * re + im * im) * f;
+      } else {
+        const float s = (cx.flags & 2u) ? sqrtf(f) : 1.f;
+        if (in[c]) *reinterpret_cast<float2*>(cx.ob + u[c] * 8u) = make_float2(re * s, im * s);
+      }
+    }
+  }
+}
+
 // LDS bytes: counting-table rows 0..K and lo[] (per-lane block data is computed from them)
 inline size_t block_lds_bytes(int K, int me) { return (size_t)(K + 1) * (me + 1) * 8 + (size_t)me * 4; }
 
@@ -199,42 +275,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_layer_block(const uint64_t* _
       char* ob = reinterpret_cast<char*>(out_all) + ((size_t)b * a.S + CB) * (KIND == 1 ? 4 : 8);
       const uint64_t avail = g_end - g;
       const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
-      for (uint32_t uc = (uint32_t)u0; uc < u_stop; uc += 64) {
-        const uint32_t u_raw = uc + lane;
-        const bool in = u_raw < u_stop;
-        const uint32_t u = in ? u_raw : u_stop - 1;  // tail lanes re-read a valid child, store nothing
-        float2 sv[MAXD];
-#pragma unroll
-        for (int s = 0; s < MAXD; ++s) sv[s] = at_bytes(sp[s], u * 8u);
-        const TupleEntry e = at_bytes(tb, u * 16u);
-        float2 lv[kSuf];
-#pragma unroll
-        for (int jj = 0; jj < kSuf; ++jj) lv[jj] = at_bytes(lp, (uint32_t)e.lr[jj] * 8u);  // real parents
-        BlockAcc acc{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < MAXD; ++s) bk_cmac<COMP>(acc, coef[s], sv[s]);
-#pragma unroll
-        for (int jj = 0; jj < kSuf; ++jj) bk_cmac<COMP>(acc, uw[(e.pos >> (8 * jj)) & 0xffu], lv[jj]);
-        float re = acc.re + acc.cre, im = acc.im + acc.cim;
-        if constexpr (KIND == 0) {
-          if (in) *reinterpret_cast<float2*>(ob + u * 8u) = make_float2(re, im);
-        } else {
-          BlockInfo cb;
-          cb.fQ = fQ;
-          cb.lastrun = lastrun;
-          cb.skip0 = skip0;
-          cb.lo = blo;
-          const float f = child_norm(cb, e, multiset);
-          re *= a.scale;
-          im *= a.scale;
-          if constexpr (KIND == 1) {
-            if (in) *reinterpret_cast<float*>(ob + u * 4u) = (re * re + im * im) * f;
-          } else {
-            const float s = (a.flags & 2u) ? sqrtf(f) : 1.f;
-            if (in) *reinterpret_cast<float2*>(ob + u * 8u) = make_float2(re * s, im * s);
-          }
-        }
-      }
+      ChunkCtx<MAXD> cx{sp, coef, lp, tb, ob, uw, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale, a.flags};
+      uint32_t uc = (uint32_t)u0;
+      for (; uc + 64u < u_stop; uc += 128u) sweep<KIND, COMP, MAXD, 2>(cx, uc);  // two chunks in flight
+      if (uc < u_stop) sweep<KIND, COMP, MAXD, 1>(cx, uc);
       g += u_stop - u0;
       u0 = 0;
     }
