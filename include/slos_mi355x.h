@@ -32,6 +32,7 @@ extern "C" {
 #endif
 
 typedef struct slos_plan slos_plan;
+typedef struct slos_group slos_group;
 typedef void* slos_stream_t; /* hipStream_t */
 
 /* output masks: photonic_engines/masks.py:18-28 */
@@ -162,6 +163,38 @@ int slos_total_mass(const float* probs, int64_t batch, uint64_t count, double* o
  * (Gray-code Ryser), for accuracy reports at sizes where no full FP64 run exists. */
 int slos_spot_probs(const slos_plan* plan, const void* U_c128, const uint64_t* idx, int64_t count,
                     double* out, slos_stream_t stream);
+
+/* ---- uneven batches ------------------------------------------------------------------------
+ * A group runs many instances of up to 8 plans of different (n, m, input, mask) together: one
+ * launch per photon layer K covers every instance of every plan with n >= K (batched U, as
+ * kernel.py:208-215 batches one shape, widened to several shapes). The reference has no
+ * counterpart for mixed shapes; each plan's output equals slos_forward's (SLOS_OUT_PROBS) for the
+ * same U, bit for bit. Per plan g: U[g] complex64 [batch[g]][m][m] (device), out[g] float32
+ * [batch[g]][S_n] (device), scale[g] the input norm 1/sqrt(prod s_i!). Plans must outlive the group. */
+int slos_group_create(int count, const slos_plan* const* plans, slos_group** out);
+int slos_group_destroy(slos_group* group);
+int slos_group_workspace(const slos_group* group, const int64_t* batch, size_t* bytes);
+int slos_group_forward(slos_group* group, const void* const* U, const int64_t* batch, float* const* out,
+                       const double* scale, unsigned flags, void* workspace, size_t workspace_bytes,
+                       slos_stream_t stream);
+
+/* Autotuner: times candidate launches of every layer on these buffers (block or thread-per-state
+ * kernel, children per wave / states per thread, persistent grid size), `reps` timed launches
+ * each, and keeps the fastest per layer for later forwards and captures. Synchronises `stream`.
+ * Leaves the full forward result in `out`. */
+int slos_group_tune(slos_group* group, const void* const* U, const int64_t* batch, float* const* out,
+                    const double* scale, unsigned flags, void* workspace, size_t workspace_bytes, int reps,
+                    slos_stream_t stream);
+/* The launch chosen for layer K: block (1 block kernel, 0 thread-per-state, -1 default), per_wave,
+ * grid (0: one work group per item). */
+int slos_group_config(const slos_group* group, int K, int* block, uint64_t* per_wave, uint32_t* grid);
+
+/* Capture one forward over these buffers as a hipGraph (on an internal stream; nothing is
+ * allocated or memset inside the graph), then replay it on any stream. The buffers must stay
+ * allocated while the graph is replayed; new inputs are written into the same U buffers. */
+int slos_group_capture(slos_group* group, const void* const* U, const int64_t* batch, float* const* out,
+                       const double* scale, unsigned flags, void* workspace, size_t workspace_bytes);
+int slos_group_replay(slos_group* group, slos_stream_t stream);
 
 #ifdef __cplusplus
 }

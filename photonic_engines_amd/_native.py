@@ -46,6 +46,17 @@ SIGNATURES = {
     "slos_mass_workspace": (_c.c_int, [_c.c_int64, _c.c_uint64, _c.POINTER(_c.c_size_t)]),
     "slos_total_mass": (_c.c_int, [_vp, _c.c_int64, _c.c_uint64, _vp, _vp, _c.c_size_t, _vp]),
     "slos_spot_probs": (_c.c_int, [_vp, _vp, _vp, _c.c_int64, _vp, _vp]),
+    "slos_group_create": (_c.c_int, [_c.c_int, _c.POINTER(_vp), _c.POINTER(_vp)]),
+    "slos_group_destroy": (_c.c_int, [_vp]),
+    "slos_group_workspace": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_size_t)]),
+    "slos_group_forward": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_c.c_int64), _c.POINTER(_vp), _f64p,
+                                      _c.c_uint, _vp, _c.c_size_t, _vp]),
+    "slos_group_tune": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_c.c_int64), _c.POINTER(_vp), _f64p,
+                                   _c.c_uint, _vp, _c.c_size_t, _c.c_int, _vp]),
+    "slos_group_config": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_int), _u64p, _c.POINTER(_c.c_uint32)]),
+    "slos_group_capture": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_c.c_int64), _c.POINTER(_vp), _f64p,
+                                      _c.c_uint, _vp, _c.c_size_t]),
+    "slos_group_replay": (_c.c_int, [_vp, _vp]),
 }
 
 _lib = None

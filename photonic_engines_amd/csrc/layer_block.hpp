@@ -93,6 +93,7 @@ struct ChunkCtx {
   int lastrun, skip0, lo, multiset;
   float scale;
   unsigned flags;
+  int kind;  // KIND 3 only: 1 probabilities, 0 raw amplitudes
 };
 
 // NC consecutive 64-child chunks starting at uc: every load of the NC chunks is issued before any
@@ -131,7 +132,7 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
 #pragma unroll
     for (int jj = 0; jj < kSuf; ++jj) bk_cmac<COMP>(acc, cx.uw[(te[c].x >> (8 * jj)) & 0xffu], lv[c][jj]);
     float re = acc.re + acc.cre, im = acc.im + acc.cim;
-    if constexpr (KIND == 0) {
+    if (KIND == 0 || (KIND == 3 && cx.kind == 0)) {
       if (in[c]) *reinterpret_cast<float2*>(cx.ob + u[c] * 8u) = make_float2(re, im);
     } else {
       // prod t_j! of the child: prefix runs, tau's runs, and the join of tau's leading run with
@@ -144,7 +145,7 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
       }
       re *= cx.scale;
       im *= cx.scale;
-      if constexpr (KIND == 1) {
+      if constexpr (KIND == 1 || KIND == 3) {
         if (in[c]) *reinterpret_cast<float*>(cx.ob + u[c] * 4u) = (re * re + im * im) * f;
       } else {
         const float s = (cx.flags & 2u) ? sqrtf(f) : 1.f;
@@ -157,42 +158,24 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
 // LDS bytes: counting-table rows 0..K and lo[] (per-lane block data is computed from them)
 inline size_t block_lds_bytes(int K, int me) { return (size_t)(K + 1) * (me + 1) * 8 + (size_t)me * 4; }
 
-// KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes
+// One wave's share of a layer slice: the children [g, g + a.per_wave) of one instance, whose parent
+// layer is `par` and whose output layer starts at `out`. `ucol` is the U column in LDS (entries >= m
+// zero), `uw` this wave's private copy of it, `tl` the counting tables in LDS, `tg` the same tables
+// in global memory (for the wave-uniform scalar unrank of g).
+// KIND 3 decides between probabilities (kind 1) and raw amplitudes (kind 0) at run time.
 template <int KIND, bool COMP, int MAXD>
-__global__ __launch_bounds__(kBlockThreads) void k_layer_block(const uint64_t* __restrict__ cg_t,
-                                                               const int* __restrict__ lo_t,
-                                                               const TupleEntry* __restrict__ tuples,
-                                                               int me, int stride, int multiset,
-                                                               const float2* __restrict__ par_all,
-                                                               void* __restrict__ out_all, BlockLayer a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ float2 ucol[256];            // U[b, :, col]; entries >= m are zero (kNoEdge = 255)
-  __shared__ float2 ucol_w[kWaves][256];  // per-wave copy with the block's own zero at lo (skip0)
-  const int K = a.K;
-  const int ncg = (K + 1) * stride;
-  uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
-  int* lo_l = reinterpret_cast<int*>(cg_l + ncg);
-  const int64_t b = blockIdx.y;
-  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = cg_t[i];
-  for (int i = threadIdx.x; i < me; i += blockDim.x) lo_l[i] = lo_t[i];
-  for (int j = threadIdx.x; j < 256; j += blockDim.x) {
-    const float2 v = j < a.m ? a.U[(size_t)b * a.m * a.m + (size_t)j * a.m + a.col] : make_float2(0.f, 0.f);
-    ucol[j] = v;
-    for (int w = 0; w < kWaves; ++w) ucol_w[w][j] = v;
-  }
-  __syncthreads();
-  const BlockTables tg{cg_t, lo_t, tuples, me, stride, multiset};  // global: scalar loads
-  const BlockTables tl{cg_l, lo_l, tuples, me, stride, multiset};  // LDS: per-lane lookups
-
+__device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTables& tl, const float2* __restrict__ par,
+                                           void* __restrict__ out, const float2* ucol, float2* uw,
+                                           const BlockLayer& a, uint64_t g, int kind = 0) {
   const int lane = threadIdx.x & 63;
-  const int wv = (int)rfl(threadIdx.x >> 6);
-  float2* uw = ucol_w[wv];
-  const uint32_t wg_l = xcd_remap(blockIdx.x, gridDim.x);
-  uint64_t g = uniform_u64(((uint64_t)wg_l * kWaves + wv) * a.per_wave);
+  const int K = a.K;
+  const int me = tg.me, stride = tg.stride, multiset = tg.multiset;
+  const uint64_t* cg_l = tl.cg;
+  const int* lo_l = tl.lo;
+  const TupleEntry* tuples = tg.tuples;
   if (g >= a.S) return;
   const uint64_t g_end = (g + a.per_wave < a.S) ? g + a.per_wave : a.S;
   const int d = K - kSuf;
-  const float2* par = par_all + (size_t)b * a.S_par;
 
   // the block holding g: its prefix rank r and g's offset inside it (scalar path)
   uint64_t r, u0;
@@ -272,10 +255,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_layer_block(const uint64_t* _
       }
       const float2* lp = par + LBp;
       const TupleEntry* tb = tuples + toff;
-      char* ob = reinterpret_cast<char*>(out_all) + ((size_t)b * a.S + CB) * (KIND == 1 ? 4 : 8);
+      const bool probs = KIND == 1 || (KIND == 3 && kind == 1);
+      char* ob = reinterpret_cast<char*>(out) + CB * (probs ? 4 : 8);
       const uint64_t avail = g_end - g;
       const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
-      ChunkCtx<MAXD> cx{sp, coef, lp, tb, ob, uw, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale, a.flags};
+      ChunkCtx<MAXD> cx{sp, coef, lp, tb, ob, uw, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale, a.flags, kind};
       uint32_t uc = (uint32_t)u0;
       for (; uc + 64u < u_stop; uc += 128u) sweep<KIND, COMP, MAXD, 2>(cx, uc);  // two chunks in flight
       if (uc < u_stop) sweep<KIND, COMP, MAXD, 1>(cx, uc);
@@ -284,6 +268,134 @@ __global__ __launch_bounds__(kBlockThreads) void k_layer_block(const uint64_t* _
     }
     r += 64;
   }
+}
+
+
+// KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes
+template <int KIND, bool COMP, int MAXD>
+__global__ __launch_bounds__(kBlockThreads) void k_layer_block(const uint64_t* __restrict__ cg_t,
+                                                               const int* __restrict__ lo_t,
+                                                               const TupleEntry* __restrict__ tuples,
+                                                               int me, int stride, int multiset,
+                                                               const float2* __restrict__ par_all,
+                                                               void* __restrict__ out_all, BlockLayer a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ float2 ucol[256];            // U[b, :, col]; entries >= m are zero (kNoEdge = 255)
+  __shared__ float2 ucol_w[kWaves][256];  // per-wave copy with the block's own zero at lo (skip0)
+  const int K = a.K;
+  const int ncg = (K + 1) * stride;
+  uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
+  int* lo_l = reinterpret_cast<int*>(cg_l + ncg);
+  const int64_t b = blockIdx.y;
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = cg_t[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo_l[i] = lo_t[i];
+  for (int j = threadIdx.x; j < 256; j += blockDim.x) {
+    const float2 v = j < a.m ? a.U[(size_t)b * a.m * a.m + (size_t)j * a.m + a.col] : make_float2(0.f, 0.f);
+    ucol[j] = v;
+    for (int w = 0; w < kWaves; ++w) ucol_w[w][j] = v;
+  }
+  __syncthreads();
+  const BlockTables tg{cg_t, lo_t, tuples, me, stride, multiset};  // global: scalar loads
+  const BlockTables tl{cg_l, lo_l, tuples, me, stride, multiset};  // LDS: per-lane lookups
+
+  const int wv = (int)rfl(threadIdx.x >> 6);
+  const uint32_t wg_l = xcd_remap(blockIdx.x, gridDim.x);
+  const uint64_t g = uniform_u64(((uint64_t)wg_l * kWaves + wv) * a.per_wave);
+  char* out_b = reinterpret_cast<char*>(out_all) + (size_t)b * a.S * (KIND == 1 ? 4 : 8);
+  block_wave<KIND, COMP, MAXD>(tg, tl, par_all + (size_t)b * a.S_par, out_b, ucol, ucol_w[wv], a, g);
+}
+
+// ------------------------------------------------------------------------------------------
+// Uneven batches: one launch per photon layer K covers every instance of every plan in a group
+// (plans of different n, m, input and mask). The work is cut into work-group items, each inside one
+// (plan, instance): plan g owns items [wg_begin, wg_begin + B * wgs_per_inst). The grid is
+// persistent: XCD x walks the contiguous item range [x T/8, (x+1) T/8) with its work groups in
+// turn, so each XCD's L2 serves neighbouring items.
+// ------------------------------------------------------------------------------------------
+
+constexpr int kMaxGroup = 8;
+
+struct GroupItem {
+  const uint64_t* cg;          // plan tables (device)
+  const int* lo;
+  const TupleEntry* tuples;
+  const float2* U;             // [B][m][m]
+  const float2* par;           // [B][S_par]
+  void* out;                   // [B][S]: float32 probabilities (kind 1) or complex64 (kind 0)
+  uint64_t S_par, S, S_pre;
+  uint64_t per_wave;           // block kernel: children per wave; simple kernel: states per thread
+  float scale;
+  unsigned flags;
+  int me, multiset, m, col, kind;
+  uint32_t wg_begin, wgs_per_inst;
+};
+
+struct GroupLaunch {
+  GroupItem it[kMaxGroup];
+  int count, K;
+  uint32_t total;              // items of the launch
+};
+
+// the plan of work-group index `item` (plans in ascending wg_begin)
+__device__ __forceinline__ int group_index(const GroupLaunch& gl, uint32_t item) {
+  int gi = 0;
+#pragma unroll
+  for (int i = 1; i < kMaxGroup; ++i)
+    if (i < gl.count && item >= gl.it[i].wg_begin) gi = i;
+  return (int)rfl((uint32_t)gi);
+}
+
+// persistent XCD-contiguous walk over [0, total): calls f(item) for this work group's items
+template <class F>
+__device__ __forceinline__ void for_each_item(uint32_t total, F&& f) {
+  const uint32_t nx = gridDim.x >> 3;  // work groups per XCD (the grid is a multiple of 8)
+  const uint32_t per_xcd = (total + 7u) >> 3;
+  const uint32_t xcd = blockIdx.x & 7u;
+  for (uint32_t t = blockIdx.x >> 3; t < per_xcd; t += nx) {
+    const uint32_t item = xcd * per_xcd + t;
+    if (item >= total) break;
+    f(item);
+  }
+}
+
+template <bool COMP, int MAXD>
+__global__ __launch_bounds__(kBlockThreads) void k_layer_block_group(GroupLaunch gl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ float2 ucol[256];
+  __shared__ float2 ucol_w[kWaves][256];
+  const int K = gl.K;
+  const int wv = (int)rfl(threadIdx.x >> 6);
+  const uint64_t* cur = nullptr;  // plan whose tables are in LDS
+  for_each_item(gl.total, [&](uint32_t item) {
+    const GroupItem& it = gl.it[group_index(gl, item)];
+    const uint32_t rel = item - it.wg_begin;
+    const uint32_t b = rel / it.wgs_per_inst, w = rel - b * it.wgs_per_inst;
+    const int stride = it.me + 1;
+    const int ncg = (K + 1) * stride;
+    uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
+    int* lo_l = reinterpret_cast<int*>(cg_l + ncg);
+    __syncthreads();  // every wave is done with the previous item's LDS
+    if (it.cg != cur) {
+      for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = it.cg[i];
+      for (int i = threadIdx.x; i < it.me; i += blockDim.x) lo_l[i] = it.lo[i];
+      cur = it.cg;
+    }
+    for (int j = threadIdx.x; j < 256; j += blockDim.x) {
+      const float2 v = j < it.m ? it.U[(size_t)b * it.m * it.m + (size_t)j * it.m + it.col] : make_float2(0.f, 0.f);
+      ucol[j] = v;
+      for (int q = 0; q < kWaves; ++q) ucol_w[q][j] = v;
+    }
+    __syncthreads();
+    const BlockTables tg{it.cg, it.lo, it.tuples, it.me, stride, it.multiset};
+    const BlockTables tl{cg_l, lo_l, it.tuples, it.me, stride, it.multiset};
+    BlockLayer a;
+    a.K = K; a.m = it.m; a.col = it.col; a.U = it.U; a.S_par = it.S_par; a.S = it.S; a.S_pre = it.S_pre;
+    a.scale = it.scale; a.flags = it.flags; a.per_wave = it.per_wave;
+    const uint64_t g = uniform_u64(((uint64_t)w * kWaves + wv) * it.per_wave);
+    const float2* par = it.par + (size_t)b * it.S_par;
+    char* out = reinterpret_cast<char*>(it.out) + (size_t)b * it.S * (it.kind == 1 ? 4 : 8);
+    block_wave<3, COMP, MAXD>(tg, tl, par, out, ucol, ucol_w[wv], a, g, it.kind);
+  });
 }
 
 }  // namespace slos

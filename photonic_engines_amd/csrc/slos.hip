@@ -430,7 +430,12 @@ void launch_block_kind(int maxd, dim3 grid, size_t sh, hipStream_t st, const Dev
 #define SLOS_BLK(D)                                                                                          \
   hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, D>), grid, dim3(slos::kBlockThreads), sh, st, dt.cg, dt.lo, \
                      dt.tuples, p->me, p->me + 1, p->multiset, par, out, L)
-  if (maxd <= 8) SLOS_BLK(8);
+  // one stream slot per prefix position: exact widths where the big layers live (d = K - 4)
+  if (maxd <= 4) SLOS_BLK(4);
+  else if (maxd == 5) SLOS_BLK(5);
+  else if (maxd == 6) SLOS_BLK(6);
+  else if (maxd == 7) SLOS_BLK(7);
+  else if (maxd == 8) SLOS_BLK(8);
   else SLOS_BLK(16);
 #undef SLOS_BLK
 }
@@ -504,6 +509,253 @@ int forward_layout(const slos_plan* p, int64_t B, int layer, WsLayout* w) {
   w->even = w->odd + align256(a);
   w->total = w->even + align256(b);
   return SLOS_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// uneven batches: a group of plans, one launch per photon layer (see layer_block.hpp)
+// ------------------------------------------------------------------------------------------
+
+// thread-per-state form of a group layer: item = (plan, instance, 256 * spt consecutive states)
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_layer_simple_group(slos::GroupLaunch gl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int K = gl.K;
+  const uint64_t* cur = nullptr;
+  slos::for_each_item(gl.total, [&](uint32_t item) {
+    const slos::GroupItem& it = gl.it[slos::group_index(gl, item)];
+    const uint32_t rel = item - it.wg_begin;
+    const uint32_t b = rel / it.wgs_per_inst, w = rel - b * it.wgs_per_inst;
+    const int stride = it.me + 1;
+    const int ncg = (K + 1) * stride;
+    uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
+    int* lo = reinterpret_cast<int*>(cg + ncg);
+    float2* ucol = reinterpret_cast<float2*>(smem + ((ncg * 8 + it.me * 4 + 15) & ~15));
+    __syncthreads();  // the previous item is done with LDS
+    if (it.cg != cur) {
+      for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = it.cg[i];
+      for (int i = threadIdx.x; i < it.me; i += blockDim.x) lo[i] = it.lo[i];
+      cur = it.cg;
+    }
+    for (int j = threadIdx.x; j < it.m; j += blockDim.x)
+      ucol[j] = it.U[(size_t)b * it.m * it.m + (size_t)j * it.m + it.col];
+    __syncthreads();
+    const slos::SpaceView sv{cg, lo, it.me, stride};
+    const float2* pb = it.par + (size_t)b * it.S_par;
+    const uint32_t spt = (uint32_t)it.per_wave;
+    const uint64_t g0 = (uint64_t)w * 256u * spt + threadIdx.x;
+    for (uint32_t i = 0; i < spt; ++i) {
+      const uint64_t g = g0 + 256ull * i;
+      if (g >= it.S) break;
+      Acc acc;
+      slos::for_each_parent(sv, it.multiset, K, g,
+                            [&](uint64_t pr, int mode, int) { cmac<COMP>(acc, ucol[mode], pb[pr]); });
+      float re = acc.re + acc.cre, im = acc.im + acc.cim;
+      if (it.kind == 1) {
+        int pos[kMaxPhotons];
+        sv.unrank(g, K, pos);
+        const float f = slos::run_factorial(pos, K, it.multiset);
+        re *= it.scale;
+        im *= it.scale;
+        reinterpret_cast<float*>(it.out)[(size_t)b * it.S + g] = (re * re + im * im) * f;
+      } else {
+        reinterpret_cast<float2*>(it.out)[(size_t)b * it.S + g] = make_float2(re, im);
+      }
+    }
+  });
+}
+
+size_t simple_lds_bytes(int K, int me, int m) {
+  return (((size_t)(K + 1) * (me + 1) * 8 + (size_t)me * 4 + 15) & ~(size_t)15) + (size_t)m * 8;
+}
+
+}  // namespace
+
+// a group of plans run together (uneven batches)
+struct slos_group {
+  struct Cfg {
+    int block = -1;         // 1 block kernel, 0 thread-per-state, -1 default choice
+    uint64_t per_wave = 0;  // children per wave (block) / states per thread (simple); 0: default
+    uint32_t grid = 0;      // work groups of the persistent launch; 0: one per item
+  };
+  std::vector<const slos_plan*> plans;
+  int nmax = 0;
+  std::vector<Cfg> cfg;     // [nmax + 1]
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int cap_device = -1;
+};
+
+namespace {
+
+struct GroupArgs {
+  std::vector<const float2*> U;
+  std::vector<int64_t> B;
+  std::vector<float*> out;
+  std::vector<float> scale;
+  unsigned flags = 0;
+  unsigned char* ws = nullptr;
+  size_t ws_bytes = 0;
+};
+
+struct GroupWs {
+  std::vector<size_t> ones, buf0, buf1;
+  size_t total = 0;
+};
+
+int group_layout(const slos_group* G, const int64_t* B, GroupWs* w) {
+  size_t off = 0;
+  for (size_t g = 0; g < G->plans.size(); ++g) {
+    const slos_plan* p = G->plans[g];
+    if (B[g] < 1 || B[g] > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
+    uint64_t maxS = 0, bytes = 0;
+    for (int k = 1; k < p->n; ++k) maxS = std::max(maxS, p->size(k));
+    if (mul_overflow(maxS, (uint64_t)B[g] * 8, &bytes)) return fail(SLOS_ERANGE, "workspace size overflows");
+    w->ones.push_back(off);
+    off += align256((size_t)B[g] * 8);
+    w->buf0.push_back(off);
+    off += align256(bytes);
+    w->buf1.push_back(off);
+    off += align256(bytes);
+  }
+  w->total = off;
+  return SLOS_OK;
+}
+
+int group_args(const slos_group* G, const void* const* U, const int64_t* B, float* const* out, const double* scale,
+               unsigned flags, void* ws, size_t ws_bytes, GroupArgs* a, GroupWs* w) {
+  if (!G || !U || !B || !out || !scale) return fail(SLOS_EINVAL, "NULL argument");
+  int rc = group_layout(G, B, w);
+  if (rc) return rc;
+  if (!ws || ws_bytes < w->total)
+    return fail(SLOS_ENOMEM, "workspace too small: need " + std::to_string(w->total) + " bytes");
+  for (size_t g = 0; g < G->plans.size(); ++g) {
+    if (!U[g] || !out[g]) return fail(SLOS_EINVAL, "NULL unitary or output");
+    a->U.push_back(static_cast<const float2*>(U[g]));
+    a->B.push_back(B[g]);
+    a->out.push_back(out[g]);
+    a->scale.push_back((float)scale[g]);
+  }
+  a->flags = flags;
+  a->ws = static_cast<unsigned char*>(ws);
+  a->ws_bytes = ws_bytes;
+  return SLOS_OK;
+}
+
+bool group_can_block(const slos_group* G, const std::vector<DevTables>& dts, int K, unsigned flags) {
+  if ((flags & SLOS_FLAG_SIMPLE) || K <= slos::kSuf || K - slos::kSuf > 16) return false;
+  for (size_t g = 0; g < G->plans.size(); ++g)
+    if (G->plans[g]->n >= K && !dts[g].tuples) return false;
+  return true;
+}
+
+template <bool COMP>
+void launch_block_group(int maxd, uint32_t grid, size_t sh, hipStream_t st, const slos::GroupLaunch& gl) {
+#define SLOS_BLKG(D) hipLaunchKernelGGL((slos::k_layer_block_group<COMP, D>), dim3(grid), dim3(slos::kBlockThreads), sh, st, gl)
+  if (maxd <= 4) SLOS_BLKG(4);
+  else if (maxd == 5) SLOS_BLKG(5);
+  else if (maxd == 6) SLOS_BLKG(6);
+  else if (maxd == 7) SLOS_BLKG(7);
+  else if (maxd == 8) SLOS_BLKG(8);
+  else SLOS_BLKG(16);
+#undef SLOS_BLKG
+}
+
+// layer K of every plan with n >= K, in one launch
+int group_layer(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w, const std::vector<DevTables>& dts,
+                const slos_group::Cfg& c, hipStream_t st) {
+  slos::GroupLaunch gl{};
+  gl.K = K;
+  const bool block = group_can_block(G, dts, K, a.flags) && c.block != 0;
+  uint64_t children = 0;
+  for (size_t g = 0; g < G->plans.size(); ++g)
+    if (G->plans[g]->n >= K) children += G->plans[g]->size(K) * (uint64_t)a.B[g];
+  uint64_t pw_default = std::min<uint64_t>(std::max<uint64_t>((children / 16384 + 63) & ~63ull, 256), 4096);
+  size_t sh = 0;
+  uint64_t total = 0;
+  int cnt = 0;
+  for (size_t g = 0; g < G->plans.size(); ++g) {
+    const slos_plan* p = G->plans[g];
+    if (p->n < K) continue;
+    slos::GroupItem& it = gl.it[cnt++];
+    const DevTables& dt = dts[g];
+    it.cg = dt.cg;
+    it.lo = dt.lo;
+    it.tuples = dt.tuples;
+    it.U = a.U[g];
+    const float2* bufs[2] = {reinterpret_cast<const float2*>(a.ws + w.buf0[g]),
+                             reinterpret_cast<const float2*>(a.ws + w.buf1[g])};
+    it.par = K == 1 ? reinterpret_cast<const float2*>(a.ws + w.ones[g]) : bufs[(K - 1) & 1];
+    const bool last = K == p->n;
+    it.out = last ? static_cast<void*>(a.out[g]) : const_cast<float2*>(bufs[K & 1]);
+    it.S_par = p->size(K - 1);
+    it.S = p->size(K);
+    it.S_pre = block ? p->size(K - slos::kSuf) : 0;
+    it.kind = last ? 1 : 0;
+    it.scale = last ? a.scale[g] : 1.f;
+    it.flags = a.flags;
+    it.me = p->me;
+    it.multiset = p->multiset;
+    it.m = p->m;
+    it.col = p->modes[K - 1];
+    uint64_t wgs;
+    if (block) {
+      it.per_wave = c.per_wave ? c.per_wave : pw_default;
+      wgs = ((it.S + it.per_wave - 1) / it.per_wave + slos::kWaves - 1) / slos::kWaves;
+      sh = std::max(sh, slos::block_lds_bytes(K, p->me));
+    } else {
+      it.per_wave = c.per_wave ? c.per_wave : 1;
+      wgs = (it.S + 256 * it.per_wave - 1) / (256 * it.per_wave);
+      sh = std::max(sh, simple_lds_bytes(K, p->me, p->m));
+    }
+    it.wg_begin = (uint32_t)total;
+    it.wgs_per_inst = (uint32_t)wgs;
+    total += wgs * (uint64_t)a.B[g];
+    if (total > 0x7fffffffull) return fail(SLOS_ERANGE, "group layer too large for one launch");
+  }
+  if (cnt == 0) return SLOS_OK;
+  gl.count = cnt;
+  gl.total = (uint32_t)total;
+  if (sh + (block ? 10 * 1024 : 0) > 64 * 1024) return fail(SLOS_ERANGE, "group layer LDS exceeds 64 KB");
+  const uint32_t all = (uint32_t)(8 * ((total + 7) / 8));
+  const uint32_t grid = c.grid ? std::min<uint32_t>(std::max<uint32_t>(c.grid & ~7u, 8), all) : all;
+  const bool comp = a.flags & SLOS_FLAG_COMPENSATED;
+  if (block) comp ? launch_block_group<true>(K - slos::kSuf, grid, sh, st, gl)
+                  : launch_block_group<false>(K - slos::kSuf, grid, sh, st, gl);
+  else if (comp) hipLaunchKernelGGL((k_layer_simple_group<true>), dim3(grid), dim3(256), sh, st, gl);
+  else hipLaunchKernelGGL((k_layer_simple_group<false>), dim3(grid), dim3(256), sh, st, gl);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int group_tables(const slos_group* G, std::vector<DevTables>* dts) {
+  dts->resize(G->plans.size());
+  for (size_t g = 0; g < G->plans.size(); ++g) {
+    int rc = device_tables(G->plans[g], &(*dts)[g]);
+    if (rc) return rc;
+  }
+  return SLOS_OK;
+}
+
+int group_forward(const slos_group* G, const GroupArgs& a, const GroupWs& w, const std::vector<DevTables>& dts,
+                  hipStream_t st) {
+  for (size_t g = 0; g < G->plans.size(); ++g) {
+    hipLaunchKernelGGL(k_fill_ones, dim3((unsigned)std::min<int64_t>((a.B[g] + 255) / 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<float2*>(a.ws + w.ones[g]), a.B[g]);
+    HIP_TRY(hipGetLastError());
+  }
+  for (int K = 1; K <= G->nmax; ++K) {
+    int rc = group_layer(G, K, a, w, dts, G->cfg[K], st);
+    if (rc) return rc;
+  }
+  return SLOS_OK;
+}
+
+void group_drop_graph(slos_group* G) {
+  if (G->exec) (void)hipGraphExecDestroy(G->exec);
+  if (G->graph) (void)hipGraphDestroy(G->graph);
+  G->exec = nullptr;
+  G->graph = nullptr;
 }
 
 }  // namespace
@@ -838,6 +1090,166 @@ int slos_spot_probs(const slos_plan* p, const void* U, const uint64_t* idx, int6
   hipLaunchKernelGGL(k_spot_ryser, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, st, dt.cg, dt.lo, p->me, p->n,
                      p->m, static_cast<const double2*>(U), dt.modes, p->s_fact, p->multiset, idx, count, out);
   HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_group_create(int count, const slos_plan* const* plans, slos_group** out) {
+  if (!out || !plans) return fail(SLOS_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (count < 1 || count > slos::kMaxGroup)
+    return fail(SLOS_EINVAL, "a group holds 1 to " + std::to_string(slos::kMaxGroup) + " plans");
+  auto* G = new slos_group();
+  for (int g = 0; g < count; ++g) {
+    const slos_plan* p = plans[g];
+    if (!p) {
+      delete G;
+      return fail(SLOS_EINVAL, "NULL plan");
+    }
+    if (simple_lds_bytes(p->n, p->me, p->m) > 64 * 1024) {
+      delete G;
+      return fail(SLOS_ERANGE, "plan too large for a group launch (LDS)");
+    }
+    G->plans.push_back(p);
+    G->nmax = std::max(G->nmax, p->n);
+  }
+  G->cfg.assign(G->nmax + 1, slos_group::Cfg());
+  *out = G;
+  return SLOS_OK;
+}
+
+int slos_group_destroy(slos_group* G) {
+  if (!G) return SLOS_OK;
+  group_drop_graph(G);
+  if (G->cap) (void)hipStreamDestroy(G->cap);
+  delete G;
+  return SLOS_OK;
+}
+
+int slos_group_workspace(const slos_group* G, const int64_t* batch, size_t* bytes) {
+  if (!G || !batch || !bytes) return fail(SLOS_EINVAL, "NULL argument");
+  GroupWs w;
+  int rc = group_layout(G, batch, &w);
+  if (rc) return rc;
+  *bytes = w.total;
+  return SLOS_OK;
+}
+
+int slos_group_forward(slos_group* G, const void* const* U, const int64_t* batch, float* const* out,
+                       const double* scale, unsigned flags, void* ws, size_t ws_bytes, slos_stream_t stream) {
+  GroupArgs a;
+  GroupWs w;
+  int rc = group_args(G, U, batch, out, scale, flags, ws, ws_bytes, &a, &w);
+  if (rc) return rc;
+  std::vector<DevTables> dts;
+  rc = group_tables(G, &dts);
+  if (rc) return rc;
+  return group_forward(G, a, w, dts, reinterpret_cast<hipStream_t>(stream));
+}
+
+int slos_group_tune(slos_group* G, const void* const* U, const int64_t* batch, float* const* out,
+                    const double* scale, unsigned flags, void* ws, size_t ws_bytes, int reps, slos_stream_t stream) {
+  GroupArgs a;
+  GroupWs w;
+  int rc = group_args(G, U, batch, out, scale, flags, ws, ws_bytes, &a, &w);
+  if (rc) return rc;
+  if (reps < 1) return fail(SLOS_EINVAL, "reps must be >= 1");
+  std::vector<DevTables> dts;
+  rc = group_tables(G, &dts);
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  int dev = 0, ncu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  group_drop_graph(G);
+  std::fill(G->cfg.begin(), G->cfg.end(), slos_group::Cfg());
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  // layers in order: each candidate rewrites layer K from the (already final) layer K - 1
+  rc = group_forward(G, a, w, dts, st);
+  for (int K = 1; K <= G->nmax && rc == SLOS_OK; ++K) {
+    std::vector<slos_group::Cfg> cand;
+    const uint32_t grids[4] = {0, (uint32_t)ncu * 2, (uint32_t)ncu * 4, (uint32_t)ncu * 8};
+    if (group_can_block(G, dts, K, flags))
+      for (uint64_t pw : {256, 512, 1024, 2048, 4096})
+        for (uint32_t gr : grids) cand.push_back({1, pw, gr});
+    for (uint64_t spt : {1, 2, 4})
+      for (uint32_t gr : grids) cand.push_back({0, spt, gr});
+    float best = 1e30f;
+    slos_group::Cfg pick;
+    for (const auto& c : cand) {
+      rc = group_layer(G, K, a, w, dts, c, st);  // warm-up
+      if (rc) break;
+      if ((rc = (hipEventRecord(e0, st) == hipSuccess ? SLOS_OK : fail(SLOS_EHIP, "hipEventRecord")))) break;
+      for (int r = 0; r < reps && rc == SLOS_OK; ++r) rc = group_layer(G, K, a, w, dts, c, st);
+      if (rc) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(SLOS_EHIP, "event timing failed");
+        break;
+      }
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    G->cfg[K] = pick;
+    // leave layer K final under the chosen launch (all candidates compute the same values)
+    if (rc == SLOS_OK) rc = group_layer(G, K, a, w, dts, pick, st);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int slos_group_config(const slos_group* G, int K, int* block, uint64_t* per_wave, uint32_t* grid) {
+  if (!G || !block || !per_wave || !grid) return fail(SLOS_EINVAL, "NULL argument");
+  if (K < 1 || K > G->nmax) return fail(SLOS_EINVAL, "layer out of range");
+  *block = G->cfg[K].block;
+  *per_wave = G->cfg[K].per_wave;
+  *grid = G->cfg[K].grid;
+  return SLOS_OK;
+}
+
+int slos_group_capture(slos_group* G, const void* const* U, const int64_t* batch, float* const* out,
+                       const double* scale, unsigned flags, void* ws, size_t ws_bytes) {
+  GroupArgs a;
+  GroupWs w;
+  int rc = group_args(G, U, batch, out, scale, flags, ws, ws_bytes, &a, &w);
+  if (rc) return rc;
+  std::vector<DevTables> dts;
+  rc = group_tables(G, &dts);  // device tables exist before capture: nothing is allocated inside it
+  if (rc) return rc;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (G->cap && G->cap_device != dev) {
+    (void)hipStreamDestroy(G->cap);
+    G->cap = nullptr;
+  }
+  if (!G->cap) {
+    HIP_TRY(hipStreamCreateWithFlags(&G->cap, hipStreamNonBlocking));
+    G->cap_device = dev;
+  }
+  group_drop_graph(G);
+  HIP_TRY(hipStreamBeginCapture(G->cap, hipStreamCaptureModeThreadLocal));
+  rc = group_forward(G, a, w, dts, G->cap);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(G->cap, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(SLOS_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  G->graph = graph;
+  HIP_TRY(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+  return SLOS_OK;
+}
+
+int slos_group_replay(slos_group* G, slos_stream_t stream) {
+  if (!G) return fail(SLOS_EINVAL, "NULL argument");
+  if (!G->exec) return fail(SLOS_EINVAL, "no captured forward: call slos_group_capture first");
+  HIP_TRY(hipGraphLaunch(G->exec, reinterpret_cast<hipStream_t>(stream)));
   return SLOS_OK;
 }
 

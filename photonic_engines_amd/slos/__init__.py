@@ -4,9 +4,12 @@
   kernel  FockSLOSKernel: the layer recurrence as gfx950 HIP kernels, fused readout
   autograd hand-written adjoint of probs(U) with every-other-layer checkpointing
   haar    seeded Haar-random unitaries (the test and bench input generator)
+  batched UnevenBatch: kernels of different shapes in one launch per layer, autotuned, hipGraph
 """
 from .basis import FockLayers, RankedLayers, clear_layers_cache, get_layers
 from .haar import haar_unitary
 from .kernel import FockSLOSKernel
+from .batched import UnevenBatch
 
-__all__ = ["FockLayers", "FockSLOSKernel", "RankedLayers", "clear_layers_cache", "get_layers", "haar_unitary"]
+__all__ = ["FockLayers", "FockSLOSKernel", "RankedLayers", "clear_layers_cache", "get_layers", "haar_unitary",
+           "UnevenBatch"]

@@ -151,8 +151,14 @@ int emu_forward(int n, int m, int mask, const int* modes, const double* U, doubl
       }
     } else {
       const int d = K - slos::kSuf;
-      if (d <= 8) layer_block<8>(s, K, ucol, par, out, last ? &norms : nullptr, per_wave, writes);
-      else layer_block<32>(s, K, ucol, par, out, last ? &norms : nullptr, per_wave, writes);
+      // the stream widths the device launcher instantiates: exact for d = 5..8
+      auto* nm = last ? &norms : nullptr;
+      if (d <= 4) layer_block<4>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 5) layer_block<5>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 6) layer_block<6>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 7) layer_block<7>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 8) layer_block<8>(s, K, ucol, par, out, nm, per_wave, writes);
+      else layer_block<16>(s, K, ucol, par, out, nm, per_wave, writes);
       for (int w : writes)
         if (w != 1) return 2;
     }

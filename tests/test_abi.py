@@ -70,3 +70,45 @@ def test_error_mapping():
         nat.check(nat.SLOS_ENOMEM, "x")
     with pytest.raises(nat.NativeError):
         nat.check(nat.SLOS_EHIP, "x")
+
+
+def test_group_host_checks():
+    """Uneven-batch groups: creation limits, workspace arithmetic and replay-before-capture, host only."""
+    L = nat.lib()
+    plans = []
+    for n, m in ((4, 12), (6, 16), (8, 20)):
+        rc, h = _create(n, m, 0, list(range(n)))
+        assert rc == nat.SLOS_OK
+        plans.append(h)
+    arr = (ctypes.c_void_p * 9)(*([plans[0].value] * 9))
+    g = ctypes.c_void_p()
+    assert L.slos_group_create(0, arr, ctypes.byref(g)) == nat.SLOS_EINVAL
+    assert L.slos_group_create(9, arr, ctypes.byref(g)) == nat.SLOS_EINVAL
+    nul = (ctypes.c_void_p * 2)(plans[0].value, None)
+    assert L.slos_group_create(2, nul, ctypes.byref(g)) == nat.SLOS_EINVAL
+    three = (ctypes.c_void_p * 3)(*[p.value for p in plans])
+    assert L.slos_group_create(3, three, ctypes.byref(g)) == nat.SLOS_OK
+    B = (ctypes.c_int64 * 3)(2, 3, 1)
+    size = ctypes.c_size_t()
+    assert L.slos_group_workspace(g, B, ctypes.byref(size)) == nat.SLOS_OK
+    # per plan: B ones + two ping-pong layers of B * max_{k<n} S_k complex64, each 256-byte aligned
+    from math import comb
+
+    def a256(x):
+        return (x + 255) // 256 * 256
+    exp = 0
+    for (n, m), b in zip(((4, 12), (6, 16), (8, 20)), (2, 3, 1)):
+        mx = max(comb(m + k - 1, k) for k in range(1, n))
+        exp += a256(8 * b) + 2 * a256(8 * b * mx)
+    assert size.value == exp
+    B0 = (ctypes.c_int64 * 3)(2, 0, 1)
+    assert L.slos_group_workspace(g, B0, ctypes.byref(size)) == nat.SLOS_EINVAL
+    assert L.slos_group_replay(g, None) == nat.SLOS_EINVAL
+    assert b"capture" in L.slos_last_error()
+    blk, pw, gr = ctypes.c_int(), ctypes.c_uint64(), ctypes.c_uint32()
+    assert L.slos_group_config(g, 8, ctypes.byref(blk), ctypes.byref(pw), ctypes.byref(gr)) == nat.SLOS_OK
+    assert (blk.value, pw.value, gr.value) == (-1, 0, 0)
+    assert L.slos_group_config(g, 9, ctypes.byref(blk), ctypes.byref(pw), ctypes.byref(gr)) == nat.SLOS_EINVAL
+    assert L.slos_group_destroy(g) == nat.SLOS_OK
+    for p in plans:
+        assert L.slos_plan_destroy(p) == nat.SLOS_OK

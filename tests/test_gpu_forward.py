@@ -40,7 +40,11 @@ CASES = [(3, 6, "fock", None), (3, 6, "unbunched", None), (3, 6, "dual_rail", No
          (3, 12, "sparse_1in4", None), (4, 7, "tail_empty", [0, 2, 1, 1, 0, 0, 0]), (6, 6, "fock", [3, 0, 2, 0, 1, 0]),
          (7, 9, "fock", [1, 0, 2, 1, 0, 3, 0, 0, 0]), (8, 10, "fock", None), (6, 12, "dual_rail", None),
          (5, 20, "sparse_1in4", None), (6, 8, "fock", [6, 0, 0, 0, 0, 0, 0, 0]), (9, 11, "fock", None),
-         (2, 129, "fock", None), (1, 5, "fock", None), (4, 130, "fock", None), (5, 60, "unbunched", None)]
+         (2, 129, "fock", None), (1, 5, "fock", None), (4, 130, "fock", None), (5, 60, "unbunched", None),
+         # prefix widths d = n - 4 of 5..9 (every stream-slot count the block kernel is built for)
+         (9, 5, "fock", [2, 2, 2, 2, 1]), (10, 5, "fock", [3, 0, 4, 1, 2]), (11, 4, "fock", [5, 0, 3, 3]),
+         (12, 4, "fock", [3, 3, 3, 3]), (13, 3, "fock", [5, 4, 4]), (9, 12, "unbunched", None),
+         (10, 20, "dual_rail", None)]
 
 
 @pytest.mark.parametrize("n,m,mask,occ", CASES)
@@ -59,7 +63,8 @@ def test_matches_oracle(n, m, mask, occ):
     _check(ps[0], ref, what=f"simple {n},{m},{mask}")
     amp = k.amplitudes(torch.as_tensor(U).to(torch.complex64))
     ref_amp = pull.forward_amps(n, m, occ, U, mask)
-    assert np.abs(amp.cpu().numpy() - ref_amp).max() < 1e-5
+    # raw (deferred) amplitudes of bunched states grow like sqrt(prod t!): relative to their scale
+    assert np.abs(amp.cpu().numpy() - ref_amp).max() < 1e-5 * max(1.0, np.abs(ref_amp).max())
 
 
 def test_c1_golden():
