@@ -1,6 +1,6 @@
 """Headline benchmark: SLOS strong simulation, n=11 photons in m=32 modes, complex64, one GPU.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--workload c5|c2]
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batched] [--workload c5|c2]
 
 A step is one full FockSLOSKernel.probs(U) pass (11 photon layers, the last one fused with the
 |amp|^2 * prod t! readout, float32 output) over one seeded Haar unitary already resident in HBM.
@@ -12,6 +12,9 @@ against FP64 Ryser permanents computed on the GPU, and the FP64 total mass.
 `cpu_baseline`: the oracle's C++/OpenMP pull-form restatement on this host's cores (a bounded
 sample: layers 1..10 complete plus the first 2^27 states of layer 11, the full-run time
 extrapolated linearly in layer-11 states).
+`variants`: the compensated-accumulation forward of the same workload, and the uneven batch of
+512 unitaries ((4,12), (6,16), (8,20)) as a single-U loop, per-shape batches, one group launch
+per layer, and the autotuned group forward replayed as a hipGraph.
 """
 from __future__ import annotations
 
@@ -57,6 +60,62 @@ def load_traffic(tag: str):
     return rec if hashlib.sha256(src).hexdigest() == rec.get("source_sha256") else None
 
 
+UNEVEN = [(4, 12, 171), (6, 16, 171), (8, 20, 170)]  # (n, m, unitaries): 512 in all, photons in modes 0..n-1
+
+
+def bench_uneven(reps: int = 5) -> dict:
+    """512 seeded Haar unitaries, 1/3 each of (n, m) = (4,12), (6,16), (8,20), complex64, lossless:
+    throughput of the plain single-U path (one probs(U) call per unitary), of one batched call per
+    shape, of one group launch per layer (untuned), and of the autotuned group forward replayed as
+    one hipGraph. All four produce bit-identical outputs; accuracy is stated once, over 32 seeded
+    outputs of every instance against FP64 Ryser permanents, plus every instance's FP64 mass."""
+    from photonic_engines_amd.slos import FockSLOSKernel, UnevenBatch, haar_unitary
+    dev = torch.device("cuda", 0)
+    ks = [FockSLOSKernel(n, m, [1] * n + [0] * (m - n), device=str(dev)) for n, m, _ in UNEVEN]
+    Us = [torch.stack([haar_unitary(m, 100_000 * (g + 1) + i) for i in range(c)]).to(torch.complex64).to(dev)
+          for g, (n, m, c) in enumerate(UNEVEN)]
+    total = sum(k.output_size() * c for k, (_, _, c) in zip(ks, UNEVEN))
+
+    def clock(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    res = {"workload": "512 Haar unitaries: 171 x (4,12), 171 x (6,16), 170 x (8,20), photons in modes 0..n-1, "
+                       "complex64, float32 probabilities", "outputs": total}
+    t = clock(lambda: [k.probs(U[i]) for k, U in zip(ks, Us) for i in range(U.shape[0])])
+    res["single_u_loop"] = {"ms": t * 1e3, "probs_per_s": total / t}
+    t = clock(lambda: [k.probs(U) for k, U in zip(ks, Us)])
+    res["batched_per_shape"] = {"ms": t * 1e3, "probs_per_s": total / t}
+    batch = UnevenBatch(ks)
+    t = clock(lambda: batch.probs(Us))
+    res["group_untuned"] = {"ms": t * 1e3, "probs_per_s": total / t}
+    cfg = batch.tune(Us, reps=2)
+    batch.capture(Us)
+    t = clock(lambda: batch.replay())
+    res["group_tuned_graph"] = {"ms": t * 1e3, "probs_per_s": total / t,
+                                "tuned": [f"L{c['layer']}:{c['kernel']}/{c['per_wave']}/{c['grid']}" for c in cfg]}
+    outs = batch.replay()
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(512)
+    worst, mass_dev = 0.0, 0.0
+    for k, U, P in zip(ks, Us, outs):
+        mass_dev = max(mass_dev, float((k.total_mass(P) - 1).abs().max()))
+        U128 = U.cpu().to(torch.complex128)
+        for i in range(U.shape[0]):
+            idx = torch.as_tensor(np.sort(rng.choice(P.shape[1], 32, replace=False)).astype(np.int64), device=dev)
+            ref = k.spot_probs(U128[i], idx).cpu().numpy()
+            worst = max(worst, float(np.abs(P[i, idx].double().cpu().numpy() - ref).max()))
+    res["accuracy"] = {"max_abs_dp": worst, "spot_outputs_per_instance": 32, "oracle": "FP64 Ryser on the GPU",
+                       "max_abs_mass_minus_1": mass_dev}
+    res["speedup_graph_vs_single_u"] = res["single_u_loop"]["ms"] / res["group_tuned_graph"]["ms"]
+    return res
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -77,6 +136,7 @@ def main() -> None:
     ap.add_argument("--cpu-last", type=int, default=1 << 27, help="layer-n states in the CPU sample")
     ap.add_argument("--no-compensated", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 Ryser spot checks (profiling passes)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batch measurement")
     ap.add_argument("--copy-calibration-gb", type=float, default=0.0,
                     help="also run one plain device copy of this many GB (HBM counter calibration)")
     args = ap.parse_args()
@@ -186,6 +246,9 @@ def main() -> None:
                                            "max_rel": float((np.abs(gc - ref) / np.maximum(ref, 1e-300)).max())}}
         del pc
     torch.cuda.empty_cache()
+    if not args.no_batched:
+        out.setdefault("variants", {})["batched_uneven"] = bench_uneven()
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         from oracle import pull
         threads = len(os.sched_getaffinity(0))

@@ -110,6 +110,23 @@ def test_batched_equals_looped_and_deterministic():                            #
     assert torch.equal(pb, k.probs(Us))
 
 
+@pytest.mark.parametrize("n,m,mask,occ", [(9, 14, "fock", None), (8, 16, "fock", [2, 0, 1, 0, 3, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0]),
+                                          (7, 18, "tail_empty", None), (6, 24, "unbunched", None),
+                                          (10, 9, "fock", [2, 2, 1, 1, 1, 1, 1, 1, 0])])
+def test_block_kernel_bitwise_equals_thread_kernel(n, m, mask, occ):
+    """Every launch form sums a child's parents in ascending photon position: the block kernel (64-child
+    steps, 128-child adjacent pairs, staged or global local parents) and the thread-per-state kernel
+    give bit-identical layers, probabilities and scaled amplitudes; compensated too."""
+    occ = occ or occupation_list(n, m, mask)
+    k = FockSLOSKernel(n, m, occ, mask=mask, device=DEV)
+    U = torch.stack([haar_unitary(m, 77 + s).to(torch.complex64) for s in range(2)]).to(DEV).contiguous()
+    for flags in (0, nat.FLAG_COMPENSATED):
+        for kind in (nat.OUT_PROBS, nat.OUT_AMPS):
+            a = k._plan.forward(U, out_kind=kind, scale=input_norm(occ), flags=flags)
+            b = k._plan.forward(U, out_kind=kind, scale=input_norm(occ), flags=flags | nat.FLAG_SIMPLE)
+            assert torch.equal(a, b), (flags, kind)
+
+
 def test_host_device_roundtrip_and_amplitude_norms():                          # test_tutorial.py:45-59
     U = haar_unitary(8, 0)
     k = FockSLOSKernel(n=3, m=8, input_occupation=[1, 0, 1, 0, 1, 0, 0, 0], mask="fock", device="cpu")
