@@ -30,7 +30,6 @@ struct BlockLayer {
   float scale;
   unsigned flags;
   uint64_t per_wave;     // children per wave
-  uint32_t stage_cap;    // local-parent staging buffer per wave (float2 entries; 0: no staging)
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -84,9 +83,7 @@ template <int MAXD>
 struct ChunkCtx {
   const float2* const (&sp)[MAXD];  // prefix-stream bases (parent of child u at sp[s] + u)
   const float2 (&coef)[MAXD];       // their U coefficients (zero for inactive slots)
-  const float2* lp;                 // local parents: lp + lr_j (global memory)
-  const float2* stage;              // the same parents staged in LDS: stage[lr_j - off3]
-  uint32_t off3;                    // (SUF-1)-tuple index of the block's first local parent
+  const float2* lp;                 // local parents: lp + lr_j
   const TupleEntry* tb;             // tuple entries of the block's children
   char* ob;                         // output of the block's first child
   const float2* ucol;               // U column (LDS; zero at positions >= m, kNoEdge included)
@@ -101,7 +98,7 @@ struct ChunkCtx {
 
 // NC consecutive 64-child chunks starting at uc: every load of the NC chunks is issued before any
 // result is needed. Lanes past u_stop re-read the last valid child and store nothing.
-template <int KIND, bool COMP, int MAXD, int NC, bool STAGED>
+template <int KIND, bool COMP, int MAXD, int NC>
 __device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
   uint32_t u[NC];
   bool in[NC];
@@ -124,10 +121,7 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
   for (int c = 0; c < NC; ++c) {
     const uint32_t lr[kSuf] = {te[c].z & 0xffffu, te[c].z >> 16, te[c].w & 0xffffu, te[c].w >> 16};
 #pragma unroll
-    for (int jj = 0; jj < kSuf; ++jj) {
-      if constexpr (STAGED) lv[c][jj] = cx.stage[lr[jj] - cx.off3];
-      else lv[c][jj] = at_bytes(cx.lp, lr[jj] * 8u);
-    }
+    for (int jj = 0; jj < kSuf; ++jj) lv[c][jj] = at_bytes(cx.lp, lr[jj] * 8u);
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -174,7 +168,7 @@ __device__ __forceinline__ float4 load_pair(const float2* base, uint32_t off_byt
 // 128 consecutive children from uc, all inside the block (uc + 128 <= u_stop): lane i takes the
 // adjacent children uc + 2i and uc + 2i + 1, so each prefix stream is one 16-byte load per lane and
 // the store is one 8- (probabilities) or 16-byte (amplitudes) store per lane.
-template <int KIND, bool COMP, int MAXD, bool STAGED>
+template <int KIND, bool COMP, int MAXD>
 __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t uc) {
   const uint32_t u = uc + 2u * (uint32_t)cx.lane;
   uint4 te[2];
@@ -188,10 +182,7 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t u
   for (int c = 0; c < 2; ++c) {
     const uint32_t lr[kSuf] = {te[c].z & 0xffffu, te[c].z >> 16, te[c].w & 0xffffu, te[c].w >> 16};
 #pragma unroll
-    for (int jj = 0; jj < kSuf; ++jj) {
-      if constexpr (STAGED) lv[c][jj] = cx.stage[lr[jj] - cx.off3];
-      else lv[c][jj] = at_bytes(cx.lp, lr[jj] * 8u);
-    }
+    for (int jj = 0; jj < kSuf; ++jj) lv[c][jj] = at_bytes(cx.lp, lr[jj] * 8u);
   }
   float res[2][2];
   float f[2] = {1.f, 1.f};
@@ -232,39 +223,17 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t u
 }
 
 // Dynamic LDS of the block kernels: counting-table rows 0..K and lo[] (per-lane block data is
-// computed from them), then one local-parent staging buffer of stage_cap float2 per wave.
+// computed from them); the U column is static (256 float2).
 __host__ __device__ inline size_t block_lds_bytes(int K, int me) { return (size_t)(K + 1) * (me + 1) * 8 + (size_t)me * 4; }
-__host__ __device__ inline size_t block_stage_offset(int K, int me) { return (block_lds_bytes(K, me) + 15) & ~(size_t)15; }
-constexpr uint32_t kStageMax = 1024;            // staged local ranges cover 3-tuples from lo >= m - 16
-constexpr size_t kBlockStaticLds = 256 * 8;     // ucol
-// the largest staging buffer (multiple of 64 entries, <= kStageMax) that keeps a work group's
-// LDS within 64 KB; 0 disables staging
-inline uint32_t block_stage_cap(int K, int me) {
-  const size_t base = block_stage_offset(K, me) + kBlockStaticLds;
-  if (base >= 64 * 1024) return 0;
-  size_t cap = (64 * 1024 - base) / (kWaves * 8);
-  cap = cap > kStageMax ? kStageMax : (cap & ~(size_t)63);
-  return (uint32_t)cap;
-}
-inline size_t block_dyn_lds(int K, int me, uint32_t cap) {
-  return block_stage_offset(K, me) + (size_t)kWaves * cap * 8;
-}
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
+constexpr size_t kBlockStaticLds = 256 * 8;  // ucol
 // One wave's share of a layer slice: the children [g, g + a.per_wave) of one instance, whose parent
 // layer is `par` and whose output layer starts at `out`. `ucol` is the U column in LDS (entries >= m
-// zero), `stage` this wave's LDS buffer of a.stage_cap entries, `tl` the counting tables in LDS,
-// `tg` the same tables in global memory (for the wave-uniform scalar unrank of g). A block whose
-// local parents (the (SUF-1)-tuples from lo: cg[SUF-1][lo] consecutive states of layer K-1) fit the
-// buffer reads them once, coalesced, and gathers them from LDS; larger ones gather from memory.
+// zero), `tl` the counting tables in LDS, `tg` the same tables in global memory (for the
+// wave-uniform scalar unrank of g).
 // KIND 3 decides between probabilities (kind 1) and raw amplitudes (kind 0) at run time.
 template <int KIND, bool COMP, int MAXD>
 __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTables& tl, const float2* __restrict__ par,
-                                           void* __restrict__ out, const float2* ucol, float2* stage,
+                                           void* __restrict__ out, const float2* ucol,
                                            const BlockLayer& a, uint64_t g, int kind = 0) {
   const int lane = threadIdx.x & 63;
   const int K = a.K;
@@ -283,7 +252,6 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
     u0 = unrank_prefix<MAXD>(tg, K, d, g, q);
     r = prefix_rank<MAXD>(tg, d, q);
   }
-  const uint64_t* cg3 = cg_l + (kSuf - 1) * stride;  // counts of (SUF-1)-tuples from x
   while (g < g_end) {
     // ---- lane i: data of the block of prefix r + i ----
     BlockInfo bi;
@@ -334,29 +302,6 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
         coef[s] = uniform_f2(ucol[md]);
         sp[s] = par + (md == (uint32_t)kNoEdge ? b0 : rl64(st.base[s], j));
       }
-      // local parents: R consecutive states of layer K-1 from LBp + off3
-      const uint32_t off3 = (uint32_t)uniform_u64(cg3[0] - cg3[blo]);
-      const uint32_t R = (uint32_t)uniform_u64(cg3[blo]);
-      const bool staged = R <= a.stage_cap;
-      if (staged) {
-        // all loads of a batch of 8 x 64 parents in flight before the first LDS write
-        const float2* src = par + LBp + off3;
-        wave_sync();  // every lane is done with the previous block's staged parents
-        for (uint32_t i0 = 0; i0 < R; i0 += 512u) {
-          float2 v[8];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            const uint32_t i = i0 + 64u * k + (uint32_t)lane;
-            if (i0 + 64u * k < R) v[k] = src[i < R ? i : R - 1u];
-          }
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            const uint32_t i = i0 + 64u * k + (uint32_t)lane;
-            if (i0 + 64u * k < R && i < R) stage[i] = v[k];
-          }
-        }
-        wave_sync();
-      }
       float fQ = 1.f;
       int lastrun = 0;
       if constexpr (KIND != 0) {
@@ -369,20 +314,18 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
       char* ob = reinterpret_cast<char*>(out) + CB * (probs ? 4 : 8);
       const uint64_t avail = g_end - g;
       const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
-      ChunkCtx<MAXD> cx{sp, coef, lp, stage, off3, tb, ob, ucol, u_stop, lane, fQ, lastrun, skip0, blo, multiset,
-                        a.scale, a.flags, kind};
+      ChunkCtx<MAXD> cx{sp, coef, lp, tb, ob, ucol, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale, a.flags,
+                        kind};
       uint32_t uc = (uint32_t)u0;
-      // full groups of 128 children as adjacent pairs per lane, the rest 64 children per step
-      if (staged) {
-        if constexpr (!COMP)  // (the compensated sums keep one child per lane: register pressure)
-          for (; uc + 128u <= u_stop; uc += 128u) sweep_pairs<KIND, COMP, MAXD, true>(cx, uc);
-        for (; uc + 64u < u_stop; uc += 128u) sweep<KIND, COMP, MAXD, 2, true>(cx, uc);
-        if (uc < u_stop) sweep<KIND, COMP, MAXD, 1, true>(cx, uc);
+      if constexpr (!COMP) {
+        // full groups of 128 children as adjacent pairs per lane, the rest 64 children per step
+        for (; uc + 128u <= u_stop; uc += 128u) sweep_pairs<KIND, COMP, MAXD>(cx, uc);
+        if (uc + 64u < u_stop) sweep<KIND, COMP, MAXD, 2>(cx, uc);
+        else if (uc < u_stop) sweep<KIND, COMP, MAXD, 1>(cx, uc);
       } else {
-        if constexpr (!COMP)
-          for (; uc + 128u <= u_stop; uc += 128u) sweep_pairs<KIND, COMP, MAXD, false>(cx, uc);
-        for (; uc + 64u < u_stop; uc += 128u) sweep<KIND, COMP, MAXD, 2, false>(cx, uc);
-        if (uc < u_stop) sweep<KIND, COMP, MAXD, 1, false>(cx, uc);
+        // the compensated sums keep one child per lane (register pressure)
+        for (; uc + 64u < u_stop; uc += 128u) sweep<KIND, COMP, MAXD, 2>(cx, uc);
+        if (uc < u_stop) sweep<KIND, COMP, MAXD, 1>(cx, uc);
       }
       g += u_stop - u0;
       u0 = 0;
@@ -406,7 +349,6 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   const int ncg = (K + 1) * stride;
   uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
   int* lo_l = reinterpret_cast<int*>(cg_l + ncg);
-  float2* stage_l = reinterpret_cast<float2*>(smem + block_stage_offset(K, me));
   const int64_t b = blockIdx.y;
   for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = cg_t[i];
   for (int i = threadIdx.x; i < me; i += blockDim.x) lo_l[i] = lo_t[i];
@@ -420,8 +362,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   const uint32_t wg_l = xcd_remap(blockIdx.x, gridDim.x);
   const uint64_t g = uniform_u64(((uint64_t)wg_l * kWaves + wv) * a.per_wave);
   char* out_b = reinterpret_cast<char*>(out_all) + (size_t)b * a.S * (KIND == 1 ? 4 : 8);
-  block_wave<KIND, COMP, MAXD>(tg, tl, par_all + (size_t)b * a.S_par, out_b, ucol, stage_l + (size_t)wv * a.stage_cap,
-                               a, g);
+  block_wave<KIND, COMP, MAXD>(tg, tl, par_all + (size_t)b * a.S_par, out_b, ucol, a, g);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -453,8 +394,6 @@ struct GroupLaunch {
   GroupItem it[kMaxGroup];
   int count, K;
   uint32_t total;              // items of the launch
-  uint32_t stage_off;          // block kernel: byte offset of the staging buffers in dynamic LDS
-  uint32_t stage_cap;          // block kernel: staging entries per wave
 };
 
 // the plan of work-group index `item` (plans in ascending wg_begin)
@@ -508,12 +447,11 @@ k_layer_block_group(GroupLaunch gl) {
     const BlockTables tl{cg_l, lo_l, it.tuples, it.me, stride, it.multiset};
     BlockLayer a;
     a.K = K; a.m = it.m; a.col = it.col; a.U = it.U; a.S_par = it.S_par; a.S = it.S; a.S_pre = it.S_pre;
-    a.scale = it.scale; a.flags = it.flags; a.per_wave = it.per_wave; a.stage_cap = gl.stage_cap;
+    a.scale = it.scale; a.flags = it.flags; a.per_wave = it.per_wave;
     const uint64_t g = uniform_u64(((uint64_t)w * kWaves + wv) * it.per_wave);
     const float2* par = it.par + (size_t)b * it.S_par;
     char* out = reinterpret_cast<char*>(it.out) + (size_t)b * it.S * (it.kind == 1 ? 4 : 8);
-    float2* stage = reinterpret_cast<float2*>(smem + gl.stage_off) + (size_t)wv * gl.stage_cap;
-    block_wave<3, COMP, MAXD>(tg, tl, par, out, ucol, stage, a, g, it.kind);
+    block_wave<3, COMP, MAXD>(tg, tl, par, out, ucol, a, g, it.kind);
   });
 }
 

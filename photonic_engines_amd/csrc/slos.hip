@@ -452,8 +452,7 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   const uint64_t wgs = (waves + slos::kWaves - 1) / slos::kWaves;
   if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
   if (a.B > 65535) return fail(SLOS_ERANGE, "batch > 65535");
-  L.stage_cap = slos::block_stage_cap(a.K, p->me);
-  const size_t sh = slos::block_dyn_lds(a.K, p->me, L.stage_cap);
+  const size_t sh = slos::block_lds_bytes(a.K, p->me);
   if (sh + slos::kBlockStaticLds > 64 * 1024) return fail(SLOS_ERANGE, "block kernel LDS exceeds 64 KB");
   dim3 grid((unsigned)wgs, (unsigned)a.B);
   const int maxd = a.K - slos::kSuf;
@@ -703,7 +702,7 @@ int group_layer(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w
     if (block) {
       it.per_wave = c.per_wave ? c.per_wave : pw_default;
       wgs = ((it.S + it.per_wave - 1) / it.per_wave + slos::kWaves - 1) / slos::kWaves;
-      sh = std::max(sh, slos::block_stage_offset(K, p->me));
+      sh = std::max(sh, slos::block_lds_bytes(K, p->me));
     } else {
       it.per_wave = c.per_wave ? c.per_wave : 1;
       wgs = (it.S + 256 * it.per_wave - 1) / (256 * it.per_wave);
@@ -717,15 +716,6 @@ int group_layer(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w
   if (cnt == 0) return SLOS_OK;
   gl.count = cnt;
   gl.total = (uint32_t)total;
-  if (block) {
-    // staging buffers after the largest table region of the launch's plans
-    gl.stage_off = (uint32_t)sh;
-    const size_t base = sh + slos::kBlockStaticLds;
-    size_t cap = base >= 64 * 1024 ? 0 : (64 * 1024 - base) / (slos::kWaves * 8);
-    cap = cap > slos::kStageMax ? slos::kStageMax : (cap & ~(size_t)63);
-    gl.stage_cap = (uint32_t)cap;
-    sh += (size_t)slos::kWaves * cap * 8;
-  }
   if (sh + (block ? slos::kBlockStaticLds : 0) > 64 * 1024) return fail(SLOS_ERANGE, "group layer LDS exceeds 64 KB");
   const uint32_t all = (uint32_t)(8 * ((total + 7) / 8));
   const uint32_t grid = c.grid ? std::min<uint32_t>(std::max<uint32_t>(c.grid & ~7u, 8), all) : all;

@@ -65,7 +65,7 @@ void layer_simple(const Space& s, int K, const cd* ucol, const std::vector<cd>& 
 // one at a time, 64 children per step; writes[c] counts how often child c is written
 template <int MAXD>
 void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const std::vector<cd>& par, std::vector<cd>& out,
-                 std::vector<double>* norms, uint64_t per_wave, uint32_t stage_cap, std::vector<int>& writes) {
+                 std::vector<double>* norms, uint64_t per_wave, std::vector<int>& writes) {
   slos::BlockTables t{s.cg.data(), s.lo.data(), s.tuples.data(), s.me, s.me + 1, s.multiset};
   const int W = s.me + 1;
   const uint64_t S = s.cg[(size_t)K * W];
@@ -94,14 +94,6 @@ void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const st
         slos::Streams<MAXD> st;
         slos::block_info<MAXD>(t, K, d, q, bi, st);
         if (bi.L == 0) continue;
-        // local parents: R consecutive states from LBp + off3, staged when they fit the buffer
-        const uint64_t* cg3 = s.cg.data() + (size_t)(slos::kSuf - 1) * W;
-        const uint32_t off3 = (uint32_t)(cg3[0] - cg3[bi.lo]);
-        const uint32_t R = (uint32_t)cg3[bi.lo];
-        const bool staged = R <= stage_cap;
-        std::vector<cd> stage;
-        if (staged)
-          for (uint32_t i = 0; i < R; ++i) stage.push_back(par.at(bi.LBp + off3 + i));
         const uint64_t avail = g_end - g;
         const uint32_t u_stop = (uint32_t)(((uint64_t)bi.L - u0 < avail) ? (uint64_t)bi.L : u0 + avail);
         for (uint32_t uc = (uint32_t)u0; uc < u_stop; uc += 64)
@@ -116,7 +108,7 @@ void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const st
               const uint32_t x = (e.pos >> (8 * jj)) & 0xffu;
               // tau's first copy of lo continuing the prefix's last run is no edge (edge 0 only)
               const cd coef = (jj == 0 && bi.skip0 && (int)x == bi.lo) ? cd(0, 0) : ucol.at(x);
-              const cd v = staged ? stage.at(e.lr[jj] - off3) : par.at(bi.LBp + e.lr[jj]);
+              const cd v = par.at(bi.LBp + e.lr[jj]);
               acc += coef * v;
             }
             const uint64_t c = bi.CB + u;
@@ -139,7 +131,7 @@ extern "C" {
 // Forward pass from the vacuum to layer n; probs[S_n] = |amp * in_norm|^2 * prod t_j!.
 // Returns 0 on success, 2 if some child was not written exactly once, 1 on bad input.
 int emu_forward(int n, int m, int mask, const int* modes, const double* U, double in_norm, uint64_t per_wave,
-                uint32_t stage_cap, double* probs) {
+                double* probs) {
   Space s;
   if (!build(n, m, mask, s) || n > 32) return 1;
   const cd* Uc = reinterpret_cast<const cd*>(U);
@@ -164,12 +156,12 @@ int emu_forward(int n, int m, int mask, const int* modes, const double* U, doubl
       const int d = K - slos::kSuf;
       // the stream widths the device launcher instantiates: exact for d = 5..8
       auto* nm = last ? &norms : nullptr;
-      if (d <= 4) layer_block<4>(s, K, ucol, par, out, nm, per_wave, stage_cap, writes);
-      else if (d == 5) layer_block<5>(s, K, ucol, par, out, nm, per_wave, stage_cap, writes);
-      else if (d == 6) layer_block<6>(s, K, ucol, par, out, nm, per_wave, stage_cap, writes);
-      else if (d == 7) layer_block<7>(s, K, ucol, par, out, nm, per_wave, stage_cap, writes);
-      else if (d == 8) layer_block<8>(s, K, ucol, par, out, nm, per_wave, stage_cap, writes);
-      else layer_block<16>(s, K, ucol, par, out, nm, per_wave, stage_cap, writes);
+      if (d <= 4) layer_block<4>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 5) layer_block<5>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 6) layer_block<6>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 7) layer_block<7>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 8) layer_block<8>(s, K, ucol, par, out, nm, per_wave, writes);
+      else layer_block<16>(s, K, ucol, par, out, nm, per_wave, writes);
       for (int w : writes)
         if (w != 1) return 2;
     }
