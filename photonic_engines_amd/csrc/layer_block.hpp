@@ -226,15 +226,36 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t u
 // computed from them); the U column is static (256 float2).
 __host__ __device__ inline size_t block_lds_bytes(int K, int me) { return (size_t)(K + 1) * (me + 1) * 8 + (size_t)me * 4; }
 constexpr size_t kBlockStaticLds = 256 * 8;  // ucol
+// Per-wave block table in LDS: the data of kBlockBatch consecutive blocks, one column per block,
+// computed by all lanes at once (lane i: block r + i) and then read back one block at a time with
+// wave-uniform LDS reads, so none of it occupies vector registers during the child sweeps. Words:
+// CB (2), LBp (2), L, toff, lo | skip0 << 8 | lastrun << 16, fQ, the slot modes (4 per word, kNoEdge
+// for inactive slots), the slot bases (2 each; inactive slots carry slot 0's base).
+constexpr int kBlockBatch = 64;
+__host__ __device__ constexpr int block_words(int maxd) { return 8 + (maxd + 3) / 4 + 2 * maxd; }
+__host__ __device__ inline size_t block_table_offset(int K, int me) { return (block_lds_bytes(K, me) + 15) & ~(size_t)15; }
+__host__ __device__ inline size_t block_dyn_lds(int K, int me, int maxd) {
+  return block_table_offset(K, me) + (size_t)kWaves * kBlockBatch * block_words(maxd) * 4;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One wave's share of a layer slice: the children [g, g + a.per_wave) of one instance, whose parent
 // layer is `par` and whose output layer starts at `out`. `ucol` is the U column in LDS (entries >= m
 // zero), `tl` the counting tables in LDS, `tg` the same tables in global memory (for the
-// wave-uniform scalar unrank of g).
+// wave-uniform scalar unrank of g), `tbl` this wave's block table (kBlockBatch * block_words(MAXD)
+// words of LDS).
 // KIND 3 decides between probabilities (kind 1) and raw amplitudes (kind 0) at run time.
 template <int KIND, bool COMP, int MAXD>
 __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTables& tl, const float2* __restrict__ par,
-                                           void* __restrict__ out, const float2* ucol,
+                                           void* __restrict__ out, const float2* ucol, uint32_t* tbl,
                                            const BlockLayer& a, uint64_t g, int kind = 0) {
+  constexpr int kModeW = (MAXD + 3) / 4;
+  constexpr int kBaseW = 8 + kModeW;
   const int lane = threadIdx.x & 63;
   const int K = a.K;
   const int me = tg.me, stride = tg.stride, multiset = tg.multiset;
@@ -253,10 +274,11 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
     r = prefix_rank<MAXD>(tg, d, q);
   }
   while (g < g_end) {
-    // ---- lane i: data of the block of prefix r + i ----
-    BlockInfo bi;
-    Streams<MAXD> st;
+    if (r >= a.S_pre) break;  // cannot happen while g < S; keeps the loop bounded regardless
+    // ---- lane i: data of the block of prefix r + i, into column i of the table ----
     {
+      BlockInfo bi;
+      Streams<MAXD> st;
       int q[MAXD];
       const uint64_t ri = r + (uint64_t)lane;
       const SpaceView sv{cg_l, lo_l, me, stride};
@@ -278,35 +300,61 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
           st.active[s] = false;
         }
       }
-    }
-    // stream slots: inactive ones read the first slot's address with a zero coefficient
-    const uint64_t base0 = st.base[0];
-    uint32_t modes[MAXD];
+      wave_sync();  // every lane is done reading the previous batch
+      uint32_t* col = tbl + lane;
+      col[0 * kBlockBatch] = (uint32_t)bi.CB;
+      col[1 * kBlockBatch] = (uint32_t)(bi.CB >> 32);
+      col[2 * kBlockBatch] = (uint32_t)bi.LBp;
+      col[3 * kBlockBatch] = (uint32_t)(bi.LBp >> 32);
+      col[4 * kBlockBatch] = bi.L;
+      col[5 * kBlockBatch] = bi.toff;
+      col[6 * kBlockBatch] = (uint32_t)bi.lo | ((uint32_t)bi.skip0 << 8) | ((uint32_t)bi.lastrun << 16);
+      col[7 * kBlockBatch] = __builtin_bit_cast(uint32_t, bi.fQ);
 #pragma unroll
-    for (int s = 0; s < MAXD; ++s) modes[s] = st.active[s] ? (uint32_t)st.mode[s] : (uint32_t)kNoEdge;
-    if (r >= a.S_pre) break;  // cannot happen while g < S; keeps the loop bounded regardless
-    const int nb = (int)((a.S_pre - r < 64) ? a.S_pre - r : 64);
-    for (int j = 0; j < nb && g < g_end; ++j) {
-      const uint32_t L = rl(bi.L, j);
-      if (L == 0) continue;                       // empty block (set masks only)
-      const uint64_t CB = rl64(bi.CB, j), LBp = rl64(bi.LBp, j);
-      const uint32_t toff = rl(bi.toff, j);
-      const int blo = (int)rl((uint32_t)bi.lo, j);
-      const int skip0 = (int)rl((uint32_t)bi.skip0, j);
-      float2 coef[MAXD];
-      const float2* sp[MAXD];
-      const uint64_t b0 = rl64(base0, j);
+      for (int w = 0; w < kModeW; ++w) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int s = 4 * w + k;
+          const uint32_t md = (s < MAXD && st.active[s < MAXD ? s : 0]) ? (uint32_t)st.mode[s < MAXD ? s : 0]
+                                                                        : (uint32_t)kNoEdge;
+          word |= (md & 0xffu) << (8 * k);
+        }
+        col[(8 + w) * kBlockBatch] = word;
+      }
 #pragma unroll
       for (int s = 0; s < MAXD; ++s) {
-        const uint32_t md = rl(modes[s], j);
+        const uint64_t base = st.active[s] ? st.base[s] : st.base[0];
+        col[(kBaseW + 2 * s) * kBlockBatch] = (uint32_t)base;
+        col[(kBaseW + 2 * s + 1) * kBlockBatch] = (uint32_t)(base >> 32);
+      }
+      wave_sync();
+    }
+    const int nb = (int)((a.S_pre - r < kBlockBatch) ? a.S_pre - r : kBlockBatch);
+    for (int j = 0; j < nb && g < g_end; ++j) {
+      const uint32_t* bj = tbl + j;
+      auto word = [&](int f) { return rfl(bj[f * kBlockBatch]); };
+      const uint32_t L = word(4);
+      if (L == 0) continue;                       // empty block (set masks only)
+      const uint64_t CB = (uint64_t)word(0) | ((uint64_t)word(1) << 32);
+      const uint64_t LBp = (uint64_t)word(2) | ((uint64_t)word(3) << 32);
+      const uint32_t toff = word(5);
+      const uint32_t w6 = word(6);
+      const int blo = (int)(w6 & 0xffu);
+      const int skip0 = (int)((w6 >> 8) & 1u);
+      float2 coef[MAXD];
+      const float2* sp[MAXD];
+#pragma unroll
+      for (int s = 0; s < MAXD; ++s) {
+        const uint32_t md = (word(8 + s / 4) >> (8 * (s % 4))) & 0xffu;
         coef[s] = uniform_f2(ucol[md]);
-        sp[s] = par + (md == (uint32_t)kNoEdge ? b0 : rl64(st.base[s], j));
+        sp[s] = par + ((uint64_t)word(kBaseW + 2 * s) | ((uint64_t)word(kBaseW + 2 * s + 1) << 32));
       }
       float fQ = 1.f;
       int lastrun = 0;
       if constexpr (KIND != 0) {
-        fQ = __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, bi.fQ), j));
-        lastrun = (int)rl((uint32_t)bi.lastrun, j);
+        fQ = __builtin_bit_cast(float, word(7));
+        lastrun = (int)(w6 >> 16);
       }
       const float2* lp = par + LBp;
       const TupleEntry* tb = tuples + toff;
@@ -330,14 +378,14 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
       g += u_stop - u0;
       u0 = 0;
     }
-    r += 64;
+    r += kBlockBatch;
   }
 }
 
 
 // KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes
 template <int KIND, bool COMP, int MAXD>
-__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(COMP ? 3 : 4))) void k_layer_block(const uint64_t* __restrict__ cg_t,
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(COMP || MAXD > 8 ? 3 : 5))) void k_layer_block(const uint64_t* __restrict__ cg_t,
                                                                const int* __restrict__ lo_t,
                                                                const TupleEntry* __restrict__ tuples,
                                                                int me, int stride, int multiset,
@@ -362,7 +410,9 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   const uint32_t wg_l = xcd_remap(blockIdx.x, gridDim.x);
   const uint64_t g = uniform_u64(((uint64_t)wg_l * kWaves + wv) * a.per_wave);
   char* out_b = reinterpret_cast<char*>(out_all) + (size_t)b * a.S * (KIND == 1 ? 4 : 8);
-  block_wave<KIND, COMP, MAXD>(tg, tl, par_all + (size_t)b * a.S_par, out_b, ucol, a, g);
+  uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + block_table_offset(K, me)) +
+                  (size_t)wv * kBlockBatch * block_words(MAXD);
+  block_wave<KIND, COMP, MAXD>(tg, tl, par_all + (size_t)b * a.S_par, out_b, ucol, tbl, a, g);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -394,6 +444,7 @@ struct GroupLaunch {
   GroupItem it[kMaxGroup];
   int count, K;
   uint32_t total;              // items of the launch
+  uint32_t table_off;          // block kernel: byte offset of the per-wave block tables in dynamic LDS
 };
 
 // the plan of work-group index `item` (plans in ascending wg_begin)
@@ -419,7 +470,7 @@ __device__ __forceinline__ void for_each_item(uint32_t total, F&& f) {
 }
 
 template <bool COMP, int MAXD>
-__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(COMP ? 3 : 4))) void
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(COMP || MAXD > 8 ? 3 : MAXD <= 5 ? 5 : 4))) void
 k_layer_block_group(GroupLaunch gl) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ float2 ucol[256];
@@ -451,7 +502,8 @@ k_layer_block_group(GroupLaunch gl) {
     const uint64_t g = uniform_u64(((uint64_t)w * kWaves + wv) * it.per_wave);
     const float2* par = it.par + (size_t)b * it.S_par;
     char* out = reinterpret_cast<char*>(it.out) + (size_t)b * it.S * (it.kind == 1 ? 4 : 8);
-    block_wave<3, COMP, MAXD>(tg, tl, par, out, ucol, a, g, it.kind);
+    uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + gl.table_off) + (size_t)wv * kBlockBatch * block_words(MAXD);
+    block_wave<3, COMP, MAXD>(tg, tl, par, out, ucol, tbl, a, g, it.kind);
   });
 }
 

@@ -424,6 +424,15 @@ __global__ __launch_bounds__(64) void k_reduce_gu(const double2* __restrict__ pa
   }
 }
 
+// the stream-slot count instantiated for prefix width d (exact where the big layers live)
+int block_maxd(int d) { return d <= 4 ? 4 : d <= 8 ? d : 16; }
+
+// dynamic LDS of the single-plan block kernel for layer K, or 0 if it does not fit 64 KB
+size_t block_lds_fit(int K, int me) {
+  const size_t sh = slos::block_dyn_lds(K, me, block_maxd(K - slos::kSuf));
+  return sh + slos::kBlockStaticLds <= 64 * 1024 ? sh : 0;
+}
+
 template <int KIND, bool COMP>
 void launch_block_kind(int maxd, dim3 grid, size_t sh, hipStream_t st, const DevTables& dt, const slos_plan* p,
                        const float2* par, void* out, const slos::BlockLayer& L) {
@@ -452,8 +461,8 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   const uint64_t wgs = (waves + slos::kWaves - 1) / slos::kWaves;
   if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
   if (a.B > 65535) return fail(SLOS_ERANGE, "batch > 65535");
-  const size_t sh = slos::block_lds_bytes(a.K, p->me);
-  if (sh + slos::kBlockStaticLds > 64 * 1024) return fail(SLOS_ERANGE, "block kernel LDS exceeds 64 KB");
+  const size_t sh = block_lds_fit(a.K, p->me);
+  if (!sh) return fail(SLOS_ERANGE, "block kernel LDS exceeds 64 KB");
   dim3 grid((unsigned)wgs, (unsigned)a.B);
   const int maxd = a.K - slos::kSuf;
   if (kind == 0) comp ? launch_block_kind<0, true>(maxd, grid, sh, st, dt, p, a.par, a.out, L)
@@ -474,7 +483,7 @@ int launch_layer(const slos_plan* p, const DevTables& dt, int K, int col, const 
   a.U = U; a.cols = nullptr; a.par = par; a.S_par = S_par; a.out = out; a.S = S; a.B = B;
   a.scale = scale; a.flags = flags;
   const bool comp = flags & SLOS_FLAG_COMPENSATED;
-  if (!(flags & SLOS_FLAG_SIMPLE) && dt.tuples && K > slos::kSuf && K - slos::kSuf <= 16)
+  if (!(flags & SLOS_FLAG_SIMPLE) && dt.tuples && K > slos::kSuf && K - slos::kSuf <= 16 && block_lds_fit(K, p->me))
     return launch_block(p, dt, a, kind, comp, st);
   const size_t sh = (((size_t)(K + 1) * (p->me + 1) * 8 + p->me * 4 + 15) & ~(size_t)15) + (size_t)p->m * 8;
   if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
@@ -644,9 +653,14 @@ int group_args(const slos_group* G, const void* const* U, const int64_t* B, floa
 
 bool group_can_block(const slos_group* G, const std::vector<DevTables>& dts, int K, unsigned flags) {
   if ((flags & SLOS_FLAG_SIMPLE) || K <= slos::kSuf || K - slos::kSuf > 16) return false;
+  size_t off = 0;
   for (size_t g = 0; g < G->plans.size(); ++g)
-    if (G->plans[g]->n >= K && !dts[g].tuples) return false;
-  return true;
+    if (G->plans[g]->n >= K) {
+      if (!dts[g].tuples) return false;
+      off = std::max(off, slos::block_table_offset(K, G->plans[g]->me));
+    }
+  const size_t sh = off + (size_t)slos::kWaves * slos::kBlockBatch * slos::block_words(block_maxd(K - slos::kSuf)) * 4;
+  return sh + slos::kBlockStaticLds <= 64 * 1024;
 }
 
 template <bool COMP>
@@ -702,7 +716,7 @@ int group_layer(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w
     if (block) {
       it.per_wave = c.per_wave ? c.per_wave : pw_default;
       wgs = ((it.S + it.per_wave - 1) / it.per_wave + slos::kWaves - 1) / slos::kWaves;
-      sh = std::max(sh, slos::block_lds_bytes(K, p->me));
+      sh = std::max(sh, slos::block_table_offset(K, p->me));
     } else {
       it.per_wave = c.per_wave ? c.per_wave : 1;
       wgs = (it.S + 256 * it.per_wave - 1) / (256 * it.per_wave);
@@ -716,6 +730,10 @@ int group_layer(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w
   if (cnt == 0) return SLOS_OK;
   gl.count = cnt;
   gl.total = (uint32_t)total;
+  if (block) {
+    gl.table_off = (uint32_t)sh;
+    sh += (size_t)slos::kWaves * slos::kBlockBatch * slos::block_words(block_maxd(K - slos::kSuf)) * 4;
+  }
   if (sh + (block ? slos::kBlockStaticLds : 0) > 64 * 1024) return fail(SLOS_ERANGE, "group layer LDS exceeds 64 KB");
   const uint32_t all = (uint32_t)(8 * ((total + 7) / 8));
   const uint32_t grid = c.grid ? std::min<uint32_t>(std::max<uint32_t>(c.grid & ~7u, 8), all) : all;
