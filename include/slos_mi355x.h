@@ -145,6 +145,14 @@ int slos_adjoint_layer(const slos_plan* plan, const void* U, int64_t batch, int 
 int slos_loss_descend(const slos_plan* plan, int k, const float* q_k, float* q_km1, int64_t batch,
                       slos_stream_t stream);
 
+/* Probability-level raise of partial distinguishability (species.classical_raise,
+ * slos/species.py:37-56): a photon that interferes with nothing enters column col, and
+ * q_k(c) = sum_{j distinct in c} w[b][j] q_{k-1}(c - e_j) with w[b][j] = |U[b, j, col]|^2 (device
+ * float32 [batch][m]), as a pull over parents instead of index_add over edges. q_km1: [batch][S_{k-1}],
+ * q_k: [batch][S_k], device float32. */
+int slos_raise(const slos_plan* plan, int k, const float* w, const float* q_km1, float* q_k, int64_t batch,
+               slos_stream_t stream);
+
 /* Adjoint of slos_loss_descend (the backward of imperfections._descend under autograd):
  * g_k(c) = sum_{j distinct in c} g_{k-1}(c - e_j) c_j / k. g_km1: [batch][S_{k-1}], g_k:
  * [batch][S_k], device float32. */

@@ -316,6 +316,80 @@ def prob_of_occupation(U, occ_in, occ_out) -> float:
 # uniform loss
 # ---------------------------------------------------------------------------------------------
 
+def gram_probs(U, n: int, m: int, occ, S, mask: str = "fock") -> np.ndarray:
+    """Partial distinguishability from the internal-state Gram matrix S (n x n, S[a, b] = <phi_a|phi_b>),
+    in the kernel's order: P(t) = (1 / prod t!) sum_{sigma, rho} prod_k M[k, sigma_k] conj(M[k, rho_k])
+    S[rho_k, sigma_k], M = U[output rows][:, input columns] (distinct input columns). S = all ones
+    gives |per M|^2 / prod t!, S = identity gives per(|M|^2) / prod t!. This is the closed form the
+    reference's tests check the x-model against (tests/test_imperfections.py:159-171); brute force
+    over permutation pairs, small n only."""
+    from itertools import permutations
+    U = np.asarray(U, dtype=np.complex128)
+    S = np.asarray(S, dtype=np.complex128)
+    ins = input_modes(occ)
+    assert len(set(ins)) == len(ins), "gram_probs: distinct input columns"
+    final = to_occupation(enumerate_layers(n, m, mask)[n], m)
+    perms = list(permutations(range(n)))
+    out = np.zeros(final.shape[0])
+    for idx, t in enumerate(final.tolist()):
+        rows = input_modes(t)
+        M = U[np.ix_(rows, ins)]
+        acc = 0j
+        for sg in perms:
+            a = np.prod([M[k, sg[k]] for k in range(n)])
+            for rh in perms:
+                acc += a * np.prod([np.conj(M[k, rh[k]]) * S[rh[k], sg[k]] for k in range(n)])
+        out[idx] = acc.real / math.prod(math.factorial(int(c)) for c in t)
+    return out
+
+
+def species_mixture(U, n: int, m: int, occ, x: float, mask: str = "fock", order=None) -> np.ndarray:
+    """The x-model as a mixture over interfering subsets, in the kernel's order. Restates
+    photonic_engines/slos/species.py:104-146 with permanents: the interfering photons G give
+    |per(U[t rows][:, G cols])|^2 / (prod s! prod t!) on the g-photon patterns of layer g; each other
+    photon is raised classically, q'(t + e_j) += q(t) |U[j, col]|^2; patterns outside the masked
+    layers drop their mass; weights x^|G| (1 - x)^(n - |G|); `order` keeps at most `order`
+    distinguishable photons. Dict-based, small n only."""
+    from itertools import combinations
+    U = np.asarray(U, dtype=np.complex128)
+    cols = input_modes(occ)
+    layers = [set(map(tuple, to_occupation(L, m).tolist())) for L in enumerate_layers(n, m, mask)]
+    b_max = n if order is None else max(0, min(int(order), n))
+    total: dict = {}
+    for b in range(b_max + 1):
+        w = x ** (n - b) * (1 - x) ** b
+        for bad in combinations(range(n), b):
+            good = [cols[j] for j in range(n) if j not in bad]
+            g = len(good)
+            if g >= 2:
+                s_norm = 1
+                for c in set(good):
+                    s_norm *= math.factorial(good.count(c))
+                dist = {}
+                for t in layers[g]:
+                    rows = input_modes(t)
+                    amp = permanent_ryser(U[np.ix_(rows, good)])
+                    dist[t] = abs(amp) ** 2 / (s_norm * math.prod(math.factorial(int(c)) for c in t))
+                k, singles = g, [cols[j] for j in bad]
+            else:
+                dist, k = {tuple([0] * m): 1.0}, 0
+                singles = [cols[j] for j in bad] + good
+            for c in singles:
+                new: dict = {}
+                for t, p in dist.items():
+                    for j in range(m):
+                        t2 = list(t)
+                        t2[j] += 1
+                        t2 = tuple(t2)
+                        if t2 in layers[k + 1]:
+                            new[t2] = new.get(t2, 0.0) + p * abs(U[j, c]) ** 2
+                dist, k = new, k + 1
+            for t, p in dist.items():
+                total[t] = total.get(t, 0.0) + w * p
+    final = to_occupation(enumerate_layers(n, m, mask)[n], m)
+    return np.array([total.get(tuple(t), 0.0) for t in final.tolist()])
+
+
 def binomial_pmf(n: int, eta: float) -> np.ndarray:
     """P(k of n survive), k = 0..n, through lgamma. Restates imperfections.py:106-111."""
     k = np.arange(n + 1, dtype=np.float64)

@@ -43,6 +43,7 @@ SIGNATURES = {
                                       _vp, _c.c_size_t, _vp]),
     "slos_loss_descend": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int64, _vp]),
     "slos_loss_ascend": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int64, _vp]),
+    "slos_raise": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _c.c_int64, _vp]),
     "slos_mass_workspace": (_c.c_int, [_c.c_int64, _c.c_uint64, _c.POINTER(_c.c_size_t)]),
     "slos_total_mass": (_c.c_int, [_vp, _c.c_int64, _c.c_uint64, _vp, _vp, _c.c_size_t, _vp]),
     "slos_spot_probs": (_c.c_int, [_vp, _vp, _vp, _c.c_int64, _vp, _vp]),

@@ -296,6 +296,34 @@ __global__ __launch_bounds__(256) void k_descend(const uint64_t* cg_g, const int
   }
 }
 
+// probability-level raise (a photon that interferes with nothing, partial distinguishability):
+// q_k(c) = sum_{j distinct in c} w[b][j] q_{k-1}(c - e_j), w = |U[:, col]|^2 (pull over parents,
+// ascending photon position, no atomics)
+__global__ __launch_bounds__(256) void k_raise(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k, int m,
+                                               const float* __restrict__ w, const float* __restrict__ qp, uint64_t Sp,
+                                               float* __restrict__ qout, uint64_t Sk, int64_t B) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int stride = me + 1;
+  const int ncg = (k + 1) * stride;
+  uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
+  int* lo = reinterpret_cast<int*>(cg + ncg);
+  float* wl = reinterpret_cast<float*>(lo + me);
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = cg_g[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo[i] = lo_g[i];
+  const slos::SpaceView sv{cg, lo, me, stride};
+  for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) wl[j] = w[(size_t)b * m + j];
+    __syncthreads();
+    const float* qb = qp + (size_t)b * Sp;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < Sk; c += (uint64_t)gridDim.x * blockDim.x) {
+      float acc = 0.f;
+      slos::for_each_parent(sv, multiset, k, c, [&](uint64_t pr, int mode, int) { acc = fmaf(wl[mode], qb[pr], acc); });
+      qout[(size_t)b * Sk + c] = acc;
+    }
+  }
+}
+
 // adjoint of the descent (gradient of q_{k-1} w.r.t. q_k): g_k[c] = sum_{j distinct in c}
 // g_{k-1}[c - e_j] * count_j(c) / k  — the forward pull structure with real weights
 __global__ __launch_bounds__(256) void k_ascend(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k,
@@ -1055,6 +1083,24 @@ int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1,
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 255) / 256, 1u << 20)), (unsigned)batch);
   hipLaunchKernelGGL(k_descend, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
                      p->multiset, k, q_k, Sk, q_km1, Sp, batch);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_raise(const slos_plan* p, int k, const float* w, const float* q_km1, float* q_k, int64_t batch,
+               slos_stream_t stream) {
+  if (!p || !w || !q_km1 || !q_k) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 1 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
+  DevTables dt;
+  int rc = device_tables(p, &dt);
+  if (rc) return rc;
+  const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
+  const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4 + (size_t)p->m * 4;
+  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sk + 255) / 256, 1u << 20)), (unsigned)batch);
+  hipLaunchKernelGGL(k_raise, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
+                     p->multiset, k, p->m, w, q_km1, Sp, q_k, Sk, batch);
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }

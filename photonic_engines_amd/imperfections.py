@@ -14,6 +14,8 @@ tensor with a gradient, and the descent has its own adjoint kernel for gradients
 """
 from __future__ import annotations
 
+import math
+
 from dataclasses import dataclass
 
 import numpy as np
@@ -46,31 +48,62 @@ def binomial_pmf(n: int, eta: Eta, dtype=torch.float64, device=None) -> torch.Te
 
 
 class Distinguishability:
-    """Minimal x-model marker for the `indist=` argument check (imperfections.py:440-526)."""
+    """Partial distinguishability (imperfections.py:465-506): the x-model (`x`, the pairwise
+    overlap of the photons' internal states, HOM visibility x^2) or random species (`species`,
+    accepted and validated; the strong simulation runs the x-model only)."""
 
-    def __init__(self, x: float = 1.0):
-        self.x = float(x)
+    def __init__(self, x: float | None = None, species=None):
+        if (x is None) == (species is None):
+            raise ValueError("give exactly one of x (x-model) or species (random species probabilities)")
+        self.x = None if x is None else float(x)
+        self.species = None
+        if self.x is not None and not 0.0 <= self.x <= 1.0:
+            raise ValueError(f"x must be in [0, 1], got {x}")
+        if species is not None:
+            q = tuple(float(v) for v in species)
+            if any(v < 0 for v in q) or sum(q) > 1.0 + 1e-12:
+                raise ValueError("species probabilities must be non-negative and sum to at most 1")
+            self.species = q
+
+    @staticmethod
+    def from_hom_visibility(V: float) -> "Distinguishability":
+        """x-model with pairwise overlap x = sqrt(V), V the two-photon HOM dip visibility."""
+        if not 0.0 <= float(V) <= 1.0:
+            raise ValueError(f"HOM visibility must be in [0, 1], got {V}")
+        return Distinguishability(x=math.sqrt(float(V)))
 
     @property
     def is_ideal(self) -> bool:
-        return self.x >= 1.0
+        if self.x is not None:
+            return self.x >= 1.0
+        return len(self.species) == 1 and self.species[0] >= 1.0
 
 
 def spec(indist):
-    """None / 1.0 / an ideal spec -> None (the lossless path); otherwise a non-ideal marker.
-    Overlaps outside [0, 1] raise ValueError (imperfections.py:508-526)."""
+    """None / 1.0 / an ideal spec -> None (the lossless path); otherwise the spec (a float is the
+    x-model). Overlaps outside [0, 1] raise ValueError (imperfections.py:508-526)."""
     if indist is None:
         return None
     if isinstance(indist, (list, tuple)) or (isinstance(indist, torch.Tensor) and indist.dim() == 1):
         xs = [float(v) for v in indist]
         if any(not 0.0 <= v <= 1.0 for v in xs):
             raise ValueError("per-photon overlaps must be in [0, 1]")
-        return None if all(v >= 1.0 for v in xs) else Distinguishability(min(xs))
+        return None if all(v >= 1.0 for v in xs) else Distinguishability(x=min(xs))
     d = indist if isinstance(indist, Distinguishability) else Distinguishability(
-        float(indist.detach()) if isinstance(indist, torch.Tensor) else float(indist))
-    if not 0.0 <= d.x <= 1.0:
-        raise ValueError("indistinguishability must be in [0, 1]")
+        x=float(indist.detach()) if isinstance(indist, torch.Tensor) else float(indist))
     return None if d.is_ideal else d
+
+
+def truncation_bound(n: int, indist, order: int | None) -> float:
+    """Mass dropped by the strong-side truncation `order=b` (imperfections.py:608-619): the
+    partitions with more than b distinguishable photons, P(Bin(n, 1 - x) > b)."""
+    d = spec(indist)
+    if d is None or order is None or order >= n:
+        return 0.0
+    if d.x is None:
+        raise NotImplementedError("truncation_bound: x-model only")
+    p = 1.0 - d.x
+    return float(sum(math.comb(n, b) * p ** b * (1 - p) ** (n - b) for b in range(order + 1, n + 1)))
 
 
 MULTINOMIAL_MAX = 1 << 24

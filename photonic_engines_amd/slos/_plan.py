@@ -138,6 +138,13 @@ class Plan:
                                              nat.stream_handle(g.device)), "slos_loss_ascend")
         return out
 
+    def raise_layer(self, k: int, w: torch.Tensor, q: torch.Tensor) -> torch.Tensor:
+        """q_k(c) = sum_{j distinct in c} w[b, j] q_{k-1}(c - e_j): w [B, m], q [B, S_{k-1}] float32 (GPU)."""
+        out = torch.empty((q.shape[0], self.sizes[k]), dtype=torch.float32, device=q.device)
+        nat.check(self._lib.slos_raise(self._h, k, nat.ptr(w), nat.ptr(q), nat.ptr(out), q.shape[0],
+                                       nat.stream_handle(q.device)), "slos_raise")
+        return out
+
     def total_mass(self, probs: torch.Tensor) -> torch.Tensor:
         """FP64 sum over the last axis of float32 probabilities [B, S] -> float64 [B]."""
         B, S = probs.shape

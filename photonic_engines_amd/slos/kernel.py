@@ -164,12 +164,20 @@ class FockSLOSKernel:
     # ---- probabilities ------------------------------------------------------------------------------
     def probs(self, U: torch.Tensor, loss=None, indist=None, order: int | None = None):
         """Output probabilities over the final basis (kernel.py:262-283). With `loss=eta` the
-        uniform-loss LossyDistribution over 0..n photons is returned instead."""
+        uniform-loss LossyDistribution over 0..n photons is returned instead. With `indist=x`
+        (partial distinguishability, x-model) the distribution is the mixture over the interfering
+        subsets of the photons (slos/species.py:123-146, `species.x_mixture`); `order=b` keeps at
+        most b distinguishable photons; a tensor x gets a gradient; loss is applied after."""
         if indist is not None:
             from ..imperfections import spec
             if spec(indist) is not None:
-                raise NotImplementedError("indist= (partial distinguishability) is not implemented by the "
-                                          "MI355X engine")
+                from .species import x_mixture
+                try:
+                    p = x_mixture(self, U, indist, order=order)
+                except torch.cuda.OutOfMemoryError as e:
+                    self._add_oom_hint(e)
+                    raise
+                return _maybe_thin(self.layers, p, loss)
         try:
             if U.requires_grad and torch.is_grad_enabled():
                 from .autograd import SLOSProbs

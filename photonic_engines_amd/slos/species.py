@@ -1,0 +1,128 @@
+"""Strong simulation under partial distinguishability on MI355X: the FockSLOSKernel part of
+photonic_engines/slos/species.py (`layer_probs` :84-101, `classical_raise` :37-56,
+`convolve_layers` :64-81, `species_probs` :104-120, `x_mixture` :123-146).
+
+A shot is a species partition of the input photons: the photons of one species interfere among
+themselves, different species do not, and the detector sees the sum of the species' patterns.
+On the layered basis of the kernel:
+
+  * a species of g >= 2 photons is the SLOS recurrence on its columns stopped on layer g of the
+    kernel's basis, squared with that layer's deferred normalisation (one slos_forward call with
+    the species' columns, `layer_probs`);
+  * a photon of a species of its own is a probability-level raise along the k -> k+1 edges,
+    q_{k+1}(t + e_j) += q_k(t) |U[j, col]|^2, with no bosonic factor (slos_raise: each child
+    gathers its parents; `classical_raise`);
+  * two multi-photon species combine by the explicit convolution of their layers (small n,
+    `convolve_layers`).
+
+`x_mixture` is the x-model: the sum over the interfering subsets G of the input photons with
+weights x^|G| (1 - x)^(n - |G|), each a `species_probs`; `order=b` keeps at most b
+distinguishable photons (the dropped mass is `imperfections.truncation_bound`). A tensor x gets
+a gradient (the weights are torch operations); gradients in U are not provided on this path.
+"""
+from __future__ import annotations
+
+import math
+from collections.abc import Sequence
+from itertools import combinations
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+from ..imperfections import spec
+
+
+def _batched_u(kernel, U: torch.Tensor):
+    """U [m,m] | [B,m,m] -> (contiguous complex64 [B,m,m] on the compute GPU, was batched)."""
+    if U.requires_grad:
+        raise NotImplementedError("probs(indist=...): gradients in U are not provided on the "
+                                  "partial-distinguishability path (a tensor indist x gets one)")
+    Ub, batched = kernel._prepare(U.detach())
+    return Ub, batched
+
+
+def layer_probs(kernel, Ub: torch.Tensor, cols: Sequence[int]) -> torch.Tensor:
+    """|<t| U |cols>|^2 on layer g = len(cols) of the kernel's basis, [B, S_g] float32: one forward
+    over the columns `cols` with the input norm 1/sqrt(prod s_i!) and the layer's prod t_j!.
+    g = 0 is the vacuum (probability 1)."""
+    g = len(cols)
+    B = Ub.shape[0]
+    if g == 0:
+        return torch.ones((B, 1), dtype=torch.float32, device=Ub.device)
+    counts: dict = {}
+    for c in cols:
+        counts[c] = counts.get(c, 0) + 1
+    s_norm = math.prod(math.factorial(v) for v in counts.values())
+    flags = nat.FLAG_COMPENSATED if kernel.compensated else 0
+    return kernel._plan.forward(Ub, modes=[int(c) for c in cols], layer=g, out_kind=nat.OUT_PROBS,
+                                scale=1.0 / math.sqrt(s_norm), flags=flags)
+
+
+def classical_raise(kernel, q: torch.Tensor, k: int, col: int, Ub: torch.Tensor) -> torch.Tensor:
+    """One photon that interferes with nothing enters column `col`: the layer-k distribution
+    q [B, S_k] becomes the layer-(k+1) one, q'(t + e_j) = sum q(t) |U[j, col]|^2."""
+    w = (Ub[:, :, int(col)].abs() ** 2).to(torch.float32).contiguous()
+    return kernel._plan.raise_layer(k + 1, w, q.contiguous())
+
+
+def convolve_layers(kernel, qa: torch.Tensor, ka: int, qb: torch.Tensor, kb: int) -> torch.Tensor:
+    """Distribution of the sum of two independent patterns (layers ka and kb, [B, S] each) on
+    layer ka + kb: q(t) = sum_{ta + tb = t} qa(ta) qb(tb); a sum outside the (masked) basis drops
+    its mass. Explicit S_a x S_b pairs: small layers only."""
+    layers = kernel.layers
+    occ_a, occ_b = layers.occupations(ka), layers.occupations(kb)
+    occ_t = layers.occupations(ka + kb)
+    index = {tuple(r): i for i, r in enumerate(occ_t.tolist())}
+    ia, ib, tg = [], [], []
+    for i, ta in enumerate(occ_a.tolist()):
+        for j, tb in enumerate(occ_b.tolist()):
+            t = index.get(tuple(x + y for x, y in zip(ta, tb)))
+            if t is not None:
+                ia.append(i); ib.append(j); tg.append(t)
+    dev = qa.device
+    ia_t, ib_t, tg_t = (torch.as_tensor(np.asarray(v, dtype=np.int64), device=dev) for v in (ia, ib, tg))
+    out = torch.zeros((qa.shape[0], occ_t.shape[0]), dtype=qa.dtype, device=dev)
+    return out.index_add(-1, tg_t, qa.index_select(-1, ia_t) * qb.index_select(-1, ib_t))
+
+
+def species_probs(kernel, Ub: torch.Tensor, groups: Sequence[Sequence[int]], singles: Sequence[int]) -> torch.Tensor:
+    """Distribution [B, S_n] for a species partition: `groups` the columns of each multi-photon
+    species, `singles` the columns of the photons of a species of their own."""
+    groups = [list(g) for g in groups]
+    singles = list(singles) + [g[0] for g in groups if len(g) == 1]
+    groups = sorted((g for g in groups if len(g) >= 2), key=len, reverse=True)
+    q = layer_probs(kernel, Ub, groups[0] if groups else [])
+    k = len(groups[0]) if groups else 0
+    for g in groups[1:]:
+        q, k = convolve_layers(kernel, q, k, layer_probs(kernel, Ub, g), len(g)), k + len(g)
+    for c in singles:
+        q, k = classical_raise(kernel, q, k, c, Ub), k + 1
+    if k != kernel.n:
+        raise ValueError(f"the partition covers {k} photons, the kernel has {kernel.n}")
+    return q
+
+
+def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torch.Tensor:
+    """The x-model distribution on the final layer ([S] or [B, S], float32): the sum over the
+    interfering subsets G of the input photons of x^|G| (1 - x)^(n - |G|) species_probs(G, rest).
+    `indist` a float, a Distinguishability(x=...) or a tensor x (differentiable); `order=b` keeps
+    the subsets with at most b distinguishable photons."""
+    d = spec(indist)
+    if d.x is None:
+        raise NotImplementedError("strong simulation: the x-model only (random species need a partition sum)")
+    Ub, batched = _batched_u(kernel, U)
+    x = indist if isinstance(indist, torch.Tensor) else torch.tensor(d.x, dtype=torch.float32)
+    x = x.to(dtype=torch.float32, device=Ub.device)
+    cols = kernel.input_modes
+    n = len(cols)
+    b_max = n if order is None else max(0, min(int(order), n))
+    out = None
+    for b in range(b_max + 1):
+        w = x ** (n - b) * (1 - x) ** b
+        for bad in combinations(range(n), b):
+            good = [cols[j] for j in range(n) if j not in bad]
+            term = species_probs(kernel, Ub, [good] if len(good) >= 2 else [],
+                                 [cols[j] for j in bad] + (good if len(good) == 1 else []))
+            out = term * w if out is None else out + term * w
+    return kernel._out(out, batched)

@@ -253,3 +253,60 @@ def test_golden_c4_fd_gradient_matches_autograd_restatement():
     g = U.grad.numpy()
     assert np.abs(g - g_fd).max() < 1e-6 * np.abs(g_fd).max()
     assert np.abs(g[:, 6:]).max() == 0.0                 # only the input columns receive gradient
+
+
+# ---- partial distinguishability (x-model) ---------------------------------------------------------
+
+def _classical(U, occ_in, occ_out):
+    """Distinguishable photons: per(|U|^2[out rows][:, in cols]) / prod t! (multinomial transport)."""
+    ins, outs = so.input_modes(occ_in), so.input_modes(occ_out)
+    W = np.abs(np.asarray(U)) ** 2
+    return so.permanent_brute(W[np.ix_(outs, ins)]).real / math.prod(math.factorial(int(c)) for c in occ_out)
+
+
+def test_gram_limits_are_permanents():
+    """S = all ones: |per M|^2 / prod t!; S = identity: per(|M|^2) / prod t! (two independent routes)."""
+    n, m = 3, 5
+    U = so.haar_unitary(m, 21)
+    occ = [1, 0, 1, 1, 0]
+    final = so.to_occupation(so.enumerate_layers(n, m)[n], m)
+    ones = so.gram_probs(U, n, m, occ, np.ones((n, n)))
+    assert np.abs(ones - so.reference_probs(U, n, m, occ)).max() < 1e-13
+    eye = so.gram_probs(U, n, m, occ, np.eye(n))
+    assert np.abs(eye - np.array([_classical(U, occ, t) for t in final.tolist()])).max() < 1e-13
+    assert abs(ones.sum() - 1) < 1e-12 and abs(eye.sum() - 1) < 1e-12
+
+
+@pytest.mark.parametrize("n,m,seed,x", [(3, 5, 1, 0.7), (4, 5, 2, 0.35), (3, 6, 3, 0.0), (3, 6, 4, 1.0)])
+def test_species_mixture_equals_gram(n, m, seed, x):
+    """The x-model as a mixture over interfering subsets equals the Gram-matrix distribution with
+    S = x J + (1 - x) I (the identity the reference's strong path rests on)."""
+    U = so.haar_unitary(m, seed)
+    occ = [1] * n + [0] * (m - n)
+    S = x * np.ones((n, n)) + (1 - x) * np.eye(n)
+    mix = so.species_mixture(U, n, m, occ, x)
+    assert np.abs(mix - so.gram_probs(U, n, m, occ, S)).max() < 1e-13
+
+
+def test_species_mixture_limits_bunched_and_truncation():
+    n, m = 3, 6
+    U = so.haar_unitary(m, 5)
+    occ = [2, 0, 1, 0, 0, 0]
+    final = so.to_occupation(so.enumerate_layers(n, m)[n], m)
+    assert np.abs(so.species_mixture(U, n, m, occ, 1.0) - so.reference_probs(U, n, m, occ)).max() < 1e-13
+    cl = np.array([_classical(U, occ, t) for t in final.tolist()])
+    assert np.abs(so.species_mixture(U, n, m, occ, 0.0) - cl).max() < 1e-13
+    x = 0.6
+    for b in range(n + 1):
+        dropped = sum(math.comb(n, c) * (1 - x) ** c * x ** (n - c) for c in range(b + 1, n + 1))
+        assert abs(so.species_mixture(U, n, m, occ, x, order=b).sum() - (1 - dropped)) < 1e-12
+
+
+def test_species_mixture_catches_a_planted_mistake():
+    """A raise with |U[col, j]|^2 (transposed U) instead of |U[j, col]|^2 breaks the Gram identity."""
+    n, m, x = 3, 5, 0.5
+    U = so.haar_unitary(m, 9)
+    occ = [1, 1, 1, 0, 0]
+    S = x * np.ones((n, n)) + (1 - x) * np.eye(n)
+    bad = so.species_mixture(np.asarray(U).T, n, m, [1, 1, 1, 0, 0], x)
+    assert np.abs(bad - so.gram_probs(U, n, m, occ, S)).max() > 1e-3
