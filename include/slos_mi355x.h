@@ -172,6 +172,26 @@ int slos_total_mass(const float* probs, int64_t batch, uint64_t count, double* o
 int slos_spot_probs(const slos_plan* plan, const void* U_c128, const uint64_t* idx, int64_t count,
                     double* out, slos_stream_t stream);
 
+/* ---- strong-then-sample --------------------------------------------------------------------
+ * Draws of output indices from float32 probabilities on the device (the inverse-CDF branch of
+ * draw_indices, imperfections.py:221-237, used by sample_from_probs, slos/emitter.py:215-230, and
+ * LossyDistribution.sample, imperfections.py:213-218), without a float64 copy of the [count]
+ * vector: FP64 sums of 64-entry chunks (in index order) and 4096-entry blocks (in chunk order), the
+ * block CDF (sequential within segments of 1024 blocks, plus the sequential prefix of the segment
+ * totals), then per shot u = uniforms[s] * total: the first block whose CDF reaches u, the first
+ * chunk and entry whose in-order prefix reaches the remainder (clamped to the block / count end).
+ * probs: device float32 [count] (non-negative, unnormalised); uniforms: device float64 [shots] in
+ * [0, 1); idx: device uint64 [shots]. count <= 2^32. */
+int slos_sample_workspace(uint64_t count, size_t* bytes);
+int slos_sample(const float* probs, uint64_t count, const double* uniforms, int64_t shots, uint64_t* idx,
+                void* workspace, size_t workspace_bytes, slos_stream_t stream);
+
+/* Device unrank of layer k: occ (device int32 [count][m]) is the occupation row of each lex index
+ * (device uint64 [count]); an index >= S_k gives a row of -1. The device form of slos_unrank for
+ * sampled indices (occupations as in sample_from_probs, slos/emitter.py:228-229). */
+int slos_unrank_device(const slos_plan* plan, int k, const uint64_t* idx, int64_t count, int32_t* occ,
+                       slos_stream_t stream);
+
 /* ---- uneven batches ------------------------------------------------------------------------
  * A group runs many instances of up to 8 plans of different (n, m, input, mask) together: one
  * launch per photon layer K covers every instance of every plan with n >= K (batched U, as

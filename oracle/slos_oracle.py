@@ -390,6 +390,53 @@ def species_mixture(U, n: int, m: int, occ, x: float, mask: str = "fock", order=
     return np.array([total.get(tuple(t), 0.0) for t in final.tolist()])
 
 
+def draw_indices_cumsum(p, uniforms) -> np.ndarray:
+    """The inverse-CDF branch of draw_indices (photonic_engines/imperfections.py:234-237): a float64
+    cumulative sum, u = uniform * total, the first index whose CDF reaches u, clamped to S - 1."""
+    cdf = np.cumsum(np.asarray(p, dtype=np.float64))
+    u = np.asarray(uniforms, dtype=np.float64) * cdf[-1]
+    return np.minimum(np.searchsorted(cdf, u, side="left"), cdf.size - 1)
+
+
+def draw_indices_two_level(p, uniforms, chunk: int = 64, chunks: int = 64, seg: int = 1024) -> np.ndarray:
+    """The device sampler's inverse CDF with the same FP64 operations in the same order: chunk sums
+    (in index order), block sums (in chunk order), block CDF (in-order within segments of `seg`
+    blocks, plus the in-order exclusive prefix of the segment totals), then per draw the block, the
+    chunk and the entry whose in-order prefix first reaches the remainder. Indices equal the plain
+    cumulative-sum ones except where u lies within rounding of a CDF step."""
+    p = np.asarray(p, dtype=np.float32).astype(np.float64)
+    S = p.size
+    blk = chunk * chunks
+    nb = -(-S // blk)
+    q = np.zeros(nb * blk)
+    q[:S] = p
+    cum_e = np.cumsum(q.reshape(nb, chunks, chunk), axis=2)          # in-order within a chunk
+    csum = cum_e[:, :, -1]
+    cum_c = np.cumsum(csum, axis=1)                                  # in-order over the chunks
+    bsum = cum_c[:, -1]
+    nseg = -(-nb // seg)
+    bpad = np.zeros(nseg * seg)
+    bpad[:nb] = bsum
+    local = np.cumsum(bpad.reshape(nseg, seg), axis=1)
+    tot = local[:, -1]
+    off = np.concatenate([[0.0], np.cumsum(tot)[:-1]])               # run starts at 0.0, in order
+    cdf = (off[:, None] + local).reshape(-1)[:nb]
+    u = np.asarray(uniforms, dtype=np.float64) * cdf[-1]
+    b = np.minimum(np.searchsorted(cdf, u, side="left"), nb - 1)
+    r = u - np.where(b > 0, cdf[np.maximum(b - 1, 0)], 0.0)
+    out = np.empty(u.size, dtype=np.int64)
+    for s in range(u.size):
+        cc = cum_c[b[s]]
+        hit = np.nonzero(cc >= r[s])[0]
+        c = int(hit[0]) if hit.size else chunks - 1
+        r2 = r[s] - (cc[c - 1] if c > 0 else 0.0)
+        ce = cum_e[b[s], c]
+        hit = np.nonzero(ce >= r2)[0]
+        k = int(hit[0]) if hit.size else chunk - 1
+        out[s] = min(int(b[s]) * blk + c * chunk + k, S - 1)
+    return out
+
+
 def binomial_pmf(n: int, eta: float) -> np.ndarray:
     """P(k of n survive), k = 0..n, through lgamma. Restates imperfections.py:106-111."""
     k = np.arange(n + 1, dtype=np.float64)

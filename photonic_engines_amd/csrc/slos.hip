@@ -549,6 +549,139 @@ int forward_layout(const slos_plan* p, int64_t B, int layer, WsLayout* w) {
 }
 
 // ------------------------------------------------------------------------------------------
+// strong-then-sample: two-level inverse CDF over float32 probabilities in FP64, every addition in
+// a fixed order (chunk = 64 entries summed in index order, block = 64 chunks summed in chunk
+// order, block prefix = sequential inside segments of kSampleSeg blocks plus the sequential
+// prefix of the segment totals), then the per-shot search. Replaces the float64 cumsum +
+// searchsorted of imperfections.py:221-237 (above 2^24 categories) without an [S] float64 copy.
+// ------------------------------------------------------------------------------------------
+constexpr int kSampleChunk = 64;
+constexpr int kSampleBlock = kSampleChunk * 64;   // 4096 entries
+constexpr int kSampleSeg = 1024;                   // blocks per prefix segment
+
+__device__ __forceinline__ double chunk_sum(const float* __restrict__ p, uint64_t S, uint64_t first) {
+  double acc = 0.0;
+  for (int k = 0; k < kSampleChunk; ++k) {
+    const uint64_t i = first + (uint64_t)k;
+    acc += i < S ? (double)p[i] : 0.0;
+  }
+  return acc;
+}
+
+// one wave per block: lane c sums chunk c, lane 0 sums the 64 chunk sums in order
+__global__ __launch_bounds__(256) void k_sample_block_sums(const float* __restrict__ p, uint64_t S, uint64_t nb,
+                                                           double* __restrict__ bsum) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t b = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (b >= nb) return;
+  const double cs = chunk_sum(p, S, b * kSampleBlock + (uint64_t)lane * kSampleChunk);
+  double tot = 0.0;
+  for (int c = 0; c < 64; ++c) tot += __shfl(cs, c, 64);
+  if (lane == 0) bsum[b] = tot;
+}
+
+// segment prefix: thread t owns blocks [t*kSampleSeg, (t+1)*kSampleSeg): local inclusive prefix;
+// then the exclusive prefix of the segment totals (thread 0, in order), then cdf = offset + local
+__global__ __launch_bounds__(1024) void k_sample_block_cdf(const double* __restrict__ bsum, uint64_t nb,
+                                                            double* __restrict__ cdf) {
+  __shared__ double tot[1024];
+  const uint64_t t = threadIdx.x;
+  const uint64_t a = t * kSampleSeg;
+  double acc = 0.0;
+  for (uint64_t i = a; i < a + kSampleSeg && i < nb; ++i) {
+    acc += bsum[i];
+    cdf[i] = acc;
+  }
+  tot[t] = acc;
+  __syncthreads();
+  if (t == 0) {
+    double run = 0.0;
+    for (int i = 0; i < 1024; ++i) {
+      const double v = tot[i];
+      tot[i] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  const double off = tot[t];
+  for (uint64_t i = a; i < a + kSampleSeg && i < nb; ++i) cdf[i] = off + cdf[i];
+}
+
+// one wave per shot: u = uniform * total; block = first cdf >= u (binary search), then the chunk
+// and the entry by in-order prefix sums; every lane computes the same values (lane 0 stores)
+__global__ __launch_bounds__(256) void k_sample_search(const float* __restrict__ p, uint64_t S, uint64_t nb,
+                                                       const double* __restrict__ cdf, const double* __restrict__ uni,
+                                                       int64_t shots, uint64_t* __restrict__ idx) {
+  const int lane = threadIdx.x & 63;
+  const int64_t s = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (s >= shots) return;
+  const double total = cdf[nb - 1];
+  const double u = uni[s] * total;
+  uint64_t lo = 0, hi = nb - 1;  // first b with cdf[b] >= u, else nb - 1
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (cdf[mid] >= u) hi = mid;
+    else lo = mid + 1;
+  }
+  const uint64_t b = lo;
+  double r = u - (b > 0 ? cdf[b - 1] : 0.0);
+  const double cs = chunk_sum(p, S, b * kSampleBlock + (uint64_t)lane * kSampleChunk);
+  int c = 63;
+  double pre = 0.0;
+  for (int q = 0; q < 64; ++q) {
+    const double v = __shfl(cs, q, 64);
+    if (pre + v >= r) {
+      c = q;
+      break;
+    }
+    pre += v;
+  }
+  r -= pre;
+  const uint64_t first = b * kSampleBlock + (uint64_t)c * kSampleChunk;
+  int k = 63;
+  double acc = 0.0;
+  for (int q = 0; q < kSampleChunk; ++q) {
+    const uint64_t i = first + (uint64_t)q;
+    acc += i < S ? (double)p[i] : 0.0;
+    if (acc >= r) {
+      k = q;
+      break;
+    }
+  }
+  uint64_t out = first + (uint64_t)k;
+  if (out >= S) out = S - 1;
+  if (lane == 0) idx[s] = out;
+}
+
+// lex index -> occupation row (int32 [m]) of layer k, one thread per index
+// (an index >= S_k gives a row of -1)
+__global__ __launch_bounds__(256) void k_unrank_occ(const uint64_t* cg_g, const int* lo_g, int me, int k, int m,
+                                                    const uint64_t* __restrict__ idx, int64_t count,
+                                                    int32_t* __restrict__ occ) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int stride = me + 1;
+  const int ncg = (k + 1) * stride;
+  uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
+  int* lo = reinterpret_cast<int*>(cg + ncg);
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = cg_g[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo[i] = lo_g[i];
+  __syncthreads();
+  const slos::SpaceView sv{cg, lo, me, stride};
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < count; s += (int64_t)gridDim.x * blockDim.x) {
+    int32_t* row = occ + (size_t)s * m;
+    const uint64_t g = idx[s];
+    if (g >= cg[(size_t)k * stride]) {
+      for (int j = 0; j < m; ++j) row[j] = -1;
+      continue;
+    }
+    int pos[kMaxPhotons];
+    sv.unrank(g, k, pos);
+    for (int j = 0; j < m; ++j) row[j] = 0;
+    for (int i = 0; i < k; ++i) row[pos[i]] += 1;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // uneven batches: a group of plans, one launch per photon layer (see layer_block.hpp)
 // ------------------------------------------------------------------------------------------
 
@@ -1153,6 +1286,57 @@ int slos_spot_probs(const slos_plan* p, const void* U, const uint64_t* idx, int6
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_spot_ryser, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, st, dt.cg, dt.lo, p->me, p->n,
                      p->m, static_cast<const double2*>(U), dt.modes, p->s_fact, p->multiset, idx, count, out);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_sample_workspace(uint64_t count, size_t* bytes) {
+  if (!bytes) return fail(SLOS_EINVAL, "NULL argument");
+  if (count < 1) return fail(SLOS_EINVAL, "count must be >= 1");
+  const uint64_t nb = (count + kSampleBlock - 1) / kSampleBlock;
+  if (nb > (uint64_t)kSampleSeg * 1024) return fail(SLOS_ERANGE, "too many categories for the sampler");
+  *bytes = 2 * align256(nb * sizeof(double));
+  return SLOS_OK;
+}
+
+int slos_sample(const float* probs, uint64_t count, const double* uniforms, int64_t shots, uint64_t* idx, void* ws,
+                size_t ws_bytes, slos_stream_t stream) {
+  if (shots < 0) return fail(SLOS_EINVAL, "shots must be >= 0");
+  if (!probs || (shots > 0 && (!uniforms || !idx))) return fail(SLOS_EINVAL, "NULL argument");
+  size_t need = 0;
+  int rc = slos_sample_workspace(count, &need);
+  if (rc) return rc;
+  if (!ws || ws_bytes < need) return fail(SLOS_ENOMEM, "workspace too small: need " + std::to_string(need) + " bytes");
+  if (shots == 0) return SLOS_OK;
+  const uint64_t nb = (count + kSampleBlock - 1) / kSampleBlock;
+  double* bsum = static_cast<double*>(ws);
+  double* cdf = reinterpret_cast<double*>(static_cast<unsigned char*>(ws) + align256(nb * sizeof(double)));
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_sample_block_sums, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, st, probs, count, nb, bsum);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_sample_block_cdf, dim3(1), dim3(1024), 0, st, bsum, nb, cdf);
+  HIP_TRY(hipGetLastError());
+  const uint64_t g = ((uint64_t)shots + 3) / 4;
+  if (g > 0x7fffffffull) return fail(SLOS_ERANGE, "too many shots for one launch");
+  hipLaunchKernelGGL(k_sample_search, dim3((unsigned)g), dim3(256), 0, st, probs, count, nb, cdf, uniforms, shots, idx);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_unrank_device(const slos_plan* p, int k, const uint64_t* idx, int64_t count, int32_t* occ,
+                       slos_stream_t stream) {
+  if (!p || !idx || !occ) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 0 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (count < 0) return fail(SLOS_EINVAL, "count must be >= 0");
+  if (count == 0) return SLOS_OK;
+  DevTables dt;
+  int rc = device_tables(p, &dt);
+  if (rc) return rc;
+  const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
+  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  const uint64_t blocks = std::min<uint64_t>(((uint64_t)count + 255) / 256, 1u << 16);
+  hipLaunchKernelGGL(k_unrank_occ, dim3((unsigned)blocks), dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg,
+                     dt.lo, p->me, k, p->m, idx, count, occ);
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }

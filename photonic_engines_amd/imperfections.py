@@ -119,6 +119,17 @@ def draw_indices(p: torch.Tensor, shots: int, generator: torch.Generator | None 
     return torch.searchsorted(cdf, u).clamp_(max=p.numel() - 1)
 
 
+def thin_counts(counts: torch.Tensor, eta, generator: torch.Generator | None = None) -> torch.Tensor:
+    """Output-side loss on samples (imperfections.py:116-126): every photon of counts [..., m] is kept
+    independently with probability eta (binomial draws on the host). Same shape and dtype."""
+    e = _eta(eta)
+    if e is None:
+        return counts
+    p = torch.full(counts.shape, float(e), dtype=torch.float64, device="cpu")
+    kept = torch.binomial(counts.to("cpu", torch.float64), p, generator=generator)
+    return kept.to(counts.device, counts.dtype)
+
+
 def thin_to_clicks(occupations, probs: torch.Tensor) -> tuple[np.ndarray, torch.Tensor]:
     """Group a pattern distribution by click pattern (imperfections.py:429-438)."""
     occ = np.asarray(occupations)
@@ -172,9 +183,10 @@ class LossyDistribution:
         return thin_to_clicks(occ, p)
 
     def sample(self, shots: int, generator: torch.Generator | None = None) -> torch.Tensor:
-        occ, p = self.flat()
-        if p.dim() != 1:
+        """Samples [shots, m] (one distribution) (imperfections.py:213-218): the flat host draw."""
+        if any(q.dim() != 1 for q in self.probs):
             raise ValueError("one distribution at a time")
+        occ, p = self.flat()
         return torch.as_tensor(occ)[draw_indices(p.detach().cpu().to(torch.float64), shots, generator)]
 
 

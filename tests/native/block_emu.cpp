@@ -228,3 +228,31 @@ int emu_xcd_remap_ok(uint32_t max_nwg) {
 }
 
 }  // extern "C"
+
+extern "C" {
+// every child rank that slos::for_each_child reports for parents [first, first + count) of layer
+// k-1 equals the rank of the child's own sorted positions and is < S_k; returns the number of
+// mismatches (0: all correct), -1 on bad input
+int64_t emu_check_children(int n, int m, int mask, int k, uint64_t first, uint64_t count) {
+  Space s;
+  if (!build(n, m, mask, s) || k < 1 || k > n) return -1;
+  const int W = s.me + 1;
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, W};
+  const uint64_t Sp = s.cg[(size_t)(k - 1) * W], Sk = s.cg[(size_t)k * W];
+  int64_t bad = 0;
+  for (uint64_t t = first; t < first + count && t < Sp; ++t) {
+    int p[64], c[64];
+    if (k > 1) sv.unrank(t, k - 1, p);
+    slos::for_each_child(sv, s.multiset, k, t, [&](uint64_t r, int j, int) {
+      int q = 0, placed = 0;
+      for (int i = 0; i < k - 1; ++i) {
+        if (!placed && j < p[i]) { c[q++] = j; placed = 1; }
+        c[q++] = p[i];
+      }
+      if (!placed) c[q++] = j;
+      if (r >= Sk || sv.rank(c, k) != r) ++bad;
+    });
+  }
+  return bad;
+}
+}

@@ -310,3 +310,37 @@ def test_species_mixture_catches_a_planted_mistake():
     S = x * np.ones((n, n)) + (1 - x) * np.eye(n)
     bad = so.species_mixture(np.asarray(U).T, n, m, [1, 1, 1, 0, 0], x)
     assert np.abs(bad - so.gram_probs(U, n, m, occ, S)).max() > 1e-3
+
+
+# ---- strong-then-sample: the two-level inverse CDF -----------------------------------------------
+
+def test_two_level_inverse_cdf_matches_the_cumsum_draw():
+    """The device sampler's operation order draws the same indices as the reference's float64 cumsum +
+    searchsorted (imperfections.py:234-237) except within rounding of a CDF step; zero-probability
+    entries and whole zero blocks are never drawn; the edge uniforms land on the first / last support."""
+    rng = np.random.default_rng(3)
+    S = 3 * 4096 * 1024 + 777                      # several prefix segments, ragged tail
+    p = rng.random(S).astype(np.float32) ** 4
+    p[rng.random(S) < 0.2] = 0
+    p[5 * 4096:9 * 4096] = 0                       # empty blocks
+    u = np.concatenate([[0.0, 1.0 - 2 ** -53], rng.random(4000)])
+    a = so.draw_indices_two_level(p, u)
+    b = so.draw_indices_cumsum(p, u)
+    cdf = np.cumsum(p.astype(np.float64))
+    near = np.abs(cdf[np.minimum(b, S - 1)] - u * cdf[-1]) < 1e-9 * cdf[-1]
+    assert np.all((a == b) | near) and (a != b).sum() <= 2
+    assert np.all(p[a] > 0)
+    assert a[0] == np.nonzero(p)[0][0] and a[1] == np.nonzero(p)[0][-1]
+
+
+def test_two_level_inverse_cdf_frequencies():
+    """Empirical frequencies of 400k draws from a 10-category law match it (chi-square, 9 dof)."""
+    rng = np.random.default_rng(4)
+    w = np.array([0.0, 3, 1, 0, 7, 2, 2, 5, 0.5, 4.5], dtype=np.float32)
+    u = rng.random(400_000)
+    idx = so.draw_indices_two_level(w, u)
+    f = np.bincount(idx, minlength=10)
+    e = w / w.sum() * u.size
+    keep = e > 0
+    chi2 = (((f - e) ** 2)[keep] / e[keep]).sum()
+    assert f[~keep].sum() == 0 and chi2 < 27.9      # p = 0.001 at 9 dof
