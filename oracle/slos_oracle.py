@@ -390,6 +390,24 @@ def species_mixture(U, n: int, m: int, occ, x: float, mask: str = "fock", order=
     return np.array([total.get(tuple(t), 0.0) for t in final.tolist()])
 
 
+def superposition_probs(U, n: int, m: int, occs, weights, mask: str = "fock") -> np.ndarray:
+    """Coherent superposition of K Fock inputs (photonic_engines/slos/kernel.py:296-431), in the
+    kernel's order: |sum_k w_k per(U[t rows][:, cols_k]) / sqrt(prod s_k! prod t!)|^2, with the
+    permanents of reference.py:24-36."""
+    U = np.asarray(U, dtype=np.complex128)
+    final = to_occupation(enumerate_layers(n, m, mask)[n], m)
+    out = np.zeros(final.shape[0])
+    for idx, t in enumerate(final.tolist()):
+        rows = input_modes(t)
+        t_norm = math.prod(math.factorial(int(c)) for c in t)
+        amp = 0j
+        for occ, w in zip(occs, weights):
+            s_norm = math.prod(math.factorial(int(c)) for c in occ)
+            amp += w * permanent_ryser(U[np.ix_(rows, input_modes(occ))]) / math.sqrt(s_norm * t_norm)
+        out[idx] = abs(amp) ** 2
+    return out
+
+
 def draw_indices_cumsum(p, uniforms) -> np.ndarray:
     """The inverse-CDF branch of draw_indices (photonic_engines/imperfections.py:234-237): a float64
     cumulative sum, u = uniform * total, the first index whose CDF reaches u, clamped to S - 1."""

@@ -10,9 +10,10 @@
 """
 from .basis import FockLayers, RankedLayers, clear_layers_cache, get_layers
 from .haar import haar_unitary
-from .kernel import FockSLOSKernel
+from .kernel import FockSLOSKernel, GroupedSLOSKernel, SuperpositionSLOSKernel
 from .batched import UnevenBatch
 from .sampling import sample_from_probs
 
 __all__ = ["FockLayers", "FockSLOSKernel", "RankedLayers", "clear_layers_cache", "get_layers", "haar_unitary",
-           "UnevenBatch", "sample_from_probs"]
+           "UnevenBatch", "sample_from_probs",
+           "GroupedSLOSKernel", "SuperpositionSLOSKernel"]

@@ -22,6 +22,7 @@ from __future__ import annotations
 import math
 from collections.abc import Sequence
 
+import numpy as np
 import torch
 
 from .. import _native as nat
@@ -207,3 +208,119 @@ class FockSLOSKernel:
         dev = nat.require_gpu(U.device if U.device.type == "cuda" else self.device)
         idx = torch.as_tensor(indices, dtype=torch.int64, device=dev)
         return self._plan.spot_probs(U.to(device=dev, dtype=torch.complex128), idx)
+
+    # ---- grouping ----------------------------------------------------------------------------------
+    def group_matrix(self, groups: torch.Tensor) -> "GroupedSLOSKernel":
+        """Fold a group assignment (LongTensor [S] -> bin id) into the kernel: probabilities are
+        summed into bins after the fused readout (kernel.py:286-294)."""
+        return GroupedSLOSKernel(self, groups)
+
+
+class GroupedSLOSKernel:
+    """Grouped probabilities (kernel.py:584-620): the returned output is [B, n_groups] instead of
+    [B, S]. The full recurrence and the fused |amp|^2 * prod t! readout run as for the wrapped
+    kernel; the [B, S] probabilities are then summed into their groups on the GPU."""
+
+    def __init__(self, kernel: FockSLOSKernel, groups: torch.Tensor):
+        self.kernel = kernel
+        g = torch.as_tensor(groups).to(torch.int64)
+        if g.dim() != 1 or g.numel() != kernel.output_size():
+            raise ValueError(f"groups must be [S] with S = {kernel.output_size()}")
+        if g.numel() and int(g.min()) < 0:
+            raise ValueError("group ids must be >= 0")
+        self.groups = g.to(kernel.device)
+        self.n_groups = int(self.groups.max().item()) + 1 if self.groups.numel() else 0
+
+    def output_size(self) -> int:
+        """Number of groups, the width of the returned distribution."""
+        return self.n_groups
+
+    def probs(self, U: torch.Tensor) -> torch.Tensor:
+        """U [m, m] or [B, m, m] -> [n_groups] or [B, n_groups] float32, normalised like probs(U)."""
+        p = self.kernel.probs(U)
+        batched = p.dim() == 2
+        q = p if batched else p.unsqueeze(0)
+        idx = self.groups.to(q.device)
+        out = torch.zeros((q.shape[0], self.n_groups), dtype=q.dtype, device=q.device)
+        out.scatter_add_(1, idx.unsqueeze(0).expand(q.shape[0], -1), q)
+        return out if batched else out.squeeze(0)
+
+
+class SuperpositionSLOSKernel:
+    """EBS kernel (kernel.py:296-431): K coherent input components of n photons each over one shared
+    output basis. Every component is one forward of the same plan with its own input columns (the
+    plan stores no edges, so nothing is rebuilt per component); the weighted amplitudes are summed
+    on the GPU and squared with the deferred prod t_j!. `chunk_size` is accepted for signature
+    compatibility: one component's layers are live at a time whatever its value."""
+
+    def __init__(self, n: int, m: int, input_occupations, mask: str = Mask.FOCK.value, device: str = "cpu",
+                 dtype: torch.dtype = torch.complex64, weights=None):
+        if dtype not in (torch.complex64,):
+            raise ValueError(f"dtype {dtype}: the MI355X engine computes in complex64")
+        self.n, self.m, self.mask = int(n), int(m), Mask(mask).value
+        self.device = torch.device(device)
+        self.cdtype, self.rdtype = dtype, torch.float32
+        self.layers: RankedLayers = get_layers(n, m, self.mask, device=str(self.device))
+        if self.layers.output_size() == 0:
+            raise ValueError(f"empty output space for n={n} m={m} mask={mask}")
+        occ = [[int(c) for c in row] for row in input_occupations]
+        self.K = len(occ)
+        if self.K == 0:
+            raise ValueError("need at least one input component")
+        if any(len(r) != m or sum(r) != n for r in occ):
+            raise ValueError(f"input_occupations must be [K, {m}] with {n} photons per component")
+        self._modes = [[j for j, c in enumerate(r) for _ in range(c)] for r in occ]
+        self._norms = [1.0 / math.sqrt(math.prod(math.factorial(c) for c in r)) for r in occ]
+        self.default_weights = None if weights is None else torch.as_tensor(
+            np.asarray(weights), dtype=dtype, device=self.device)
+        self._plan = Plan(self.n, self.m, self.mask, self._modes[0])
+        self._output_norm_is_one = self.mask in ("unbunched", "dual_rail", "sparse_1in4") or self.n < 2
+        self._prob_norm = None
+
+    @classmethod
+    def from_state(cls, state, mask: str = Mask.FOCK.value, device: str = "cpu", dtype: torch.dtype = torch.complex64):
+        """Kernel for a Fock superposition state (anything with n, m, occupations, amplitudes and
+        norm()): its normalised amplitudes become the default weights."""
+        return cls(state.n, state.m, state.occupations, mask=mask, device=device, dtype=dtype,
+                   weights=np.asarray(state.amplitudes) / state.norm())
+
+    def output_size(self) -> int:
+        return self.layers.output_size()
+
+    def amplitudes(self, U: torch.Tensor, weights=None, chunk_size: int | None = None) -> torch.Tensor:
+        """Coherently summed output amplitudes [S] / [B, S] (complex64): sum_k w_k * norm_k * raw_k.
+        Default weights: the state's (from_state / weights=), else uniform 1/sqrt(K)."""
+        if U.requires_grad and torch.is_grad_enabled():
+            raise NotImplementedError("SuperpositionSLOSKernel is forward-only")
+        batched = U.dim() == 3
+        Ub = (U if batched else U.unsqueeze(0))
+        if Ub.shape[-2:] != (self.m, self.m):
+            raise ValueError(f"U must be [m, m] or [B, m, m] with m={self.m}")
+        dev = nat.require_gpu(U.device if U.device.type == "cuda" else self.device)
+        Ub = Ub.to(device=dev, dtype=torch.complex64).contiguous()
+        if weights is None and self.default_weights is not None:
+            w = self.default_weights
+        elif weights is None:
+            w = torch.full((self.K,), 1.0 / math.sqrt(self.K), dtype=torch.complex64)
+        else:
+            w = torch.as_tensor(np.asarray(weights), dtype=torch.complex64)
+        w = w.to(dev)
+        if w.numel() != self.K:
+            raise ValueError(f"weights must have K = {self.K} entries")
+        amp = None
+        for k in range(self.K):
+            a = self._plan.forward(Ub, modes=self._modes[k], layer=self.n, out_kind=nat.OUT_AMPS,
+                                   scale=self._norms[k])
+            amp = a * w[k] if amp is None else amp.add_(a * w[k])
+        amp = amp if batched else amp.squeeze(0)
+        return amp.to(self.device) if self.device.type != "cuda" else amp
+
+    def probs(self, U: torch.Tensor, weights=None, chunk_size: int | None = None, loss=None):
+        """Output distribution of the coherent superposition (`loss=eta`: thinned, as probs)."""
+        amp = self.amplitudes(U, weights=weights, chunk_size=chunk_size)
+        p = amp.real ** 2 + amp.imag ** 2
+        if not self._output_norm_is_one:
+            if self._prob_norm is None:
+                self._prob_norm = torch.as_tensor(self.layers.output_prob_norms()).to(dtype=torch.float32)
+            p = p * self._prob_norm.to(p.device)
+        return _maybe_thin(self.layers, p, loss)

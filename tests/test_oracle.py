@@ -344,3 +344,24 @@ def test_two_level_inverse_cdf_frequencies():
     keep = e > 0
     chi2 = (((f - e) ** 2)[keep] / e[keep]).sum()
     assert f[~keep].sum() == 0 and chi2 < 27.9      # p = 0.001 at 9 dof
+
+
+# ---- coherent superpositions ---------------------------------------------------------------------
+
+def test_superposition_oracle_pins():
+    """One component with weight 1 is the single-input distribution; the coherent sum equals the
+    recurrence amplitudes (an independent route) combined by the weights; orthogonal Fock inputs keep
+    the total mass sum |w_k|^2."""
+    n, m = 3, 6
+    U = so.haar_unitary(m, 31)
+    occs = [[1, 1, 1, 0, 0, 0], [2, 0, 0, 1, 0, 0], [0, 0, 1, 1, 1, 0]]
+    w = np.array([0.6, 0.3 - 0.5j, -0.2 + 0.1j])
+    w = w / np.linalg.norm(w)
+    one = so.superposition_probs(U, n, m, occs[:1], [1.0])
+    assert np.abs(one - so.reference_probs(U, n, m, occs[0])).max() < 1e-13
+    sup = so.superposition_probs(U, n, m, occs, w)
+    amps = [so.OracleSLOS(n, m, o).amplitudes(U) for o in occs]
+    norms = so.OracleSLOS(n, m, occs[0]).norms
+    rec = np.abs(sum(wk * a for wk, a in zip(w, amps))) ** 2 * norms
+    assert np.abs(sup - rec).max() < 1e-13
+    assert abs(sup.sum() - 1) < 1e-12
