@@ -173,16 +173,6 @@ SLOS_HD bool next_prefix(const BlockTables& t, int K, int d, int* q) {
   return true;
 }
 
-// XCD-aware workgroup order. The dispatcher hands consecutive workgroups to the 8 XCDs in turn
-// (wg = 8 slot + xcd); this bijection of [0, nwg) gives each XCD one contiguous share of the work
-// (q + 1 workgroups for the first r XCDs, q for the rest, nwg = 8 q + r), so neighbouring child
-// ranges, which read overlapping parent runs, share that XCD's L2.
-SLOS_HD uint32_t xcd_remap(uint32_t wg, uint32_t nwg) {
-  const uint32_t q = nwg >> 3, r = nwg & 7u;
-  const uint32_t xcd = wg & 7u, slot = wg >> 3;
-  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
-}
-
 // lex rank of the prefix Q among all d-tuples of the space (layer d)
 template <int MAXD>
 SLOS_HD uint64_t prefix_rank(const BlockTables& t, int d, const int* q) {

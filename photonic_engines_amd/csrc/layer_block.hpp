@@ -165,15 +165,23 @@ __device__ __forceinline__ float4 load_pair(const float2* base, uint32_t off_byt
   return v;
 }
 
-// 128 consecutive children from uc, all inside the block (uc + 128 <= u_stop): lane i takes the
-// adjacent children uc + 2i and uc + 2i + 1, so each prefix stream is one 16-byte load per lane and
-// the store is one 8- (probabilities) or 16-byte (amplitudes) store per lane.
-template <int KIND, bool COMP, int MAXD>
-__device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t uc) {
+// the tuple entries of lane i's two adjacent children uc + 2i and uc + 2i + 1
+template <int MAXD>
+__device__ __forceinline__ void load_tuple_pair(const ChunkCtx<MAXD>& cx, uint32_t uc, uint4 (&te)[2]) {
   const uint32_t u = uc + 2u * (uint32_t)cx.lane;
-  uint4 te[2];
   te[0] = at_bytes(reinterpret_cast<const uint4*>(cx.tb), u * 16u);
   te[1] = at_bytes(reinterpret_cast<const uint4*>(cx.tb), u * 16u + 16u);
+}
+
+// 128 consecutive children from uc, all inside the block (uc + 128 <= u_stop): lane i takes the
+// adjacent children uc + 2i and uc + 2i + 1, so each prefix stream is one 16-byte load per lane and
+// the store is one 8- (probabilities) or 16-byte (amplitudes) store per lane. `tn` holds this
+// step's tuple entries on entry; if `next`, the next step's (uc + 128, inside the block) are loaded
+// into it behind this step's loads, so the suffix-parent loads of a step never wait for a tuple load.
+template <int KIND, bool COMP, int MAXD>
+__device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t uc, uint4 (&tn)[2], bool next) {
+  const uint32_t u = uc + 2u * (uint32_t)cx.lane;
+  const uint4 te[2] = {tn[0], tn[1]};
   float4 sv[MAXD];
 #pragma unroll
   for (int s = 0; s < MAXD; ++s) sv[s] = load_pair(cx.sp[s], u * 8u);
@@ -184,6 +192,7 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t u
 #pragma unroll
     for (int jj = 0; jj < kSuf; ++jj) lv[c][jj] = at_bytes(cx.lp, lr[jj] * 8u);
   }
+  if (next) load_tuple_pair(cx, uc + 128u, tn);
   float res[2][2];
   float f[2] = {1.f, 1.f};
 #pragma unroll
@@ -367,7 +376,11 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
       uint32_t uc = (uint32_t)u0;
       if constexpr (!COMP) {
         // full groups of 128 children as adjacent pairs per lane, the rest 64 children per step
-        for (; uc + 128u <= u_stop; uc += 128u) sweep_pairs<KIND, COMP, MAXD>(cx, uc);
+        if (uc + 128u <= u_stop) {
+          uint4 tn[2];
+          load_tuple_pair(cx, uc, tn);
+          for (; uc + 128u <= u_stop; uc += 128u) sweep_pairs<KIND, COMP, MAXD>(cx, uc, tn, uc + 256u <= u_stop);
+        }
         if (uc + 64u < u_stop) sweep<KIND, COMP, MAXD, 2>(cx, uc);
         else if (uc < u_stop) sweep<KIND, COMP, MAXD, 1>(cx, uc);
       } else {
@@ -406,9 +419,12 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   const BlockTables tg{cg_t, lo_t, tuples, me, stride, multiset};  // global: scalar loads
   const BlockTables tl{cg_l, lo_l, tuples, me, stride, multiset};  // LDS: per-lane lookups
 
+  // Work groups in launch order: the dispatcher deals them round-robin to the 8 XCDs, so all XCDs
+  // sweep the same region of the layer at once and the far prefix streams one XCD reads are found
+  // by the others in the shared last-level cache (measured faster than giving each XCD a contiguous
+  // eighth of the layer).
   const int wv = (int)rfl(threadIdx.x >> 6);
-  const uint32_t wg_l = xcd_remap(blockIdx.x, gridDim.x);
-  const uint64_t g = uniform_u64(((uint64_t)wg_l * kWaves + wv) * a.per_wave);
+  const uint64_t g = uniform_u64(((uint64_t)blockIdx.x * kWaves + wv) * a.per_wave);
   char* out_b = reinterpret_cast<char*>(out_all) + (size_t)b * a.S * (KIND == 1 ? 4 : 8);
   uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + block_table_offset(K, me)) +
                   (size_t)wv * kBlockBatch * block_words(MAXD);

@@ -214,19 +214,6 @@ int64_t emu_child_edges(int n, int m, int mask, int k, int64_t cap, int64_t* out
   return e;
 }
 
-// 1 if xcd_remap is a bijection of [0, nwg) for every nwg in [1, max_nwg]
-int emu_xcd_remap_ok(uint32_t max_nwg) {
-  std::vector<int> seen;
-  for (uint32_t nwg = 1; nwg <= max_nwg; ++nwg) {
-    seen.assign(nwg, 0);
-    for (uint32_t wg = 0; wg < nwg; ++wg) {
-      const uint32_t v = slos::xcd_remap(wg, nwg);
-      if (v >= nwg || seen[v]++) return 0;
-    }
-  }
-  return 1;
-}
-
 }  // extern "C"
 
 extern "C" {

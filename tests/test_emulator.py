@@ -77,6 +77,3 @@ def test_descent_adjoint_and_child_edges(emu, n, m, mask):
         exp = sorted(zip(par.tolist(), chi.tolist(), mode.tolist(), np.round(fac ** 2).astype(int).tolist()))
         assert got == exp
 
-
-def test_xcd_remap_is_a_bijection(emu):
-    assert emu.emu_xcd_remap_ok(4096) == 1
