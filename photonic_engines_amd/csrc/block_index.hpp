@@ -10,6 +10,8 @@
 //    a fixed Q these parents form an aligned contiguous run of layer K-1: parent = base_l + u;
 //  * dropping the suffix photon x_j gives (Q, tau - x_j) inside the block of Q in layer K-1
 //    (suffix length SUF-1): parent = LBp + lr_j, lr_j = global (SUF-1)-tuple index of tau - x_j.
+//    With cg[r][x] - cg[r-1][lo[x]] = cg[r][x+1] (the counting recurrence), the global index of a
+//    3-tuple (a, b, c) is cg[3][0] - cg[3][a+1] - cg[2][b+1] - cg[1][c]: three per-position terms.
 // Ranks use the counting table of space.hpp: with lo_0 = 0, lo_{i+1} = lo[q_i],
 //   T_i = cg[K-i][lo_i] - cg[K-i][q_i]        (child terms, photon count K)
 //   A_i = cg[K-1-i][lo_i] - cg[K-1-i][q_i]    (same position, one photon fewer)
@@ -27,23 +29,48 @@ namespace slos {
 constexpr int kSuf = 4;
 constexpr int kNoEdge = 255;
 
-// suffix-tuple table entry (16 B) for a SUF-tuple tau, in lex order over all SUF-tuples:
-//   pos   x0 | x1 << 8 | x2 << 16 | x3 << 24   (positions < 255; a repeated copy of a mode,
-//         which is not an edge, carries the position kNoEdge, whose U coefficient is zero)
-//   flags valid (bits 0-3: x_j is the first copy of its mode inside tau) | lead << 4 (3 bits:
-//         copies of x0 in tau) | fct << 7 (5 bits: prod of the run factorials of tau, <= 24)
-//   lr[j] global (SUF-1)-tuple index of tau without x_j (< 65536, checked at plan creation)
-struct TupleEntry {
-  uint32_t pos;
-  uint16_t flags, pad;
-  uint16_t lr[kSuf];
-};
-static_assert(sizeof(TupleEntry) == 16, "tuple entry is 16 bytes");
+// Suffix-tuple word (4 B) of a SUF-tuple tau, in lex order over all SUF-tuples: byte j is
+// x_j | mark_j << 6, where mark_j = 1 when x_j repeats x_{j-1} (a multiset's later copy of a mode,
+// which is not an edge). Positions are < 64 (kBlockModes, checked at plan creation). A byte is
+// also the row of the per-launch suffix table (128 rows): rows x hold U[x, col] and the rank terms
+// of position x, rows x | 64 the same rank terms with a zero coefficient.
+typedef uint32_t TupleWord;
+constexpr int kBlockModes = 64;
+constexpr uint32_t kTupleMark = 64;
+constexpr int kSufRows = 128;
+constexpr int kNoEdgeRow = 127;  // a zero-coefficient row: inactive prefix-stream slots
+
+// the marks of x1, x2, x3 as a 3-bit code (bit j-1: x_j repeats x_{j-1})
+SLOS_HD uint32_t tuple_code(TupleWord w) { return ((w >> 14) & 1u) | ((w >> 21) & 2u) | ((w >> 28) & 4u); }
+// copies of x0 in tau
+SLOS_HD int code_lead(uint32_t c) { return 1 + (int)(c & 1u) + ((c & 3u) == 3u) + (c == 7u); }
+// prod of the run factorials of tau
+SLOS_HD int code_fct(uint32_t c) {
+  int f = 1, run = 1;
+  for (int j = 0; j < kSuf - 1; ++j) {
+    run = ((c >> j) & 1u) ? run + 1 : 1;
+    f *= run;
+  }
+  return f;
+}
+
+// Global (SUF-1)-tuple indices of tau without x_j, j = 0..3, from the per-position terms of tau's
+// real positions: a3[j] = cg[3][x_j + 1], b2[j] = cg[2][x_j + 1], b1[j] = cg[1][x_j], C0 = cg[3][0].
+SLOS_HD void suffix_parent_ranks(uint32_t C0, const uint32_t (&a3)[kSuf], const uint32_t (&b2)[kSuf],
+                                 const uint32_t (&b1)[kSuf], uint32_t (&lr)[kSuf]) {
+  const uint32_t s23 = b2[2] + b1[3];
+  const uint32_t t0 = C0 - a3[0];
+  const uint32_t t01 = t0 - b2[1];
+  lr[0] = C0 - a3[1] - s23;  // (x1, x2, x3)
+  lr[1] = t0 - s23;          // (x0, x2, x3)
+  lr[2] = t01 - b1[3];       // (x0, x1, x3)
+  lr[3] = t01 - b1[2];       // (x0, x1, x2)
+}
 
 struct BlockTables {
   const uint64_t* cg;        // (n + 2) x (me + 1)
   const int* lo;             // me
-  const TupleEntry* tuples;  // cg[SUF][0] entries
+  const TupleWord* tuples;   // cg[SUF][0] entries
   int me, stride, multiset;
 };
 
@@ -207,36 +234,16 @@ SLOS_HD uint64_t unrank_prefix(const BlockTables& t, int K, int d, uint64_t g, i
   return g;
 }
 
-// the table entry of one SUF-tuple (host side, at plan creation)
-inline TupleEntry make_tuple_entry(const SpaceView& sv, int multiset, uint64_t u) {
-  int x[kSuf], r[kSuf - 1];
+// the tuple word of one SUF-tuple (host side, at plan creation)
+inline TupleWord make_tuple_word(const SpaceView& sv, int multiset, uint64_t u) {
+  int x[kSuf];
   sv.unrank(u, kSuf, x);
-  TupleEntry e;
-  e.pos = 0;
-  uint32_t valid = 0;
+  TupleWord w = 0;
   for (int j = 0; j < kSuf; ++j) {
-    const bool first = !multiset || j == 0 || x[j] != x[j - 1];
-    if (first) valid |= 1u << j;
-    e.pos |= (uint32_t)(j == 0 || first ? x[j] : kNoEdge) << (8 * j);
+    const bool mark = multiset && j > 0 && x[j] == x[j - 1];
+    w |= ((uint32_t)x[j] | (mark ? kTupleMark : 0u)) << (8 * j);
   }
-  int lead = 1;
-  if (multiset)
-    while (lead < kSuf && x[lead] == x[0]) ++lead;
-  int fct = 1, run = 1;
-  if (multiset)
-    for (int j = 1; j < kSuf; ++j) {
-      run = (x[j] == x[j - 1]) ? run + 1 : 1;
-      fct *= run;
-    }
-  e.flags = (uint16_t)(valid | ((uint32_t)lead << 4) | ((uint32_t)fct << 7));
-  e.pad = 0;
-  for (int j = 0; j < kSuf; ++j) {
-    int c = 0;
-    for (int i = 0; i < kSuf; ++i)
-      if (i != j) r[c++] = x[i];
-    e.lr[j] = (uint16_t)sv.rank(r, kSuf - 1);
-  }
-  return e;
+  return w;
 }
 
 static_assert(kSuf <= 4, "lead_binom covers runs of up to 4 copies");
@@ -251,12 +258,13 @@ SLOS_HD float lead_binom(int r, int lead) {
   return (float)p;
 }
 
-// prod t_j! of the child (Q, tau) from the block's run data and the tuple entry
-SLOS_HD float child_norm(const BlockInfo& bi, const TupleEntry& e, int multiset) {
+// prod t_j! of the child (Q, tau) from the block's run data and the tuple word
+SLOS_HD float child_norm(const BlockInfo& bi, TupleWord w, int multiset) {
   if (!multiset) return 1.f;
-  float f = bi.fQ * (float)(e.flags >> 7);
+  const uint32_t c = tuple_code(w);
+  float f = bi.fQ * (float)code_fct(c);
   // tau's leading run continues the last run of Q: (r + c)! = r! c! binom(r + c, c)
-  if (bi.skip0 && (int)(e.pos & 0xffu) == bi.lo) f *= lead_binom(bi.lastrun, (int)((e.flags >> 4) & 7u));
+  if (bi.skip0 && (int)(w & 0xffu) == bi.lo) f *= lead_binom(bi.lastrun, code_lead(c));
   return f;
 }
 

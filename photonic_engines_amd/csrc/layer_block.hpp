@@ -4,12 +4,13 @@
 //
 // Work split. A wave owns a contiguous range of children. It walks the blocks it meets in batches
 // of 64 consecutive prefixes: lane i unranks prefix r + i and computes that block's data (start,
-// size, stream bases, coefficients) with the counting tables in LDS, all lanes in parallel. The
-// wave then takes the blocks one at a time, broadcasting the block's data from its lane into
-// scalar registers (readlane), and sweeps the block's children 64 at a time. In that sweep every
-// address is a scalar base plus a 32-bit lane offset, the prefix streams and the child writes are
-// fully coalesced, all loads of a step are in flight together (no branches), and nothing is
-// written twice or atomically.
+// size, stream bases, coefficients) with the counting tables in LDS, all lanes in parallel, and
+// writes it to the wave's block table in LDS. The wave then takes the blocks one at a time, reading
+// the block's data with wave-uniform LDS reads into scalar registers, and sweeps the block's
+// children 128 at a time (two adjacent children per lane). In that sweep every address is a scalar
+// base plus a 32-bit lane offset, the prefix streams and the child writes are fully coalesced, all
+// loads of a step are in flight together (no branches), and nothing is written twice or
+// atomically. A child's suffix edges come from its 4-byte tuple word and the suffix table in LDS.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -78,15 +79,53 @@ __device__ __forceinline__ void bk_cmac(BlockAcc& a, float2 u, float2 v) {
   }
 }
 
+// One row of the per-launch suffix table (LDS, kSufRows rows, row = tuple-word byte): the U
+// coefficient of the position (zero for marked rows, x | 64) and its rank terms cg[3][x+1], cg[2][x+1]
+// and cg[1][x] (the last two packed as 16-bit halves).
+struct SufRow {
+  float re, im;
+  uint32_t a3, b21;
+};
+
+// rows for U[b, :, col] (Ucol = &U[b][0][col], stride m), positions < me; cg: the global table
+__device__ __forceinline__ void fill_suffix_rows(SufRow* rows, const float2* Ucol, int m, const uint64_t* cg, int stride,
+                                                 int me) {
+  for (int i = threadIdx.x; i < kSufRows; i += blockDim.x) {
+    const int x = i & (kBlockModes - 1);
+    const float2 u = (i < kBlockModes && x < m) ? Ucol[(size_t)x * m] : make_float2(0.f, 0.f);
+    SufRow r;
+    r.re = u.x;
+    r.im = u.y;
+    r.a3 = x < me ? (uint32_t)cg[3 * stride + x + 1] : 0u;
+    r.b21 = x < me ? ((uint32_t)cg[2 * stride + x + 1] | ((uint32_t)cg[stride + x] << 16)) : 0u;
+    rows[i] = r;
+  }
+}
+
+// Readout norms by (row, code): row 0 is prod of tau's run factorials for the 3-bit mark code; row
+// 1 + r multiplies it by binom(r + lead, lead) (tau's leading run joins a last prefix run of r
+// copies). kNormRows covers r <= 16, the widest prefix of the block kernel.
+constexpr int kNormRows = 18;
+__device__ __forceinline__ void fill_norm_rows(float* rn) {
+  for (int i = threadIdx.x; i < kNormRows * 8; i += blockDim.x) {
+    const int row = i >> 3;
+    const uint32_t c = (uint32_t)(i & 7);
+    const float f = (float)code_fct(c);
+    rn[i] = row == 0 ? f : f * lead_binom(row - 1, code_lead(c));
+  }
+}
+
 // per-block state of the chunk sweep (all wave-uniform except `lane`)
 template <int MAXD>
 struct ChunkCtx {
   const float2* const (&sp)[MAXD];  // prefix-stream bases (parent of child u at sp[s] + u)
   const float2 (&coef)[MAXD];       // their U coefficients (zero for inactive slots)
   const float2* lp;                 // local parents: lp + lr_j
-  const TupleEntry* tb;             // tuple entries of the block's children
+  const TupleWord* tb;              // tuple words of the block's children
   char* ob;                         // output of the block's first child
-  const float2* ucol;               // U column (LDS; zero at positions >= m, kNoEdge included)
+  const SufRow* rows;               // suffix table (LDS)
+  const float* rn;                  // readout norms (LDS)
+  uint32_t C0;                      // cg[3][0]
   uint32_t u_stop;
   int lane;
   float fQ;
@@ -95,6 +134,34 @@ struct ChunkCtx {
   unsigned flags;
   int kind;  // KIND 3 only: 1 probabilities, 0 raw amplitudes
 };
+
+// the suffix edges of one child from its tuple word: coefficients c[j] (edge 0 of a tau that
+// continues the prefix's last run is no edge: zero) and local-parent byte offsets off[j]
+template <int MAXD>
+__device__ __forceinline__ void suffix_edges(const ChunkCtx<MAXD>& cx, TupleWord w, float2 (&c)[kSuf],
+                                             uint32_t (&off)[kSuf]) {
+  uint32_t a3[kSuf], b2[kSuf], b1[kSuf], lr[kSuf];
+#pragma unroll
+  for (int j = 0; j < kSuf; ++j) {
+    const SufRow r = cx.rows[(w >> (8 * j)) & 0xffu];
+    c[j] = make_float2(r.re, r.im);
+    a3[j] = r.a3;
+    b2[j] = r.b21 & 0xffffu;
+    b1[j] = r.b21 >> 16;
+  }
+  if (cx.skip0 && (int)(w & 0xffu) == cx.lo) c[0] = make_float2(0.f, 0.f);
+  suffix_parent_ranks(cx.C0, a3, b2, b1, lr);
+#pragma unroll
+  for (int j = 0; j < kSuf; ++j) off[j] = lr[j] * 8u;
+}
+
+// prod t_j! of a child from its tuple word and the block's prefix runs
+template <int MAXD>
+__device__ __forceinline__ float child_norm_w(const ChunkCtx<MAXD>& cx, TupleWord w) {
+  if (!cx.multiset) return 1.f;
+  const uint32_t row = (cx.skip0 && (int)(w & 0xffu) == cx.lo) ? (uint32_t)cx.lastrun + 1u : 0u;
+  return cx.fQ * cx.rn[row * 8u + tuple_code(w)];
+}
 
 // NC consecutive 64-child chunks starting at uc: every load of the NC chunks is issued before any
 // result is needed. Lanes past u_stop re-read the last valid child and store nothing.
@@ -108,44 +175,34 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
     in[c] = ur < cx.u_stop;
     u[c] = in[c] ? ur : cx.u_stop - 1u;
   }
-  uint4 te[NC];  // TupleEntry as {pos, flags | pad << 16, lr0 | lr1 << 16, lr2 | lr3 << 16}
+  TupleWord tw[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) te[c] = at_bytes(reinterpret_cast<const uint4*>(cx.tb), u[c] * 16u);
+  for (int c = 0; c < NC; ++c) tw[c] = at_bytes(cx.tb, u[c] * 4u);
   float2 sv[NC][MAXD];
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
     for (int s = 0; s < MAXD; ++s) sv[c][s] = at_bytes(cx.sp[s], u[c] * 8u);
-  float2 lv[NC][kSuf];
+  float2 lc[NC][kSuf], lv[NC][kSuf];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const uint32_t lr[kSuf] = {te[c].z & 0xffffu, te[c].z >> 16, te[c].w & 0xffffu, te[c].w >> 16};
+    uint32_t off[kSuf];
+    suffix_edges(cx, tw[c], lc[c], off);
 #pragma unroll
-    for (int jj = 0; jj < kSuf; ++jj) lv[c][jj] = at_bytes(cx.lp, lr[jj] * 8u);
+    for (int jj = 0; jj < kSuf; ++jj) lv[c][jj] = at_bytes(cx.lp, off[jj]);
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     BlockAcc acc{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int s = 0; s < MAXD; ++s) bk_cmac<COMP>(acc, cx.coef[s], sv[c][s]);
-    // suffix edges; tau's first copy of lo continues the prefix's last run when skip0: no edge
-    const uint32_t x0 = te[c].x & 0xffu;
-    const float2 c0 = cx.ucol[x0];
-    bk_cmac<COMP>(acc, (cx.skip0 && (int)x0 == cx.lo) ? make_float2(0.f, 0.f) : c0, lv[c][0]);
 #pragma unroll
-    for (int jj = 1; jj < kSuf; ++jj) bk_cmac<COMP>(acc, cx.ucol[(te[c].x >> (8 * jj)) & 0xffu], lv[c][jj]);
+    for (int jj = 0; jj < kSuf; ++jj) bk_cmac<COMP>(acc, lc[c][jj], lv[c][jj]);
     float re = acc.re + acc.cre, im = acc.im + acc.cim;
     if (KIND == 0 || (KIND == 3 && cx.kind == 0)) {
       if (in[c]) *reinterpret_cast<float2*>(cx.ob + u[c] * 8u) = make_float2(re, im);
     } else {
-      // prod t_j! of the child: prefix runs, tau's runs, and the join of tau's leading run with
-      // the prefix's last run ((r + c)! = r! c! binom(r + c, c))
-      float f = 1.f;
-      if (cx.multiset) {
-        const uint32_t flags = te[c].y & 0xffffu;
-        f = cx.fQ * (float)(flags >> 7);
-        if (cx.skip0 && (int)(te[c].x & 0xffu) == cx.lo) f *= lead_binom(cx.lastrun, (int)((flags >> 4) & 7u));
-      }
+      const float f = child_norm_w(cx, tw[c]);
       re *= cx.scale;
       im *= cx.scale;
       if constexpr (KIND == 1 || KIND == 3) {
@@ -165,34 +222,35 @@ __device__ __forceinline__ float4 load_pair(const float2* base, uint32_t off_byt
   return v;
 }
 
-// the tuple entries of lane i's two adjacent children uc + 2i and uc + 2i + 1
+// the tuple words of lane i's two adjacent children uc + 2i and uc + 2i + 1 (one 8-byte load)
 template <int MAXD>
-__device__ __forceinline__ void load_tuple_pair(const ChunkCtx<MAXD>& cx, uint32_t uc, uint4 (&te)[2]) {
-  const uint32_t u = uc + 2u * (uint32_t)cx.lane;
-  te[0] = at_bytes(reinterpret_cast<const uint4*>(cx.tb), u * 16u);
-  te[1] = at_bytes(reinterpret_cast<const uint4*>(cx.tb), u * 16u + 16u);
+__device__ __forceinline__ uint2 load_tuple_pair(const ChunkCtx<MAXD>& cx, uint32_t uc) {
+  uint2 w;
+  __builtin_memcpy(&w, reinterpret_cast<const char*>(cx.tb) + (uc + 2u * (uint32_t)cx.lane) * 4u, 8);
+  return w;
 }
 
 // 128 consecutive children from uc, all inside the block (uc + 128 <= u_stop): lane i takes the
 // adjacent children uc + 2i and uc + 2i + 1, so each prefix stream is one 16-byte load per lane and
 // the store is one 8- (probabilities) or 16-byte (amplitudes) store per lane. `tn` holds this
-// step's tuple entries on entry; if `next`, the next step's (uc + 128, inside the block) are loaded
+// step's tuple words on entry; if `next`, the next step's (uc + 128, inside the block) are loaded
 // into it behind this step's loads, so the suffix-parent loads of a step never wait for a tuple load.
 template <int KIND, bool COMP, int MAXD>
-__device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t uc, uint4 (&tn)[2], bool next) {
+__device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t uc, uint2& tn, bool next) {
   const uint32_t u = uc + 2u * (uint32_t)cx.lane;
-  const uint4 te[2] = {tn[0], tn[1]};
+  const TupleWord tw[2] = {tn.x, tn.y};
   float4 sv[MAXD];
 #pragma unroll
   for (int s = 0; s < MAXD; ++s) sv[s] = load_pair(cx.sp[s], u * 8u);
-  float2 lv[2][kSuf];
+  float2 lc[2][kSuf], lv[2][kSuf];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    const uint32_t lr[kSuf] = {te[c].z & 0xffffu, te[c].z >> 16, te[c].w & 0xffffu, te[c].w >> 16};
+    uint32_t off[kSuf];
+    suffix_edges(cx, tw[c], lc[c], off);
 #pragma unroll
-    for (int jj = 0; jj < kSuf; ++jj) lv[c][jj] = at_bytes(cx.lp, lr[jj] * 8u);
+    for (int jj = 0; jj < kSuf; ++jj) lv[c][jj] = at_bytes(cx.lp, off[jj]);
   }
-  if (next) load_tuple_pair(cx, uc + 128u, tn);
+  if (next) tn = load_tuple_pair(cx, uc + 128u);
   float res[2][2];
   float f[2] = {1.f, 1.f};
 #pragma unroll
@@ -201,18 +259,11 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t u
 #pragma unroll
     for (int s = 0; s < MAXD; ++s)
       bk_cmac<COMP>(acc, cx.coef[s], c == 0 ? make_float2(sv[s].x, sv[s].y) : make_float2(sv[s].z, sv[s].w));
-    const uint32_t x0 = te[c].x & 0xffu;
-    const float2 c0 = cx.ucol[x0];
-    bk_cmac<COMP>(acc, (cx.skip0 && (int)x0 == cx.lo) ? make_float2(0.f, 0.f) : c0, lv[c][0]);
 #pragma unroll
-    for (int jj = 1; jj < kSuf; ++jj) bk_cmac<COMP>(acc, cx.ucol[(te[c].x >> (8 * jj)) & 0xffu], lv[c][jj]);
+    for (int jj = 0; jj < kSuf; ++jj) bk_cmac<COMP>(acc, lc[c][jj], lv[c][jj]);
     res[c][0] = acc.re + acc.cre;
     res[c][1] = acc.im + acc.cim;
-    if (!(KIND == 0 || (KIND == 3 && cx.kind == 0)) && cx.multiset) {
-      const uint32_t flags = te[c].y & 0xffffu;
-      f[c] = cx.fQ * (float)(flags >> 7);
-      if (cx.skip0 && (int)x0 == cx.lo) f[c] *= lead_binom(cx.lastrun, (int)((flags >> 4) & 7u));
-    }
+    if (!(KIND == 0 || (KIND == 3 && cx.kind == 0))) f[c] = child_norm_w(cx, tw[c]);
   }
   if (KIND == 0 || (KIND == 3 && cx.kind == 0)) {
     const float4 v = make_float4(res[0][0], res[0][1], res[1][0], res[1][1]);
@@ -232,14 +283,14 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t u
 }
 
 // Dynamic LDS of the block kernels: counting-table rows 0..K and lo[] (per-lane block data is
-// computed from them); the U column is static (256 float2).
+// computed from them); the suffix table and the readout norms are static.
 __host__ __device__ inline size_t block_lds_bytes(int K, int me) { return (size_t)(K + 1) * (me + 1) * 8 + (size_t)me * 4; }
-constexpr size_t kBlockStaticLds = 256 * 8;  // ucol
+constexpr size_t kBlockStaticLds = kSufRows * sizeof(SufRow) + kNormRows * 8 * sizeof(float);  // rows, rn
 // Per-wave block table in LDS: the data of kBlockBatch consecutive blocks, one column per block,
 // computed by all lanes at once (lane i: block r + i) and then read back one block at a time with
 // wave-uniform LDS reads, so none of it occupies vector registers during the child sweeps. Words:
-// CB (2), LBp (2), L, toff, lo | skip0 << 8 | lastrun << 16, fQ, the slot modes (4 per word, kNoEdge
-// for inactive slots), the slot bases (2 each; inactive slots carry slot 0's base).
+// CB (2), LBp (2), L, toff, lo | skip0 << 8 | lastrun << 16, fQ, the slot modes (4 per word,
+// kNoEdgeRow for inactive slots), the slot bases (2 each; inactive slots carry slot 0's base).
 constexpr int kBlockBatch = 64;
 __host__ __device__ constexpr int block_words(int maxd) { return 8 + (maxd + 3) / 4 + 2 * maxd; }
 __host__ __device__ inline size_t block_table_offset(int K, int me) { return (block_lds_bytes(K, me) + 15) & ~(size_t)15; }
@@ -254,15 +305,15 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // One wave's share of a layer slice: the children [g, g + a.per_wave) of one instance, whose parent
-// layer is `par` and whose output layer starts at `out`. `ucol` is the U column in LDS (entries >= m
-// zero), `tl` the counting tables in LDS, `tg` the same tables in global memory (for the
-// wave-uniform scalar unrank of g), `tbl` this wave's block table (kBlockBatch * block_words(MAXD)
-// words of LDS).
+// layer is `par` and whose output layer starts at `out`. `rows` is the suffix table and `rn` the
+// readout norms (LDS), `tl` the counting tables in LDS, `tg` the same tables in global memory (for
+// the wave-uniform scalar unrank of g), `tbl` this wave's block table (kBlockBatch *
+// block_words(MAXD) words of LDS).
 // KIND 3 decides between probabilities (kind 1) and raw amplitudes (kind 0) at run time.
 template <int KIND, bool COMP, int MAXD>
 __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTables& tl, const float2* __restrict__ par,
-                                           void* __restrict__ out, const float2* ucol, uint32_t* tbl,
-                                           const BlockLayer& a, uint64_t g, int kind = 0) {
+                                           void* __restrict__ out, const SufRow* rows, const float* rn,
+                                           uint32_t* tbl, const BlockLayer& a, uint64_t g, int kind = 0) {
   constexpr int kModeW = (MAXD + 3) / 4;
   constexpr int kBaseW = 8 + kModeW;
   const int lane = threadIdx.x & 63;
@@ -270,7 +321,8 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
   const int me = tg.me, stride = tg.stride, multiset = tg.multiset;
   const uint64_t* cg_l = tl.cg;
   const int* lo_l = tl.lo;
-  const TupleEntry* tuples = tg.tuples;
+  const TupleWord* tuples = tg.tuples;
+  const uint32_t C0 = (uint32_t)tg.cg[3 * stride];
   if (g >= a.S) return;
   const uint64_t g_end = (g + a.per_wave < a.S) ? g + a.per_wave : a.S;
   const int d = K - kSuf;
@@ -305,7 +357,7 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
 #pragma unroll
         for (int s = 0; s < MAXD; ++s) {
           st.base[s] = 0;
-          st.mode[s] = kNoEdge;
+          st.mode[s] = kNoEdgeRow;
           st.active[s] = false;
         }
       }
@@ -326,7 +378,7 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
         for (int k = 0; k < 4; ++k) {
           const int s = 4 * w + k;
           const uint32_t md = (s < MAXD && st.active[s < MAXD ? s : 0]) ? (uint32_t)st.mode[s < MAXD ? s : 0]
-                                                                        : (uint32_t)kNoEdge;
+                                                                        : (uint32_t)kNoEdgeRow;
           word |= (md & 0xffu) << (8 * k);
         }
         col[(8 + w) * kBlockBatch] = word;
@@ -356,7 +408,8 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
 #pragma unroll
       for (int s = 0; s < MAXD; ++s) {
         const uint32_t md = (word(8 + s / 4) >> (8 * (s % 4))) & 0xffu;
-        coef[s] = uniform_f2(ucol[md]);
+        const SufRow rw = rows[md];
+        coef[s] = uniform_f2(make_float2(rw.re, rw.im));
         sp[s] = par + ((uint64_t)word(kBaseW + 2 * s) | ((uint64_t)word(kBaseW + 2 * s + 1) << 32));
       }
       float fQ = 1.f;
@@ -366,19 +419,18 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
         lastrun = (int)(w6 >> 16);
       }
       const float2* lp = par + LBp;
-      const TupleEntry* tb = tuples + toff;
+      const TupleWord* tb = tuples + toff;
       const bool probs = KIND == 1 || (KIND == 3 && kind == 1);
       char* ob = reinterpret_cast<char*>(out) + CB * (probs ? 4 : 8);
       const uint64_t avail = g_end - g;
       const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
-      ChunkCtx<MAXD> cx{sp, coef, lp, tb, ob, ucol, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale, a.flags,
-                        kind};
+      ChunkCtx<MAXD> cx{sp, coef, lp, tb, ob, rows, rn, C0, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale,
+                        a.flags, kind};
       uint32_t uc = (uint32_t)u0;
       if constexpr (!COMP) {
         // full groups of 128 children as adjacent pairs per lane, the rest 64 children per step
         if (uc + 128u <= u_stop) {
-          uint4 tn[2];
-          load_tuple_pair(cx, uc, tn);
+          uint2 tn = load_tuple_pair(cx, uc);
           for (; uc + 128u <= u_stop; uc += 128u) sweep_pairs<KIND, COMP, MAXD>(cx, uc, tn, uc + 256u <= u_stop);
         }
         if (uc + 64u < u_stop) sweep<KIND, COMP, MAXD, 2>(cx, uc);
@@ -400,12 +452,13 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
 template <int KIND, bool COMP, int MAXD>
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(COMP || MAXD > 8 ? 3 : 5))) void k_layer_block(const uint64_t* __restrict__ cg_t,
                                                                const int* __restrict__ lo_t,
-                                                               const TupleEntry* __restrict__ tuples,
+                                                               const TupleWord* __restrict__ tuples,
                                                                int me, int stride, int multiset,
                                                                const float2* __restrict__ par_all,
                                                                void* __restrict__ out_all, BlockLayer a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ float2 ucol[256];  // U[b, :, col]; entries >= m are zero (kNoEdge = 255 included)
+  __shared__ SufRow rows[kSufRows];
+  __shared__ float rn[kNormRows * 8];
   const int K = a.K;
   const int ncg = (K + 1) * stride;
   uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
@@ -413,8 +466,8 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   const int64_t b = blockIdx.y;
   for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = cg_t[i];
   for (int i = threadIdx.x; i < me; i += blockDim.x) lo_l[i] = lo_t[i];
-  for (int j = threadIdx.x; j < 256; j += blockDim.x)
-    ucol[j] = j < a.m ? a.U[(size_t)b * a.m * a.m + (size_t)j * a.m + a.col] : make_float2(0.f, 0.f);
+  fill_suffix_rows(rows, a.U + (size_t)b * a.m * a.m + a.col, a.m, cg_t, stride, me);
+  fill_norm_rows(rn);
   __syncthreads();
   const BlockTables tg{cg_t, lo_t, tuples, me, stride, multiset};  // global: scalar loads
   const BlockTables tl{cg_l, lo_l, tuples, me, stride, multiset};  // LDS: per-lane lookups
@@ -428,7 +481,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   char* out_b = reinterpret_cast<char*>(out_all) + (size_t)b * a.S * (KIND == 1 ? 4 : 8);
   uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + block_table_offset(K, me)) +
                   (size_t)wv * kBlockBatch * block_words(MAXD);
-  block_wave<KIND, COMP, MAXD>(tg, tl, par_all + (size_t)b * a.S_par, out_b, ucol, tbl, a, g);
+  block_wave<KIND, COMP, MAXD>(tg, tl, par_all + (size_t)b * a.S_par, out_b, rows, rn, tbl, a, g);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -444,7 +497,7 @@ constexpr int kMaxGroup = 8;
 struct GroupItem {
   const uint64_t* cg;          // plan tables (device)
   const int* lo;
-  const TupleEntry* tuples;
+  const TupleWord* tuples;
   const float2* U;             // [B][m][m]
   const float2* par;           // [B][S_par]
   void* out;                   // [B][S]: float32 probabilities (kind 1) or complex64 (kind 0)
@@ -489,8 +542,10 @@ template <bool COMP, int MAXD>
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(COMP || MAXD > 8 ? 3 : MAXD <= 5 ? 5 : 4))) void
 k_layer_block_group(GroupLaunch gl) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ float2 ucol[256];
+  __shared__ SufRow rows[kSufRows];
+  __shared__ float rn[kNormRows * 8];
   const int K = gl.K;
+  fill_norm_rows(rn);
   const int wv = (int)rfl(threadIdx.x >> 6);
   const uint64_t* cur = nullptr;  // plan whose tables are in LDS
   for_each_item(gl.total, [&](uint32_t item) {
@@ -507,8 +562,7 @@ k_layer_block_group(GroupLaunch gl) {
       for (int i = threadIdx.x; i < it.me; i += blockDim.x) lo_l[i] = it.lo[i];
       cur = it.cg;
     }
-    for (int j = threadIdx.x; j < 256; j += blockDim.x)
-      ucol[j] = j < it.m ? it.U[(size_t)b * it.m * it.m + (size_t)j * it.m + it.col] : make_float2(0.f, 0.f);
+    fill_suffix_rows(rows, it.U + (size_t)b * it.m * it.m + it.col, it.m, it.cg, stride, it.me);
     __syncthreads();
     const BlockTables tg{it.cg, it.lo, it.tuples, it.me, stride, it.multiset};
     const BlockTables tl{cg_l, lo_l, it.tuples, it.me, stride, it.multiset};
@@ -519,7 +573,7 @@ k_layer_block_group(GroupLaunch gl) {
     const float2* par = it.par + (size_t)b * it.S_par;
     char* out = reinterpret_cast<char*>(it.out) + (size_t)b * it.S * (it.kind == 1 ? 4 : 8);
     uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + gl.table_off) + (size_t)wv * kBlockBatch * block_words(MAXD);
-    block_wave<3, COMP, MAXD>(tg, tl, par, out, ucol, tbl, a, g, it.kind);
+    block_wave<3, COMP, MAXD>(tg, tl, par, out, rows, rn, tbl, a, g, it.kind);
   });
 }
 

@@ -46,7 +46,7 @@ struct DevTables {
   uint64_t* cg = nullptr;
   int* lo = nullptr;
   int* modes = nullptr;                    // the plan's input modes (spot permanents)
-  slos::TupleEntry* tuples = nullptr;      // block-kernel suffix table (may be absent)
+  slos::TupleWord* tuples = nullptr;       // block-kernel suffix-tuple words (may be absent)
 };
 
 struct slos_plan {
@@ -55,7 +55,7 @@ struct slos_plan {
   std::vector<uint64_t> cg;  // (n + 2) x (me + 1)
   std::vector<int> lo;
   double s_fact = 1.0;       // prod_i s_i!
-  std::vector<slos::TupleEntry> tuples;  // suffix-tuple table of the block kernel (empty: unused)
+  std::vector<slos::TupleWord> tuples;   // suffix-tuple words of the block kernel (empty: unused)
   std::vector<DevTables> dev;
   std::mutex mu;
   slos::SpaceView host_view() const { return slos::SpaceView{cg.data(), lo.data(), me, me + 1}; }
@@ -83,8 +83,8 @@ int device_tables(const slos_plan* cp, DevTables* out) {
   HIP_TRY(hipMalloc(&t.modes, p->modes.size() * sizeof(int)));
   HIP_TRY(hipMemcpy(t.modes, p->modes.data(), p->modes.size() * sizeof(int), hipMemcpyHostToDevice));
   if (!p->tuples.empty()) {
-    HIP_TRY(hipMalloc(&t.tuples, p->tuples.size() * sizeof(slos::TupleEntry)));
-    HIP_TRY(hipMemcpy(t.tuples, p->tuples.data(), p->tuples.size() * sizeof(slos::TupleEntry),
+    HIP_TRY(hipMalloc(&t.tuples, p->tuples.size() * sizeof(slos::TupleWord)));
+    HIP_TRY(hipMemcpy(t.tuples, p->tuples.data(), p->tuples.size() * sizeof(slos::TupleWord),
                       hipMemcpyHostToDevice));
   }
   p->dev.push_back(t);
@@ -943,7 +943,7 @@ void build_tuples(slos_plan* p) {
   const auto sv = p->host_view();
   const uint64_t N = p->size(slos::kSuf);
   p->tuples.resize(N);
-  for (uint64_t u = 0; u < N; ++u) p->tuples[u] = slos::make_tuple_entry(sv, p->multiset, u);
+  for (uint64_t u = 0; u < N; ++u) p->tuples[u] = slos::make_tuple_word(sv, p->multiset, u);
 }
 
 extern "C" {
@@ -994,9 +994,8 @@ int slos_plan_create(int n, int m, int mask, const int32_t* input_modes, slos_pl
       }
       p->cg[(size_t)r * st + x] = v;
     }
-  // the block kernel needs modes < 255 (8-bit fields, 255 = no edge) and (SUF-1)-tuple indices < 65536
-  if (n > slos::kSuf && p->m <= 255 && p->size(slos::kSuf - 1) < 65536 && p->size(slos::kSuf) <= (4u << 20))
-    build_tuples(p);
+  // the block kernel's tuple words hold positions < 64 (6 bits and a mark bit per byte)
+  if (n > slos::kSuf && p->m <= slos::kBlockModes && p->size(slos::kSuf) <= (4u << 20)) build_tuples(p);
   if (p->size(n) == 0) {
     delete p;
     return fail(SLOS_EINVAL, "empty output space for n=" + std::to_string(n) + " m=" + std::to_string(m));

@@ -17,7 +17,7 @@ struct Space {
   int me = 0, multiset = 1;
   std::vector<uint64_t> cg;
   std::vector<int> lo;
-  std::vector<slos::TupleEntry> tuples;
+  std::vector<slos::TupleWord> tuples;
 };
 
 bool build(int n, int m, int mask, Space& s) {
@@ -44,7 +44,7 @@ bool build(int n, int m, int mask, Space& s) {
   if (n > slos::kSuf) {
     const uint64_t N = s.cg[(size_t)slos::kSuf * W];
     s.tuples.resize(N);
-    for (uint64_t u = 0; u < N; ++u) s.tuples[u] = slos::make_tuple_entry(sv, s.multiset, u);
+    for (uint64_t u = 0; u < N; ++u) s.tuples[u] = slos::make_tuple_word(sv, s.multiset, u);
   }
   return true;
 }
@@ -72,8 +72,21 @@ void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const st
   const int d = K - slos::kSuf;
   const uint64_t S_pre = s.cg[(size_t)d * W];
   slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, W};
-  std::vector<cd> ucol(256, cd(0, 0));
-  for (size_t j = 0; j < ucol_in.size() && j < 255; ++j) ucol[j] = ucol_in[j];
+  // the device's suffix table: row x | mark << 6 = (U coefficient or zero, cg[3][x+1], cg[2][x+1], cg[1][x])
+  struct Row { cd c; uint32_t a3, b2, b1; };
+  std::vector<Row> rows(slos::kSufRows);
+  for (int i = 0; i < slos::kSufRows; ++i) {
+    const int x = i & (slos::kBlockModes - 1);
+    Row r{cd(0, 0), 0, 0, 0};
+    if (i < slos::kBlockModes && x < (int)ucol_in.size()) r.c = ucol_in[x];
+    if (x < s.me) {
+      r.a3 = (uint32_t)s.cg[(size_t)3 * W + x + 1];
+      r.b2 = (uint32_t)s.cg[(size_t)2 * W + x + 1];
+      r.b1 = (uint32_t)s.cg[(size_t)W + x];
+    }
+    rows[i] = r;
+  }
+  const uint32_t C0 = (uint32_t)s.cg[(size_t)3 * W];
   out.assign(S, 0);
   writes.assign(S, 0);
   if (norms) norms->assign(S, 0.0);
@@ -102,19 +115,25 @@ void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const st
             if (u >= u_stop) continue;
             cd acc = 0;
             for (int si = 0; si < MAXD; ++si)
-              if (st.active[si]) acc += ucol[st.mode[si]] * par.at(st.base[si] + u);
-            const slos::TupleEntry e = t.tuples[bi.toff + u];
+              if (st.active[si]) acc += rows.at(st.mode[si]).c * par.at(st.base[si] + u);
+            const slos::TupleWord w = t.tuples[bi.toff + u];
+            uint32_t a3[slos::kSuf], b2[slos::kSuf], b1[slos::kSuf], lr[slos::kSuf];
+            cd coef[slos::kSuf];
             for (int jj = 0; jj < slos::kSuf; ++jj) {
-              const uint32_t x = (e.pos >> (8 * jj)) & 0xffu;
-              // tau's first copy of lo continuing the prefix's last run is no edge (edge 0 only)
-              const cd coef = (jj == 0 && bi.skip0 && (int)x == bi.lo) ? cd(0, 0) : ucol.at(x);
-              const cd v = par.at(bi.LBp + e.lr[jj]);
-              acc += coef * v;
+              const Row& rw = rows.at((w >> (8 * jj)) & 0xffu);
+              coef[jj] = rw.c;
+              a3[jj] = rw.a3;
+              b2[jj] = rw.b2;
+              b1[jj] = rw.b1;
             }
+            // tau's first copy of lo continuing the prefix's last run is no edge (edge 0 only)
+            if (bi.skip0 && (int)(w & 0xffu) == bi.lo) coef[0] = cd(0, 0);
+            slos::suffix_parent_ranks(C0, a3, b2, b1, lr);
+            for (int jj = 0; jj < slos::kSuf; ++jj) acc += coef[jj] * par.at(bi.LBp + lr[jj]);
             const uint64_t c = bi.CB + u;
             out.at(c) = acc;
             writes.at(c)++;
-            if (norms) (*norms)[c] = slos::child_norm(bi, e, t.multiset);
+            if (norms) (*norms)[c] = slos::child_norm(bi, w, t.multiset);
           }
         g += u_stop - u0;
         u0 = 0;
@@ -220,6 +239,53 @@ extern "C" {
 // every child rank that slos::for_each_child reports for parents [first, first + count) of layer
 // k-1 equals the rank of the child's own sorted positions and is < S_k; returns the number of
 // mismatches (0: all correct), -1 on bad input
+// Every suffix tuple of the space: the tuple word's positions and marks, the four local-parent ranks
+// from the per-position terms (slos::suffix_parent_ranks), and the run data from the mark code
+// against direct computation (unrank, rank of tau minus one position, runs). Returns the number of
+// mismatches, or -1 for bad input.
+int64_t emu_check_tuple_words(int m, int mask) {
+  Space s;
+  if (!build(slos::kSuf + 1, m, mask, s)) return -1;
+  const int W = s.me + 1;
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, W};
+  const uint32_t C0 = (uint32_t)s.cg[(size_t)3 * W];
+  int64_t bad = 0;
+  for (uint64_t u = 0; u < s.tuples.size(); ++u) {
+    int x[slos::kSuf];
+    sv.unrank(u, slos::kSuf, x);
+    const slos::TupleWord w = s.tuples[u];
+    uint32_t a3[slos::kSuf], b2[slos::kSuf], b1[slos::kSuf], lr[slos::kSuf];
+    for (int j = 0; j < slos::kSuf; ++j) {
+      const uint32_t byte = (w >> (8 * j)) & 0xffu;
+      const int pos = (int)(byte & (slos::kBlockModes - 1));
+      const bool mark = (byte & slos::kTupleMark) != 0;
+      if (pos != x[j] || mark != (s.multiset && j > 0 && x[j] == x[j - 1]) || byte >= (uint32_t)slos::kSufRows) ++bad;
+      a3[j] = (uint32_t)s.cg[(size_t)3 * W + pos + 1];
+      b2[j] = (uint32_t)s.cg[(size_t)2 * W + pos + 1];
+      b1[j] = (uint32_t)s.cg[(size_t)W + pos];
+    }
+    slos::suffix_parent_ranks(C0, a3, b2, b1, lr);
+    for (int j = 0; j < slos::kSuf; ++j) {
+      int r[slos::kSuf - 1], c = 0;
+      for (int i = 0; i < slos::kSuf; ++i)
+        if (i != j) r[c++] = x[i];
+      if ((uint64_t)lr[j] != sv.rank(r, slos::kSuf - 1)) ++bad;
+    }
+    int lead = 1, fct = 1, run = 1;
+    if (s.multiset) {
+      while (lead < slos::kSuf && x[lead] == x[0]) ++lead;
+      for (int j = 1; j < slos::kSuf; ++j) {
+        run = (x[j] == x[j - 1]) ? run + 1 : 1;
+        fct *= run;
+      }
+    }
+    const uint32_t code = slos::tuple_code(w);
+    if (s.multiset && (slos::code_lead(code) != lead || slos::code_fct(code) != fct)) ++bad;
+    if (!s.multiset && code != 0) ++bad;
+  }
+  return bad;
+}
+
 int64_t emu_check_children(int n, int m, int mask, int k, uint64_t first, uint64_t count) {
   Space s;
   if (!build(n, m, mask, s) || k < 1 || k > n) return -1;

@@ -23,6 +23,7 @@ def emu(tmp_path_factory):
                     os.path.join(ROOT, "tests", "native", "block_emu.cpp")], check=True)
     L = ctypes.CDLL(str(out))
     L.emu_child_edges.restype = ctypes.c_int64
+    L.emu_check_tuple_words.restype = ctypes.c_int64
     return L
 
 
@@ -35,7 +36,7 @@ FWD = [(4, 6, "fock", None), (5, 7, "fock", None), (6, 6, "fock", [3, 0, 2, 0, 1
        (7, 9, "fock", [1, 0, 2, 1, 0, 3, 0, 0, 0]), (8, 10, "fock", None), (6, 8, "fock", [6, 0, 0, 0, 0, 0, 0, 0]),
        (7, 12, "fock", None), (5, 20, "sparse_1in4", None), (9, 5, "fock", [2, 2, 2, 2, 1]), (10, 5, "fock", [3, 0, 4, 1, 2]),
        (11, 4, "fock", [5, 0, 3, 3]), (12, 4, "fock", [3, 3, 3, 3]), (13, 3, "fock", [5, 4, 4]), (9, 12, "unbunched", None),
-       (10, 20, "dual_rail", None)]
+       (10, 20, "dual_rail", None), (5, 40, "unbunched", None)]
 
 
 @pytest.mark.parametrize("n,m,mask,occ", FWD)
@@ -77,3 +78,12 @@ def test_descent_adjoint_and_child_edges(emu, n, m, mask):
         exp = sorted(zip(par.tolist(), chi.tolist(), mode.tolist(), np.round(fac ** 2).astype(int).tolist()))
         assert got == exp
 
+
+@pytest.mark.parametrize("m,mask", [(4, "fock"), (9, "fock"), (32, "fock"), (64, "fock"), (10, "tail_empty"),
+                                    (64, "tail_empty"), (33, "unbunched"), (64, "unbunched"), (20, "dual_rail"),
+                                    (24, "sparse_1in4")])
+def test_tuple_words_and_local_parent_ranks(emu, m, mask):
+    """Every suffix tuple: the 4-byte word's positions and marks, the four local-parent ranks from the
+    per-position terms cg[3][x+1], cg[2][x+1], cg[1][x], and the run data from the mark code equal the
+    direct unrank / rank / run computation."""
+    assert emu.emu_check_tuple_words(m, pull.MASK_IDS[mask]) == 0
