@@ -138,11 +138,14 @@ int slos_adjoint_layer(const slos_plan* plan, const void* U, int64_t batch, int 
                        size_t workspace_bytes, slos_stream_t stream);
 
 /* One step of the uniform photon-removal recurrence of output loss (imperfections.py:254-276):
- * q_{k-1}(t) = sum_j q_k(t + e_j) (t_j + 1) / k, for 1 <= k <= n. Pull form (each (k-1)-photon
- * state gathers its children), no atomics. q_k: device float32 [batch][S_k];
- * q_km1: device float32 [batch][S_{k-1}]. The binomial weights (imperfections.py:106-111) are
- * applied by the caller. */
-int slos_loss_descend(const slos_plan* plan, int k, const float* q_k, float* q_km1, int64_t batch,
+ * q_{k-1}(t) = scale * (sum_j q_k(t + e_j) (t_j + 1)) / k, for 1 <= k <= n. Pull form (each
+ * (k-1)-photon state gathers its children), no atomics; children in ascending mode order. q_k:
+ * device float32 [batch][S_k]; q_km1: device float32 [batch][S_{k-1}]. `scale` lets a caller carry
+ * the binomial weights (imperfections.py:106-111) down the layers (weight ratio of the two layers);
+ * 1 gives the plain recurrence. flags: SLOS_FLAG_SIMPLE forces the thread-per-state kernel
+ * (otherwise multiset spaces with k - 1 > 4 use the block kernel; both give bit-identical results). */
+int slos_loss_descend(const slos_plan* plan, int k, const float* q_k, float* q_km1, int64_t batch, unsigned flags,
+                      double scale,
                       slos_stream_t stream);
 
 /* Probability-level raise of partial distinguishability (species.classical_raise,

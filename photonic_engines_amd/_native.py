@@ -41,7 +41,7 @@ SIGNATURES = {
     "slos_adjoint_workspace": (_c.c_int, [_vp, _c.c_int64, _c.c_int, _c.POINTER(_c.c_size_t)]),
     "slos_adjoint_layer": (_c.c_int, [_vp, _vp, _c.c_int64, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int,
                                       _vp, _c.c_size_t, _vp]),
-    "slos_loss_descend": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int64, _vp]),
+    "slos_loss_descend": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int64, _c.c_uint, _c.c_double, _vp]),
     "slos_loss_ascend": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int64, _vp]),
     "slos_raise": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _c.c_int64, _vp]),
     "slos_mass_workspace": (_c.c_int, [_c.c_int64, _c.c_uint64, _c.POINTER(_c.c_size_t)]),

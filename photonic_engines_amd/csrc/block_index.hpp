@@ -246,6 +246,126 @@ inline TupleWord make_tuple_word(const SpaceView& sv, int multiset, uint64_t u) 
   return w;
 }
 
+// ---- loss descent on the block structure (pull over children, multiset spaces: lo[x] = x) -----
+// A parent of layer D = K - 1 is (P, s): P its first dp = D - SUF positions, s = (s0..s3) its last
+// four, u its offset in the block of P. Inserting mode a gives the child (K photons):
+//  * a in segment i < dp, i.e. a in [lo_i, p_i) (lo_0 = 0, lo_i = p_{i-1}): the child lies in block
+//    P + a of layer K at the same offset u, child = B_i - cg[K-i][a+1] + u, with
+//    B_i = sum_{l<i} (cg[K-l][lo_l] - cg[K-l][p_l]) + cg[K-i][lo_i] - cg[D-i][p_i]
+//          + sum_{i<l<dp} (cg[D-l][lo_l] - cg[D-l][p_l]);
+//  * last(P) <= a <= s0: child = CB0 + cg[5][last(P)+1] - cg[5][a+1] + u (block (P, a)),
+//    CB0 = sum_{l<dp} (cg[K-l][lo_l] - cg[K-l][p_l]) the first child of block (P, last(P));
+//  * a > s0: child = CB0 + cg[5][last(P)] - cg[5][s0+1] - G(sorted(s1, s2, s3, a)) (block (P, s0)),
+//    G(r) = cg[4][r0+1] + cg[3][r1+1] + cg[2][r2+1] + cg[1][r3].
+// (the blocks (P, x), x >= last(P), are consecutive in layer K, cg[5][x] - cg[4][x] = cg[5][x+1]).
+// The copies of a in the child, the weight's numerator, are 1 + its copies in P and in s.
+template <int MAXD>
+struct DescBlock {
+  uint64_t CBp;      // first parent of the block in layer D
+  uint64_t CB0;      // first child of block (P, last(P)) in layer K
+  uint64_t B[MAXD];  // segment bases
+  uint32_t L;        // block size
+  uint32_t toff;     // tuple-word index of the block's first suffix
+  int lastP, lastrun;
+  int p[MAXD];       // the prefix positions
+  int run[MAXD];     // run[i], i >= 1: copies of p_{i-1} in P
+};
+
+template <int MAXD>
+SLOS_HD void descend_block_info(const BlockTables& t, int K, int dp, const int* q, DescBlock<MAXD>& db) {
+  const uint64_t* cg = t.cg;
+  const int W = t.stride;
+  const int D = K - 1;
+#define SLOS_C(r, x) cg[(r) * W + (x)]
+  uint64_t TD[MAXD];  // parent terms cg[D-l][lo_l] - cg[D-l][p_l]
+  uint64_t CBp = 0, CB0 = 0;
+  int lo = 0, run = 0, prev = -1;
+#pragma unroll
+  for (int l = 0; l < MAXD; ++l) {
+    TD[l] = 0;
+    db.p[l] = 0;
+    db.run[l] = 0;
+    db.B[l] = 0;
+    if (l < dp) {
+      db.p[l] = q[l];
+      db.run[l] = run;                    // copies of p_{l-1} in P so far (segment l's first mode)
+      TD[l] = SLOS_C(D - l, lo) - SLOS_C(D - l, q[l]);
+      CBp += TD[l];
+      CB0 += SLOS_C(K - l, lo) - SLOS_C(K - l, q[l]);
+      run = (q[l] == prev) ? run + 1 : 1;
+      prev = q[l];
+      lo = q[l];
+    }
+  }
+  // segment bases, from the back: SP = sum_{i<l<dp} TD[l], PB = sum_{l<i} child terms
+  uint64_t SP = 0;
+#pragma unroll
+  for (int i = MAXD - 1; i >= 0; --i)
+    if (i < dp) {
+      uint64_t PB = 0;
+      int lo_l = 0;
+#pragma unroll
+      for (int l = 0; l < MAXD; ++l)
+        if (l < i) {
+          PB += SLOS_C(K - l, lo_l) - SLOS_C(K - l, q[l]);
+          lo_l = q[l];
+        }
+      db.B[i] = PB + SLOS_C(K - i, lo_l) - SLOS_C(D - i, q[i]) + SP;
+      SP += TD[i];
+    }
+  db.CBp = CBp;
+  db.CB0 = CB0;
+  db.lastP = q[dp - 1];
+  db.lastrun = run;
+  db.L = (uint32_t)SLOS_C(kSuf, db.lastP);
+  db.toff = (uint32_t)(SLOS_C(kSuf, 0) - SLOS_C(kSuf, db.lastP));
+#undef SLOS_C
+}
+
+// Per-parent terms of the children with a > s0: child - CB0 = Zl - Wp[p] - A_p(a), p = number of
+// s1..s3 below a, A_0..A_3(a) = cg[4][a+1], cg[3][a+1], cg[2][a+1], cg[1][a].
+struct DescLane {
+  uint32_t Zl;
+  uint32_t Wp[4];
+};
+
+SLOS_HD void desc_lane_init(const uint64_t* cg, int W, int lastP, const int (&s)[kSuf], DescLane& dl) {
+#define SLOS_C(r, x) (uint32_t)cg[(r) * W + (x)]
+  dl.Zl = SLOS_C(5, lastP) - SLOS_C(5, s[0] + 1);
+  const uint32_t c4s1 = SLOS_C(4, s[1] + 1), c3s1 = SLOS_C(3, s[1] + 1), c3s2 = SLOS_C(3, s[2] + 1);
+  const uint32_t c2s2 = SLOS_C(2, s[2] + 1), c2s3 = SLOS_C(2, s[3] + 1), c1s3 = SLOS_C(1, s[3]);
+  dl.Wp[0] = c3s1 + c2s2 + c1s3;   // (a, s1, s2, s3)
+  dl.Wp[1] = c4s1 + c2s2 + c1s3;   // (s1, a, s2, s3)
+  dl.Wp[2] = c4s1 + c3s2 + c1s3;   // (s1, s2, a, s3)
+  dl.Wp[3] = c4s1 + c3s2 + c2s3;   // (s1, s2, s3, a)
+#undef SLOS_C
+}
+
+// offset from CB0 of the child with mode a > s0 (block (P, s0)); A = (cg[4][a+1], cg[3][a+1],
+// cg[2][a+1], cg[1][a]) are the same for every parent
+SLOS_HD uint32_t desc_local_offset(const DescLane& dl, const int (&s)[kSuf], int a, uint32_t A0, uint32_t A1, uint32_t A2,
+                                   uint32_t A3) {
+  // values, not array elements, in the selects (no select of addresses, nothing kept in memory)
+  const int p = (s[1] < a) + (s[2] < a) + (s[3] < a);
+  const uint32_t w0 = dl.Wp[0], w1 = dl.Wp[1], w2 = dl.Wp[2], w3 = dl.Wp[3];
+  const uint32_t wsel = p == 0 ? w0 : p == 1 ? w1 : p == 2 ? w2 : w3;
+  const uint32_t asel = p == 0 ? A0 : p == 1 ? A1 : p == 2 ? A2 : A3;
+  return dl.Zl - wsel - asel;
+}
+
+// offset from CB0 of the child with mode a >= last(P): the stream of block (P, a) at u while
+// a <= s0 (S5a = cg[5][last(P)+1] - cg[5][a+1], the same for every parent), else block (P, s0)
+SLOS_HD uint32_t desc_tail_offset(const DescLane& dl, const int (&s)[kSuf], uint32_t u, int a, uint32_t S5a,
+                                  const uint32_t (&A)[4]) {
+  const uint32_t a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3];
+  return a <= s[0] ? S5a + u : desc_local_offset(dl, s, a, a0, a1, a2, a3);
+}
+
+// copies of mode a >= last(P) in the child of parent (P, s)
+SLOS_HD int desc_tail_copies(const int (&s)[kSuf], int a, int lastP, int lastrun) {
+  return 1 + (a == lastP ? lastrun : 0) + (s[0] == a) + (s[1] == a) + (s[2] == a) + (s[3] == a);
+}
+
 static_assert(kSuf <= 4, "lead_binom covers runs of up to 4 copies");
 
 // binom(r + lead, lead) for lead <= kSuf, exact in integers (each step divides exactly)

@@ -1,0 +1,216 @@
+// Block-structured pull over children for one step of the uniform-loss recurrence
+// (photonic_engines/imperfections.py:254-276): q_{K-1}(t) = sum_a q_K(t + e_a) (t_a + 1) / K, for
+// multiset spaces (lo[x] = x). The index arithmetic (segment bases, tail offsets) is in
+// block_index.hpp.
+//
+// Work split, as in the forward block kernel: a wave owns a contiguous range of parents of layer
+// K-1. It computes the data of 64 consecutive parent blocks at once (lane i: block r + i, with the
+// counting tables in LDS) into its block table in LDS, then takes the blocks one at a time with
+// wave-uniform LDS reads and sweeps their parents 64 at a time, one per lane. For each mode a in
+// ascending order a lane adds q_K(child) * copies: modes below the block's last prefix position
+// reach a child at a block-uniform base plus the lane's offset (a scalar base, coalesced loads);
+// the others reach it at a per-lane offset from the first child of block (P, last(P)). The sums
+// run in the same order, with the same fused multiply-adds, as the thread-per-state kernel, so the
+// two give bit-identical results. No atomics; each parent is written by one lane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "layer_block.hpp"
+
+namespace slos {
+
+// per-wave descent block table: CBp (2), CB0 (2), L, toff, lastP | lastrun << 8, the prefix
+// positions (4 per word), the runs (4 per word), the segment bases (2 each)
+__host__ __device__ constexpr int desc_words(int maxd) { return 7 + 2 * ((maxd + 3) / 4) + 2 * maxd; }
+__host__ __device__ inline size_t desc_table_offset(int K, int me) { return (block_lds_bytes(K, me) + 15) & ~(size_t)15; }
+__host__ __device__ inline size_t desc_dyn_lds(int K, int me, int maxd) {
+  return desc_table_offset(K, me) + (size_t)kWaves * kBlockBatch * desc_words(maxd) * 4;
+}
+
+// ME: a compile-time bound on the mode count (me <= ME <= 64), so that the mode loop unrolls and
+// every child load of a step is issued before the first sum.
+template <int MAXD, int ME>
+__device__ __forceinline__ void descend_wave(const BlockTables& tg, const BlockTables& tl, int K, const float* __restrict__ qk,
+                                             float* __restrict__ qout, uint64_t Sp, uint64_t S_pre, uint64_t per_wave,
+                                             uint32_t* tbl, uint64_t g, float scale) {
+  constexpr int kPW = (MAXD + 3) / 4;
+  constexpr int kBW = 7 + 2 * kPW;
+  const int lane = threadIdx.x & 63;
+  const int D = K - 1, dp = D - kSuf;
+  const int W = tg.stride, me = tg.me;
+  const uint64_t* cgl = tl.cg;
+  if (g >= Sp) return;
+  const uint64_t g_end = (g + per_wave < Sp) ? g + per_wave : Sp;
+  const float invk = 1.f / (float)K;
+  // lane a holds the per-mode rank terms of mode a: cg[4][a+1], cg[3][a+1], cg[2][a+1], cg[1][a], cg[5][a+1]
+  uint32_t tA0 = 0, tA1 = 0, tA2 = 0, tA3 = 0, tC5 = 0;
+  if (lane < me) {
+    tA0 = (uint32_t)cgl[4 * W + lane + 1];
+    tA1 = (uint32_t)cgl[3 * W + lane + 1];
+    tA2 = (uint32_t)cgl[2 * W + lane + 1];
+    tA3 = (uint32_t)cgl[W + lane];
+    tC5 = (uint32_t)cgl[5 * W + lane + 1];
+  }
+  uint64_t r, u0;
+  {
+    int q[MAXD];
+    u0 = unrank_prefix<MAXD>(tg, D, dp, g, q);
+    r = prefix_rank<MAXD>(tg, dp, q);
+  }
+  while (g < g_end) {
+    if (r >= S_pre) break;  // cannot happen while g < Sp; keeps the loop bounded regardless
+    {
+      DescBlock<MAXD> db;
+      const uint64_t ri = r + (uint64_t)lane;
+      if (ri < S_pre) {
+        int q[MAXD];
+        const SpaceView sv{tl.cg, tl.lo, me, W};
+        sv.unrank(ri, dp, q);
+        descend_block_info<MAXD>(tl, K, dp, q, db);
+      } else {
+        db.CBp = db.CB0 = 0;
+        db.L = 0;
+        db.toff = 0;
+        db.lastP = db.lastrun = 0;
+#pragma unroll
+        for (int i = 0; i < MAXD; ++i) {
+          db.p[i] = db.run[i] = 0;
+          db.B[i] = 0;
+        }
+      }
+      wave_sync();  // every lane is done reading the previous batch
+      uint32_t* col = tbl + lane;
+      col[0 * kBlockBatch] = (uint32_t)db.CBp;
+      col[1 * kBlockBatch] = (uint32_t)(db.CBp >> 32);
+      col[2 * kBlockBatch] = (uint32_t)db.CB0;
+      col[3 * kBlockBatch] = (uint32_t)(db.CB0 >> 32);
+      col[4 * kBlockBatch] = db.L;
+      col[5 * kBlockBatch] = db.toff;
+      col[6 * kBlockBatch] = (uint32_t)db.lastP | ((uint32_t)db.lastrun << 8);
+#pragma unroll
+      for (int w = 0; w < kPW; ++w) {
+        uint32_t pw = 0, rw = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int i = 4 * w + k;
+          if (i < MAXD) {
+            pw |= ((uint32_t)db.p[i < MAXD ? i : 0] & 0xffu) << (8 * k);
+            rw |= ((uint32_t)db.run[i < MAXD ? i : 0] & 0xffu) << (8 * k);
+          }
+        }
+        col[(7 + w) * kBlockBatch] = pw;
+        col[(7 + kPW + w) * kBlockBatch] = rw;
+      }
+#pragma unroll
+      for (int i = 0; i < MAXD; ++i) {
+        col[(kBW + 2 * i) * kBlockBatch] = (uint32_t)db.B[i];
+        col[(kBW + 2 * i + 1) * kBlockBatch] = (uint32_t)(db.B[i] >> 32);
+      }
+      wave_sync();
+    }
+    const int nb = (int)((S_pre - r < kBlockBatch) ? S_pre - r : kBlockBatch);
+    for (int j = 0; j < nb && g < g_end; ++j) {
+      const uint32_t* bj = tbl + j;
+      auto word = [&](int f) { return rfl(bj[f * kBlockBatch]); };
+      const uint32_t L = word(4);
+      if (L == 0) continue;
+      const uint64_t CBp = (uint64_t)word(0) | ((uint64_t)word(1) << 32);
+      const uint64_t CB0 = (uint64_t)word(2) | ((uint64_t)word(3) << 32);
+      const uint32_t toff = word(5);
+      const uint32_t w6 = word(6);
+      const int lastP = (int)(w6 & 0xffu), lastrun = (int)(w6 >> 8);
+      const uint32_t c5l = (uint32_t)cgl[5 * W + lastP + 1];
+      // lane a: the base of mode a's child stream (child = sb + u while a <= s0) and, below
+      // last(P), the copies of a in its children (the same for every parent of the block)
+      uint64_t sb = 0;
+      float scnt = 1.f;
+      {
+        const int a = lane;
+        if (a < lastP) {
+          int i = 0, lo_i = 0, run_i = 0;
+          uint64_t Bi = (uint64_t)word(kBW) | ((uint64_t)word(kBW + 1) << 32);
+#pragma unroll
+          for (int l = 1; l < MAXD; ++l)
+            if (l < dp) {
+              const int pl = (int)((word(7 + (l - 1) / 4) >> (8 * ((l - 1) % 4))) & 0xffu);
+              if (pl <= a) {
+                i = l;
+                lo_i = pl;
+                run_i = (int)((word(7 + kPW + l / 4) >> (8 * (l % 4))) & 0xffu);
+                Bi = (uint64_t)word(kBW + 2 * l) | ((uint64_t)word(kBW + 2 * l + 1) << 32);
+              }
+            }
+          sb = Bi - cgl[(K - i) * W + a + 1];
+          scnt = (i > 0 && a == lo_i) ? (float)(run_i + 1) : 1.f;
+        } else if (a < me) {
+          sb = CB0 + c5l - tC5;
+        }
+      }
+      const TupleWord* tb = tg.tuples + toff;
+      const uint64_t avail = g_end - g;
+      const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
+      for (uint32_t uc = (uint32_t)u0; uc < u_stop; uc += 64u) {
+        const uint32_t u = uc + (uint32_t)lane;
+        const bool in = u < u_stop;
+        const uint32_t uu = in ? u : u_stop - 1u;
+        const TupleWord w = tb[uu];
+        const int sx[kSuf] = {(int)(w & 63u), (int)((w >> 8) & 63u), (int)((w >> 16) & 63u), (int)((w >> 24) & 63u)};
+        DescLane dl;
+        desc_lane_init(cgl, W, lastP, sx, dl);
+        // every child load first (ascending modes), then the sums in the same order
+        float qv[ME];
+#pragma unroll
+        for (int a = 0; a < ME; ++a) {
+          qv[a] = 0.f;
+          if (a < me) {
+            const uint64_t sba = rl64(sb, a);
+            if (a < lastP) {
+              qv[a] = at_bytes(qk + sba, uu * 4u);
+            } else {
+              const uint32_t loc = desc_local_offset(dl, sx, a, rl(tA0, a), rl(tA1, a), rl(tA2, a), rl(tA3, a));
+              const uint64_t idx = a <= sx[0] ? sba + uu : CB0 + loc;
+              qv[a] = qk[idx];
+            }
+          }
+        }
+        float acc = 0.f;
+#pragma unroll
+        for (int a = 0; a < ME; ++a)
+          if (a < me) {
+            const float cnt = a < lastP ? __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, scnt), a))
+                                        : (float)desc_tail_copies(sx, a, lastP, lastrun);
+            acc = fmaf(qv[a], cnt, acc);
+          }
+        if (in) qout[CBp + u] = (acc * invk) * scale;
+      }
+      g += u_stop - u0;
+      u0 = 0;
+    }
+    r += kBlockBatch;
+  }
+}
+
+template <int MAXD, int ME>
+__global__ __launch_bounds__(kBlockThreads) void k_descend_block(const uint64_t* __restrict__ cg_t, const int* __restrict__ lo_t,
+                                                                 const TupleWord* __restrict__ tuples, int me, int stride,
+                                                                 int K, const float* __restrict__ qk_all, uint64_t Sk,
+                                                                 float* __restrict__ qout_all, uint64_t Sp, uint64_t S_pre,
+                                                                 uint64_t per_wave, float scale) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ncg = (K + 1) * stride;
+  uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
+  int* lo_l = reinterpret_cast<int*>(cg_l + ncg);
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = cg_t[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo_l[i] = lo_t[i];
+  __syncthreads();
+  const BlockTables tg{cg_t, lo_t, tuples, me, stride, 1};
+  const BlockTables tl{cg_l, lo_l, tuples, me, stride, 1};
+  const int64_t b = blockIdx.y;
+  const int wv = (int)rfl(threadIdx.x >> 6);
+  const uint64_t g = uniform_u64(((uint64_t)blockIdx.x * kWaves + wv) * per_wave);
+  uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + desc_table_offset(K, me)) + (size_t)wv * kBlockBatch * desc_words(MAXD);
+  descend_wave<MAXD, ME>(tg, tl, K, qk_all + (size_t)b * Sk, qout_all + (size_t)b * Sp, Sp, S_pre, per_wave, tbl, g, scale);
+}
+
+}  // namespace slos

@@ -19,6 +19,7 @@
 #include "../../include/slos_mi355x.h"
 #include "space.hpp"
 #include "layer_block.hpp"
+#include "descend_block.hpp"
 #include "pull_index.hpp"
 
 namespace {
@@ -275,7 +276,7 @@ __global__ __launch_bounds__(64) void k_spot_ryser(const uint64_t* cg_g, const i
 // (child ranks: slos::for_each_child in pull_index.hpp)
 __global__ __launch_bounds__(256) void k_descend(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k,
                                                  const float* __restrict__ qk, uint64_t Sk,
-                                                 float* __restrict__ qout, uint64_t Sp, int64_t B) {
+                                                 float* __restrict__ qout, uint64_t Sp, int64_t B, float scale) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int stride = me + 1;
   const int ncg = (k + 1) * stride;
@@ -290,8 +291,8 @@ __global__ __launch_bounds__(256) void k_descend(const uint64_t* cg_g, const int
     const float* qb = qk + (size_t)b * Sk;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < Sp; t += (uint64_t)gridDim.x * blockDim.x) {
       float acc = 0.f;
-      slos::for_each_child(sv, multiset, k, t, [&](uint64_t c, int, int cnt) { acc += qb[c] * (float)cnt; });
-      qout[(size_t)b * Sp + t] = acc * invk;
+      slos::for_each_child(sv, multiset, k, t, [&](uint64_t c, int, int cnt) { acc = fmaf(qb[c], (float)cnt, acc); });
+      qout[(size_t)b * Sp + t] = (acc * invk) * scale;
     }
   }
 }
@@ -1202,7 +1203,8 @@ int slos_adjoint_layer(const slos_plan* p, const void* U, int64_t batch, int k, 
   return SLOS_OK;
 }
 
-int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1, int64_t batch, slos_stream_t stream) {
+int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1, int64_t batch, unsigned flags,
+                      double scale, slos_stream_t stream) {
   if (!p || !q_k || !q_km1) return fail(SLOS_EINVAL, "NULL argument");
   if (k < 1 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
   if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
@@ -1210,11 +1212,39 @@ int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1,
   int rc = device_tables(p, &dt);
   if (rc) return rc;
   const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
+  const int dp = k - 1 - slos::kSuf;
+  if (!(flags & SLOS_FLAG_SIMPLE) && p->multiset && dt.tuples && dp >= 1 && dp <= 8) {
+    const int maxd = dp <= 4 ? 4 : dp <= 6 ? 6 : 8;
+    const size_t shb = slos::desc_dyn_lds(k, p->me, maxd);
+    if (shb <= 64 * 1024) {
+      uint64_t pw = (Sp / 16384 + 63) & ~63ull;
+      pw = std::min<uint64_t>(std::max<uint64_t>(pw, 256), 4096);
+      const uint64_t wgs = ((Sp + pw - 1) / pw + slos::kWaves - 1) / slos::kWaves;
+      if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
+      const dim3 grid((unsigned)wgs, (unsigned)batch);
+      const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+      const uint64_t S_pre = p->size(dp);
+#define SLOS_DESC(D, E)                                                                                                  \
+  hipLaunchKernelGGL((slos::k_descend_block<D, E>), grid, dim3(slos::kBlockThreads), shb, st, dt.cg, dt.lo, dt.tuples, \
+                     p->me, p->me + 1, k, q_k, Sk, q_km1, Sp, S_pre, pw, (float)scale)
+#define SLOS_DESC_E(D)          \
+  if (p->me <= 16) SLOS_DESC(D, 16); \
+  else if (p->me <= 32) SLOS_DESC(D, 32); \
+  else SLOS_DESC(D, 64)
+      if (maxd == 4) { SLOS_DESC_E(4); }
+      else if (maxd == 6) { SLOS_DESC_E(6); }
+      else { SLOS_DESC_E(8); }
+#undef SLOS_DESC_E
+#undef SLOS_DESC
+      HIP_TRY(hipGetLastError());
+      return SLOS_OK;
+    }
+  }
   const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
   if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 255) / 256, 1u << 20)), (unsigned)batch);
   hipLaunchKernelGGL(k_descend, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
-                     p->multiset, k, q_k, Sk, q_km1, Sp, batch);
+                     p->multiset, k, q_k, Sk, q_km1, Sp, batch, (float)scale);
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }

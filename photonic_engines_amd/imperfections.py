@@ -203,11 +203,23 @@ class _Descend(torch.autograd.Function):
         return ctx.plan.loss_ascend(ctx.k, g.contiguous().float()), None, None
 
 
-def thin_distribution(layers, probs: torch.Tensor, eta: Eta, chunk: int | None = None) -> LossyDistribution:
+def carried_weights(n: int, eta: Eta, device) -> torch.Tensor | None:
+    """The float32 binomial weights of layers 0..n when the weighted blocks can be carried down
+    the layers (a loss without gradient whose weights are all positive), else None."""
+    e = _eta(eta)
+    if e is None or (isinstance(e, torch.Tensor) and e.requires_grad):
+        return None
+    w32 = binomial_pmf(n, e, dtype=torch.float64, device=device).to(torch.float32)
+    return w32 if bool((w32 > 0).all()) else None
+
+
+def thin_distribution(layers, probs: torch.Tensor, eta: Eta, chunk: int | None = None,
+                      _top_weighted: bool = False) -> LossyDistribution:
     """Uniform output loss on the n-photon distribution `probs` ([S] or [B, S] over
     layers.final_states), returned as a LossyDistribution over 0..n photons
     (imperfections.py:309-337). Differentiable in probs and eta. `chunk` is accepted for
-    signature compatibility (the pull kernels need no edge chunks)."""
+    signature compatibility (the pull kernels need no edge chunks). `_top_weighted`: `probs` already
+    carries the weight of layer n (the forward's readout applied it)."""
     n = layers.n
     e = _eta(eta)
     user_dev = probs.device
@@ -215,6 +227,23 @@ def thin_distribution(layers, probs: torch.Tensor, eta: Eta, chunk: int | None =
     dev = nat.require_gpu(probs.device if probs.device.type == "cuda" else None)
     q_top = (probs if batched else probs.unsqueeze(0)).to(device=dev, dtype=torch.float32).contiguous()
     plan = layers._plan
+    grad = torch.is_grad_enabled() and (q_top.requires_grad or (isinstance(e, torch.Tensor) and e.requires_grad))
+    w32 = None if grad else carried_weights(n, e, dev)
+    if _top_weighted and w32 is None:
+        raise ValueError("_top_weighted needs a loss whose weights can be carried down the layers")
+    if w32 is not None:
+        # carry the weights down the layers: block[k-1] = descent(block[k]) * w[k-1] / w[k], one pass
+        # per layer and no unweighted copies
+        ratio = (w32[:-1].double() / w32[1:].double()).cpu().tolist()
+        blocks = [None] * (n + 1)
+        blocks[n] = q_top if _top_weighted else q_top * w32[n]
+        for k in range(n, 0, -1):
+            blocks[k - 1] = plan.loss_descend(k, blocks[k], scale=ratio[k - 1])
+        out = []
+        for b in blocks:
+            b = b if batched else b.squeeze(0)
+            out.append(b.to(user_dev) if user_dev.type != "cuda" else b)
+        return LossyDistribution(layers, out)
     q = [None] * (n + 1)
     q[n] = q_top
     grad = torch.is_grad_enabled() and q_top.requires_grad

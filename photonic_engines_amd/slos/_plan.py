@@ -124,10 +124,10 @@ class Plan:
                   "slos_adjoint_layer")
         return gprev
 
-    def loss_descend(self, k: int, q: torch.Tensor) -> torch.Tensor:
-        """q_{k-1} = descent of q_k ([B, S_k] float32 contiguous on the GPU)."""
+    def loss_descend(self, k: int, q: torch.Tensor, flags: int = 0, scale: float = 1.0) -> torch.Tensor:
+        """q_{k-1} = scale * descent of q_k ([B, S_k] float32 contiguous on the GPU)."""
         out = torch.empty((q.shape[0], self.sizes[k - 1]), dtype=torch.float32, device=q.device)
-        nat.check(self._lib.slos_loss_descend(self._h, k, nat.ptr(q), nat.ptr(out), q.shape[0],
+        nat.check(self._lib.slos_loss_descend(self._h, k, nat.ptr(q), nat.ptr(out), q.shape[0], flags, float(scale),
                                               nat.stream_handle(q.device)), "slos_loss_descend")
         return out
 

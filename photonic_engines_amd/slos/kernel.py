@@ -188,8 +188,17 @@ class FockSLOSKernel:
                 p = p.to(self.device) if self.device.type != "cuda" else p
             else:
                 Ug, batched = self._prepare(U)
-                p = self._out(self._plan.forward(Ug, out_kind=nat.OUT_PROBS, scale=self._input_norm,
-                                                 flags=self._flags()), batched)
+                # a loss without gradient: the readout applies the weight of layer n (|amp s|^2 w_n)
+                w32 = None
+                if loss is not None and not (isinstance(loss, torch.Tensor) and loss.requires_grad):
+                    from ..imperfections import carried_weights
+                    w32 = carried_weights(self.n, loss, Ug.device)
+                scale = self._input_norm * (math.sqrt(float(w32[self.n])) if w32 is not None else 1.0)
+                p = self._out(self._plan.forward(Ug, out_kind=nat.OUT_PROBS, scale=scale, flags=self._flags()),
+                              batched)
+                if w32 is not None:
+                    from ..imperfections import thin_distribution
+                    return thin_distribution(self.layers, p, loss, _top_weighted=True)
         except torch.cuda.OutOfMemoryError as e:
             self._add_oom_hint(e)
             raise

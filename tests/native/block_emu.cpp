@@ -2,8 +2,10 @@
 // with the kernel's own wave / block / 64-lane loop structure, in complex128. Test infrastructure:
 // it lets the CPU test suite check the GPU kernel's decomposition (block starts, prefix-stream
 // bases, suffix-tuple table, norms, wave boundaries) against the oracle without a GPU.
+#include <cmath>
 #include <complex>
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "../../photonic_engines_amd/csrc/block_index.hpp"
@@ -284,6 +286,96 @@ int64_t emu_check_tuple_words(int m, int mask) {
     if (!s.multiset && code != 0) ++bad;
   }
   return bad;
+}
+
+// The block descent's child of every parent in [first, first + count) of layer k-1 and every mode
+// (slos::descend_block_info, desc_lane_init, desc_tail_offset, desc_tail_copies, as the device
+// kernel uses them) against slos::for_each_child: same child index, same copies, same order.
+// Multiset spaces with k-1 >= SUF+1 only. Returns the number of mismatches, or -1 for bad input.
+int64_t emu_check_descend_block(int n, int m, int mask, int k, uint64_t first, uint64_t count) {
+  constexpr int MAXD = 16;
+  Space s;
+  if (!build(n, m, mask, s) || !s.multiset || k - 1 < slos::kSuf + 1 || k > n || k - 1 - slos::kSuf > MAXD) return -1;
+  const int W = s.me + 1, D = k - 1, dp = D - slos::kSuf;
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, W};
+  slos::BlockTables t{s.cg.data(), s.lo.data(), s.tuples.data(), s.me, W, s.multiset};
+  const uint64_t Sp = s.cg[(size_t)D * W], Sk = s.cg[(size_t)k * W];
+  int64_t bad = 0;
+  for (uint64_t g = first; g < first + count && g < Sp; ++g) {
+    int q[MAXD];
+    const uint64_t u = slos::unrank_prefix<MAXD>(t, D, dp, g, q);
+    slos::DescBlock<MAXD> db;
+    slos::descend_block_info<MAXD>(t, k, dp, q, db);
+    if (db.CBp + u != g || u >= db.L) { ++bad; continue; }
+    const slos::TupleWord w = s.tuples[db.toff + u];
+    int sx[slos::kSuf];
+    for (int j = 0; j < slos::kSuf; ++j) sx[j] = (int)((w >> (8 * j)) & 63u);
+    slos::DescLane dl;
+    slos::desc_lane_init(s.cg.data(), W, db.lastP, sx, dl);
+    std::vector<std::pair<uint64_t, int>> mine, ref;
+    int lo_i = 0;
+    for (int i = 0; i < dp; ++i) {
+      for (int a = lo_i; a < db.p[i]; ++a)
+        mine.push_back({db.B[i] - s.cg[(size_t)(k - i) * W + a + 1] + u, (i > 0 && a == lo_i) ? db.run[i] + 1 : 1});
+      lo_i = db.p[i];
+    }
+    const uint32_t c5l = (uint32_t)s.cg[(size_t)5 * W + db.lastP + 1];
+    for (int a = db.lastP; a < s.me; ++a) {
+      const uint32_t A[4] = {(uint32_t)s.cg[(size_t)4 * W + a + 1], (uint32_t)s.cg[(size_t)3 * W + a + 1],
+                             (uint32_t)s.cg[(size_t)2 * W + a + 1], (uint32_t)s.cg[(size_t)W + a]};
+      const uint32_t S5a = c5l - (uint32_t)s.cg[(size_t)5 * W + a + 1];
+      mine.push_back({db.CB0 + slos::desc_tail_offset(dl, sx, (uint32_t)u, a, S5a, A),
+                      slos::desc_tail_copies(sx, a, db.lastP, db.lastrun)});
+    }
+    slos::for_each_child(sv, s.multiset, k, g, [&](uint64_t r, int, int cnt) { ref.push_back({r, cnt}); });
+    if (mine != ref) ++bad;
+    for (auto& e : mine)
+      if (e.first >= Sk) ++bad;
+  }
+  return bad;
+}
+
+// The block descent on a whole layer, in the device kernel's order of operations (float32 sums of
+// q * copies in ascending mode order, then times 1/k): out[t] for every parent t of layer k-1.
+int emu_descend_block(int n, int m, int mask, int k, const float* qk, float* out) {
+  constexpr int MAXD = 16;
+  Space s;
+  if (!build(n, m, mask, s) || !s.multiset || k - 1 < slos::kSuf + 1 || k > n) return 1;
+  const int W = s.me + 1, D = k - 1, dp = D - slos::kSuf;
+  slos::BlockTables t{s.cg.data(), s.lo.data(), s.tuples.data(), s.me, W, s.multiset};
+  const uint64_t Sp = s.cg[(size_t)D * W], Sk = s.cg[(size_t)k * W];
+  const float invk = 1.f / (float)k;
+  for (uint64_t g = 0; g < Sp; ++g) {
+    int q[MAXD];
+    const uint64_t u = slos::unrank_prefix<MAXD>(t, D, dp, g, q);
+    slos::DescBlock<MAXD> db;
+    slos::descend_block_info<MAXD>(t, k, dp, q, db);
+    const slos::TupleWord w = s.tuples[db.toff + u];
+    int sx[slos::kSuf];
+    for (int j = 0; j < slos::kSuf; ++j) sx[j] = (int)((w >> (8 * j)) & 63u);
+    slos::DescLane dl;
+    slos::desc_lane_init(s.cg.data(), W, db.lastP, sx, dl);
+    float acc = 0.f;
+    int lo_i = 0;
+    for (int i = 0; i < dp; ++i) {
+      for (int a = lo_i; a < db.p[i]; ++a) {
+        const uint64_t c = db.B[i] - s.cg[(size_t)(k - i) * W + a + 1] + u;
+        if (c >= Sk) return 2;
+        acc = std::fma(qk[c], (float)((i > 0 && a == lo_i) ? db.run[i] + 1 : 1), acc);
+      }
+      lo_i = db.p[i];
+    }
+    const uint32_t c5l = (uint32_t)s.cg[(size_t)5 * W + db.lastP + 1];
+    for (int a = db.lastP; a < s.me; ++a) {
+      const uint32_t A[4] = {(uint32_t)s.cg[(size_t)4 * W + a + 1], (uint32_t)s.cg[(size_t)3 * W + a + 1],
+                             (uint32_t)s.cg[(size_t)2 * W + a + 1], (uint32_t)s.cg[(size_t)W + a]};
+      const uint64_t c = db.CB0 + slos::desc_tail_offset(dl, sx, (uint32_t)u, a, c5l - (uint32_t)s.cg[(size_t)5 * W + a + 1], A);
+      if (c >= Sk) return 2;
+      acc = std::fma(qk[c], (float)slos::desc_tail_copies(sx, a, db.lastP, db.lastrun), acc);
+    }
+    out[g] = acc * invk;
+  }
+  return 0;
 }
 
 int64_t emu_check_children(int n, int m, int mask, int k, uint64_t first, uint64_t count) {

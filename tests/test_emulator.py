@@ -87,3 +87,42 @@ def test_tuple_words_and_local_parent_ranks(emu, m, mask):
     per-position terms cg[3][x+1], cg[2][x+1], cg[1][x], and the run data from the mark code equal the
     direct unrank / rank / run computation."""
     assert emu.emu_check_tuple_words(m, pull.MASK_IDS[mask]) == 0
+
+
+@pytest.mark.parametrize("n,m,mask,k,first", [(6, 5, "fock", 6, 0), (7, 6, "fock", 7, 0), (7, 6, "fock", 6, 0),
+                                              (8, 9, "fock", 8, 0), (9, 7, "tail_empty", 9, 0), (10, 12, "fock", 10, 0),
+                                              (12, 6, "fock", 12, 0), (13, 5, "fock", 13, 0),
+                                              (11, 32, "fock", 11, 0), (11, 32, "fock", 11, 560549704),
+                                              (11, 32, "fock", 11, 1121099408 - 200000), (11, 32, "fock", 10, 136719440)])
+def test_descend_block_child_indices(emu, n, m, mask, k, first):
+    """The block descent's child of each parent and mode (segment bases, the stream of block (P, a),
+    the tail offsets in block (P, s0)) and its copies equal slos::for_each_child, in the same order,
+    for 200,000 parents from `first` (n=11, m=32: the start, middle and end of layers 10 and 9)."""
+    emu.emu_check_descend_block.restype = ctypes.c_int64
+    emu.emu_check_descend_block.argtypes = [ctypes.c_int] * 4 + [ctypes.c_uint64] * 2
+    assert emu.emu_check_descend_block(n, m, pull.MASK_IDS[mask], k, first, 200_000) == 0
+
+
+@pytest.mark.parametrize("n,m,mask,occ", [(6, 6, "fock", None), (7, 5, "fock", [3, 0, 2, 1, 1]),
+                                          (8, 7, "tail_empty", [2, 2, 1, 1, 1, 1, 0]), (9, 4, "fock", [9, 0, 0, 0])])
+def test_descend_block_emulation(emu, n, m, mask, occ):
+    """The block descent emulated in float32 (the device's order of operations) against the
+    double-precision pull descent on the oracle's loss of a real distribution."""
+    occ = occ or occupation_list(n, m, mask)
+    U = haar_unitary(m, 5 * n + m).numpy()
+    q = pull.forward_probs(n, m, occ, U, mask)
+    for k in range(n, slos_min_k(), -1):
+        q32 = np.ascontiguousarray(q, dtype=np.float32)
+        out32 = np.zeros(len(so.enumerate_layers(k - 1, m, mask)[k - 1]), dtype=np.float32)
+        assert emu.emu_descend_block(n, m, pull.MASK_IDS[mask], k, _p(q32), _p(out32)) == 0
+        ref = np.zeros(len(out32))
+        q64 = np.ascontiguousarray(q, dtype=np.float64)
+        assert emu.emu_descend(n, m, pull.MASK_IDS[mask], k, _p(q64), _p(ref)) == 0
+        assert np.abs(out32 - ref).max() <= 1e-6 * max(ref.max(), 1e-30)
+        assert abs(out32.astype(np.float64).sum() - q64.sum()) < 1e-5     # the descent keeps the mass
+        q = ref
+
+
+def slos_min_k():
+    """The block descent covers k - 1 >= 5 (a parent prefix of at least one position)."""
+    return 5
