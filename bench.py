@@ -12,9 +12,12 @@ against FP64 Ryser permanents computed on the GPU, and the FP64 total mass.
 `cpu_baseline`: the oracle's C++/OpenMP pull-form restatement on this host's cores (a bounded
 sample: layers 1..10 complete plus the first 2^27 states of layer 11, the full-run time
 extrapolated linearly in layer-11 states).
-`variants`: the compensated-accumulation forward of the same workload, and the uneven batch of
-512 unitaries ((4,12), (6,16), (8,20)) as a single-U loop, per-shape batches, one group launch
-per layer, and the autotuned group forward replayed as a hipGraph.
+`variants`: the compensated-accumulation forward of the same workload; the lossy output
+(probs(U, loss=0.9): the forward with the top binomial weight in its readout and the 11 descent
+steps, with the k=11 descent's time and roofline and the FP64 mass of every photon-number block
+against the binomial law); and the uneven batch of 512 unitaries ((4,12), (6,16), (8,20)) as a
+single-U loop, per-shape batches, one group launch per layer, and the autotuned group forward
+replayed as a hipGraph.
 """
 from __future__ import annotations
 
@@ -116,6 +119,42 @@ def bench_uneven(reps: int = 5) -> dict:
     return res
 
 
+def bench_lossy(kern, U: torch.Tensor, eta: float = 0.9, steps: int = 3) -> dict:
+    """probs(U, loss=eta) end to end, and the top descent step alone (HIP events on its stream)."""
+    from photonic_engines_amd.imperfections import binomial_pmf
+    n, plan, S = kern.n, kern._plan, kern._plan.sizes
+    ld = kern.probs(U, loss=eta)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        del ld
+        ld = kern.probs(U, loss=eta)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    pmf = binomial_pmf(n, eta).tolist()
+    masses = [float(kern.total_mass(q)) for q in ld.probs]
+    top = ld.probs[n].unsqueeze(0)
+    del ld
+    stream = torch.cuda.current_stream()
+    ms = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        q = plan.loss_descend(n, top)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+        del q
+    d_ms = float(np.median(ms))
+    d_bytes = 4 * S[n] + 4 * S[n - 1]
+    outputs = int(sum(S[: n + 1]))
+    return {"eta": eta, "ms_per_step": dt * 1e3, "outputs": outputs, "probs_per_s": outputs / dt,
+            "descent_top": {"kernel": f"k_descend_block (layer {n} -> {n - 1})", "kernel_ms": d_ms,
+                            "alg_bytes": d_bytes, "achieved_GBps": d_bytes / (d_ms * 1e-3) / 1e9,
+                            "frac": d_bytes / (d_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
+            "max_abs_block_mass_minus_binomial": max(abs(a - b) for a, b in zip(masses, pmf))}
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -137,6 +176,7 @@ def main() -> None:
     ap.add_argument("--no-compensated", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 Ryser spot checks (profiling passes)")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batch measurement")
+    ap.add_argument("--no-lossy", action="store_true", help="skip the lossy-output measurement")
     ap.add_argument("--copy-calibration-gb", type=float, default=0.0,
                     help="also run one plain device copy of this many GB (HBM counter calibration)")
     args = ap.parse_args()
@@ -246,6 +286,9 @@ def main() -> None:
                                            "max_rel": float((np.abs(gc - ref) / np.maximum(ref, 1e-300)).max())}}
         del pc
     torch.cuda.empty_cache()
+    if not args.no_lossy:
+        out.setdefault("variants", {})["lossy"] = bench_lossy(kern, U[0])
+        torch.cuda.empty_cache()
     if not args.no_batched:
         out.setdefault("variants", {})["batched_uneven"] = bench_uneven()
         torch.cuda.empty_cache()
