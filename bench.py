@@ -15,9 +15,10 @@ extrapolated linearly in layer-11 states).
 `variants`: the compensated-accumulation forward of the same workload; the lossy output
 (probs(U, loss=0.9): the forward with the top binomial weight in its readout and the 11 descent
 steps, with the k=11 descent's time and roofline and the FP64 mass of every photon-number block
-against the binomial law); and the uneven batch of 512 unitaries ((4,12), (6,16), (8,20)) as a
-single-U loop, per-shape batches, one group launch per layer, and the autotuned group forward
-replayed as a hipGraph.
+against the binomial law); the gradient step of n=6, m=16 (forward, L = -mean log p over 1,024
+outputs drawn from the exact distribution, dL/dU through the hand-written adjoint); and the
+uneven batch of 512 unitaries ((4,12), (6,16), (8,20)) as a single-U loop, per-shape batches, one
+group launch per layer, and the autotuned group forward replayed as a hipGraph.
 """
 from __future__ import annotations
 
@@ -155,6 +156,35 @@ def bench_lossy(kern, U: torch.Tensor, eta: float = 0.9, steps: int = 3) -> dict
             "max_abs_block_mass_minus_binomial": max(abs(a - b) for a, b in zip(masses, pmf))}
 
 
+def bench_gradient(steps: int = 20) -> dict:
+    """n=6, m=16: one step = probs(U) with U requiring a gradient, L = -mean log p[idx] over 1,024
+    indices drawn (seeded) from the exact distribution, L.backward() (adjoint kernels)."""
+    from photonic_engines_amd.slos import FockSLOSKernel, haar_unitary
+    n, m = 6, 16
+    k = FockSLOSKernel(n, m, [1] * n + [0] * (m - n), device="cuda")
+    U0 = haar_unitary(m, 6016).to(torch.complex64).cuda()
+    with torch.no_grad():
+        p = k.probs(U0)
+    idx = torch.multinomial(p.double(), 1024, replacement=True, generator=torch.Generator(device="cuda").manual_seed(4))
+
+    def step():
+        U = U0.clone().requires_grad_(True)
+        loss = -torch.log(k.probs(U)[idx]).mean()
+        loss.backward()
+        return U.grad
+
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {"workload": "n=6 m=16, first 6 modes, L = -mean log p over 1,024 sampled outputs", "states": int(p.numel()),
+            "ms_per_step": dt * 1e3, "grad_nonzero_columns": int((g.abs().sum(0) > 0).sum())}
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -177,6 +207,7 @@ def main() -> None:
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 Ryser spot checks (profiling passes)")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batch measurement")
     ap.add_argument("--no-lossy", action="store_true", help="skip the lossy-output measurement")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the gradient-step measurement")
     ap.add_argument("--copy-calibration-gb", type=float, default=0.0,
                     help="also run one plain device copy of this many GB (HBM counter calibration)")
     args = ap.parse_args()
@@ -289,6 +320,8 @@ def main() -> None:
     if not args.no_lossy:
         out.setdefault("variants", {})["lossy"] = bench_lossy(kern, U[0])
         torch.cuda.empty_cache()
+    if not args.no_gradient:
+        out.setdefault("variants", {})["gradient_n6_m16"] = bench_gradient()
     if not args.no_batched:
         out.setdefault("variants", {})["batched_uneven"] = bench_uneven()
         torch.cuda.empty_cache()

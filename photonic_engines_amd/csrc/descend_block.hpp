@@ -28,7 +28,110 @@ __host__ __device__ inline size_t desc_dyn_lds(int K, int me, int maxd) {
   return desc_table_offset(K, me) + (size_t)kWaves * kBlockBatch * desc_words(maxd) * 4;
 }
 
-// ME: a compile-time bound on the mode count (me <= ME <= 64), so that the mode loop unrolls and
+// per-block state of the parent sweep (wave-uniform except the lane-held tables)
+struct DescCtx {
+  const float* qk;        // child layer
+  float* qout;            // parent layer
+  const TupleWord* tb;    // tuple words of the block's parents
+  const uint64_t* cgl;    // counting table (LDS)
+  uint64_t CBp, CB0;
+  uint64_t sb;            // lane a: base of mode a's child stream
+  float scnt;             // lane a < last(P): copies of a in the children
+  uint32_t tA0, tA1, tA2, tA3;  // lane a: cg[4][a+1], cg[3][a+1], cg[2][a+1], cg[1][a]
+  uint32_t u_stop;
+  int W, me, lastP, lastrun, lane;
+  float invk, scale;
+};
+
+// q_K of one child (mode a) of each parent held by a lane, one parent per lane
+template <int ME>
+__device__ __forceinline__ void descend_step1(const DescCtx& c, uint32_t uc) {
+  const uint32_t u = uc + (uint32_t)c.lane;
+  const bool in = u < c.u_stop;
+  const uint32_t uu = in ? u : c.u_stop - 1u;
+  const TupleWord w = c.tb[uu];
+  const int sx[kSuf] = {(int)(w & 63u), (int)((w >> 8) & 63u), (int)((w >> 16) & 63u), (int)((w >> 24) & 63u)};
+  DescLane dl;
+  desc_lane_init(c.cgl, c.W, c.lastP, sx, dl);
+  // every child load first (ascending modes), then the sums in the same order
+  float qv[ME];
+#pragma unroll
+  for (int a = 0; a < ME; ++a) {
+    qv[a] = 0.f;
+    if (a < c.me) {
+      const uint64_t sba = rl64(c.sb, a);
+      if (a < c.lastP) {
+        qv[a] = at_bytes(c.qk + sba, uu * 4u);
+      } else {
+        const uint32_t loc = desc_local_offset(dl, sx, a, rl(c.tA0, a), rl(c.tA1, a), rl(c.tA2, a), rl(c.tA3, a));
+        const uint64_t idx = a <= sx[0] ? sba + uu : c.CB0 + loc;
+        qv[a] = c.qk[idx];
+      }
+    }
+  }
+  float acc = 0.f;
+#pragma unroll
+  for (int a = 0; a < ME; ++a)
+    if (a < c.me) {
+      const float cnt = a < c.lastP ? __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, c.scnt), a))
+                                    : (float)desc_tail_copies(sx, a, c.lastP, c.lastrun);
+      acc = fmaf(qv[a], cnt, acc);
+    }
+  if (in) c.qout[c.CBp + u] = (acc * c.invk) * c.scale;
+}
+
+// 128 parents from uc (uc + 128 <= u_stop), two adjacent per lane. A mode below last(P), or not
+// above the step's smallest s0 (lane 0's first parent: s0 never decreases inside a block), reaches
+// both children through one 8-byte load of its stream; the other modes load per parent.
+template <int ME>
+__device__ __forceinline__ void descend_step2(const DescCtx& c, uint32_t uc) {
+  const uint32_t u = uc + 2u * (uint32_t)c.lane;
+  uint2 w2;
+  __builtin_memcpy(&w2, reinterpret_cast<const char*>(c.tb) + u * 4u, 8);
+  const int s0x[kSuf] = {(int)(w2.x & 63u), (int)((w2.x >> 8) & 63u), (int)((w2.x >> 16) & 63u), (int)((w2.x >> 24) & 63u)};
+  const int s1x[kSuf] = {(int)(w2.y & 63u), (int)((w2.y >> 8) & 63u), (int)((w2.y >> 16) & 63u), (int)((w2.y >> 24) & 63u)};
+  const int s0min = (int)rfl((uint32_t)s0x[0]);
+  DescLane d0, d1;
+  desc_lane_init(c.cgl, c.W, c.lastP, s0x, d0);
+  desc_lane_init(c.cgl, c.W, c.lastP, s1x, d1);
+  float q0[ME], q1[ME];
+#pragma unroll
+  for (int a = 0; a < ME; ++a) {
+    q0[a] = q1[a] = 0.f;
+    if (a < c.me) {
+      const uint64_t sba = rl64(c.sb, a);
+      if (a < c.lastP || a <= s0min) {
+        float2 v;
+        __builtin_memcpy(&v, reinterpret_cast<const char*>(c.qk + sba) + u * 4u, 8);
+        q0[a] = v.x;
+        q1[a] = v.y;
+      } else {
+        const uint32_t A0 = rl(c.tA0, a), A1 = rl(c.tA1, a), A2 = rl(c.tA2, a), A3 = rl(c.tA3, a);
+        const uint64_t i0 = a <= s0x[0] ? sba + u : c.CB0 + desc_local_offset(d0, s0x, a, A0, A1, A2, A3);
+        const uint64_t i1 = a <= s1x[0] ? sba + u + 1u : c.CB0 + desc_local_offset(d1, s1x, a, A0, A1, A2, A3);
+        q0[a] = c.qk[i0];
+        q1[a] = c.qk[i1];
+      }
+    }
+  }
+  float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+  for (int a = 0; a < ME; ++a)
+    if (a < c.me) {
+      if (a < c.lastP) {
+        const float cnt = __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, c.scnt), a));
+        acc0 = fmaf(q0[a], cnt, acc0);
+        acc1 = fmaf(q1[a], cnt, acc1);
+      } else {
+        acc0 = fmaf(q0[a], (float)desc_tail_copies(s0x, a, c.lastP, c.lastrun), acc0);
+        acc1 = fmaf(q1[a], (float)desc_tail_copies(s1x, a, c.lastP, c.lastrun), acc1);
+      }
+    }
+  const float2 o = make_float2((acc0 * c.invk) * c.scale, (acc1 * c.invk) * c.scale);
+  __builtin_memcpy(c.qout + c.CBp + u, &o, 8);
+}
+
+// ME: a compile-time bound on the mode count (me <= ME <= 64), so that the mode loops unroll and
 // every child load of a step is issued before the first sum.
 template <int MAXD, int ME>
 __device__ __forceinline__ void descend_wave(const BlockTables& tg, const BlockTables& tl, int K, const float* __restrict__ qk,
@@ -147,43 +250,13 @@ __device__ __forceinline__ void descend_wave(const BlockTables& tg, const BlockT
           sb = CB0 + c5l - tC5;
         }
       }
-      const TupleWord* tb = tg.tuples + toff;
       const uint64_t avail = g_end - g;
       const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
-      for (uint32_t uc = (uint32_t)u0; uc < u_stop; uc += 64u) {
-        const uint32_t u = uc + (uint32_t)lane;
-        const bool in = u < u_stop;
-        const uint32_t uu = in ? u : u_stop - 1u;
-        const TupleWord w = tb[uu];
-        const int sx[kSuf] = {(int)(w & 63u), (int)((w >> 8) & 63u), (int)((w >> 16) & 63u), (int)((w >> 24) & 63u)};
-        DescLane dl;
-        desc_lane_init(cgl, W, lastP, sx, dl);
-        // every child load first (ascending modes), then the sums in the same order
-        float qv[ME];
-#pragma unroll
-        for (int a = 0; a < ME; ++a) {
-          qv[a] = 0.f;
-          if (a < me) {
-            const uint64_t sba = rl64(sb, a);
-            if (a < lastP) {
-              qv[a] = at_bytes(qk + sba, uu * 4u);
-            } else {
-              const uint32_t loc = desc_local_offset(dl, sx, a, rl(tA0, a), rl(tA1, a), rl(tA2, a), rl(tA3, a));
-              const uint64_t idx = a <= sx[0] ? sba + uu : CB0 + loc;
-              qv[a] = qk[idx];
-            }
-          }
-        }
-        float acc = 0.f;
-#pragma unroll
-        for (int a = 0; a < ME; ++a)
-          if (a < me) {
-            const float cnt = a < lastP ? __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, scnt), a))
-                                        : (float)desc_tail_copies(sx, a, lastP, lastrun);
-            acc = fmaf(qv[a], cnt, acc);
-          }
-        if (in) qout[CBp + u] = (acc * invk) * scale;
-      }
+      DescCtx c{qk, qout, tg.tuples + toff, cgl, CBp, CB0, sb, scnt, tA0, tA1, tA2, tA3, u_stop, W, me, lastP, lastrun,
+                lane, invk, scale};
+      uint32_t uc = (uint32_t)u0;
+      for (; uc + 128u <= u_stop; uc += 128u) descend_step2<ME>(c, uc);
+      for (; uc < u_stop; uc += 64u) descend_step1<ME>(c, uc);
       g += u_stop - u0;
       u0 = 0;
     }
