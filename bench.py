@@ -18,7 +18,10 @@ steps, with the k=11 descent's time and roofline and the FP64 mass of every phot
 against the binomial law); the gradient step of n=6, m=16 (forward, L = -mean log p over 1,024
 outputs drawn from the exact distribution, dL/dU through the hand-written adjoint); and the
 uneven batch of 512 unitaries ((4,12), (6,16), (8,20)) as a single-U loop, per-shape batches, one
-group launch per layer, and the autotuned group forward replayed as a hipGraph.
+group launch per layer, and the autotuned group forward replayed as a hipGraph; and the components
+built on the path (`next_rows`): device sampling of 10^6 patterns from the n=11, m=32
+distribution, the x-model and a 4-component coherent superposition at n=8, m=24, grouped
+probabilities, and rank lookups of the strong-simulation backend at n=11, m=32.
 """
 from __future__ import annotations
 
@@ -185,6 +188,73 @@ def bench_gradient(steps: int = 20) -> dict:
             "ms_per_step": dt * 1e3, "grad_nonzero_columns": int((g.abs().sum(0) > 0).sum())}
 
 
+def bench_next_rows(kern, U: torch.Tensor) -> dict:
+    """Timings of the components built on the path (sampling, x-model, superposition, grouped
+    probabilities, backend lookups); their parity is in the GPU tests."""
+    from photonic_engines_amd.adapters.perceval import StrongBackend
+    from photonic_engines_amd.slos import FockSLOSKernel, SuperpositionSLOSKernel, haar_unitary
+    from photonic_engines_amd.slos.sampling import draw_indices, unrank_rows
+
+    def clock(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, r
+
+    out = {}
+    n, S = kern.n, kern._plan.sizes[kern.n]
+    p = kern.probs(U)
+    shots = 1_000_000
+    g = torch.Generator().manual_seed(17)
+    dt_draw, idx = clock(lambda: draw_indices(p, shots, generator=g))
+    dt_rows, _ = clock(lambda: unrank_rows(kern.layers, n, idx))
+    out["sampling_n11_m32"] = {"shots": shots, "draw_ms": dt_draw * 1e3, "unrank_ms": dt_rows * 1e3,
+                               "draw_read_GBps": 4 * S / dt_draw / 1e9, "frac": 4 * S / dt_draw / 1e9 / PEAK_HBM_GBS}
+    del p, idx
+    torch.cuda.empty_cache()
+    n2, m2 = 8, 24
+    U2 = haar_unitary(m2, 8024).to(torch.complex64).cuda()
+    k2 = FockSLOSKernel(n2, m2, [1] * n2 + [0] * (m2 - n2), device="cuda")
+    S2 = k2.output_size()
+    dt_plain, _ = clock(lambda: k2.probs(U2))
+    dt_x, px = clock(lambda: k2.probs(U2, indist=0.9))
+    dt_x2, _ = clock(lambda: k2.probs(U2, indist=0.9, order=2))
+    out["xmodel_n8_m24"] = {"x": 0.9, "lossless_ms": dt_plain * 1e3, "all_subsets_ms": dt_x * 1e3, "order2_ms": dt_x2 * 1e3,
+                            "mass_fp64": float(k2.total_mass(px))}
+    rng = np.random.default_rng(5)
+    occs = []
+    for _ in range(4):
+        o = np.zeros(m2, dtype=int)
+        np.add.at(o, rng.choice(m2, size=n2, replace=False), 1)
+        occs.append(o.tolist())
+    ks = SuperpositionSLOSKernel(n2, m2, occs, device="cuda")
+    dt_s, ps = clock(lambda: ks.probs(U2))
+    out["superposition_n8_m24_K4"] = {"ms": dt_s * 1e3, "probs_per_s": S2 / dt_s, "mass_fp64": float(k2.total_mass(ps))}
+    occ2 = torch.as_tensor(k2.layers.occupations(n2))
+    groups = occ2[:, : m2 // 2].sum(1).to(torch.int64)          # photons in the first half of the modes
+    kg = k2.group_matrix(groups)
+    dt_g, _ = clock(lambda: kg.probs(U2))
+    out["grouped_n8_m24"] = {"groups": kg.output_size(), "ms": dt_g * 1e3}
+    be = StrongBackend(device="cuda")
+    be.set_circuit(U.cpu().numpy())
+    be.set_input_state([1] * n + [0] * (kern.m - n))
+    t0 = time.perf_counter()
+    be.probability([1] * n + [0] * (kern.m - n))
+    t_first = time.perf_counter() - t0
+    rows = unrank_rows(kern.layers, n, torch.as_tensor(rng.integers(0, S, 200, dtype=np.int64))).tolist()
+    t0 = time.perf_counter()
+    for r in rows:
+        be.probability(r)
+    dt_l = (time.perf_counter() - t0) / len(rows)
+    out["backend_n11_m32"] = {"first_probability_ms": t_first * 1e3, "lookup_us": dt_l * 1e6}
+    del be
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -208,6 +278,8 @@ def main() -> None:
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batch measurement")
     ap.add_argument("--no-lossy", action="store_true", help="skip the lossy-output measurement")
     ap.add_argument("--no-gradient", action="store_true", help="skip the gradient-step measurement")
+    ap.add_argument("--no-next-rows", action="store_true",
+                    help="skip the sampling / x-model / superposition / grouped / backend timings")
     ap.add_argument("--copy-calibration-gb", type=float, default=0.0,
                     help="also run one plain device copy of this many GB (HBM counter calibration)")
     args = ap.parse_args()
@@ -322,6 +394,9 @@ def main() -> None:
         torch.cuda.empty_cache()
     if not args.no_gradient:
         out.setdefault("variants", {})["gradient_n6_m16"] = bench_gradient()
+    if not args.no_next_rows:
+        out.setdefault("variants", {})["next_rows"] = bench_next_rows(kern, U[0])
+        torch.cuda.empty_cache()
     if not args.no_batched:
         out.setdefault("variants", {})["batched_uneven"] = bench_uneven()
         torch.cuda.empty_cache()

@@ -184,7 +184,8 @@ int slos_spot_probs(const slos_plan* plan, const void* U_c128, const uint64_t* i
  * totals), then per shot u = uniforms[s] * total: the first block whose CDF reaches u, the first
  * chunk and entry whose in-order prefix reaches the remainder (clamped to the block / count end).
  * probs: device float32 [count] (non-negative, unnormalised); uniforms: device float64 [shots] in
- * [0, 1); idx: device uint64 [shots]. count <= 2^32. */
+ * [0, 1); idx: device uint64 [shots]. count <= 2^32. The workspace holds the block sums, the block
+ * CDF and the FP64 chunk sums (count / 64 of them). */
 int slos_sample_workspace(uint64_t count, size_t* bytes);
 int slos_sample(const float* probs, uint64_t count, const double* uniforms, int64_t shots, uint64_t* idx,
                 void* workspace, size_t workspace_bytes, slos_stream_t stream);

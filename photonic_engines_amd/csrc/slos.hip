@@ -560,22 +560,40 @@ constexpr int kSampleChunk = 64;
 constexpr int kSampleBlock = kSampleChunk * 64;   // 4096 entries
 constexpr int kSampleSeg = 1024;                   // blocks per prefix segment
 
-__device__ __forceinline__ double chunk_sum(const float* __restrict__ p, uint64_t S, uint64_t first) {
-  double acc = 0.0;
-  for (int k = 0; k < kSampleChunk; ++k) {
-    const uint64_t i = first + (uint64_t)k;
-    acc += i < S ? (double)p[i] : 0.0;
-  }
-  return acc;
-}
-
-// one wave per block: lane c sums chunk c, lane 0 sums the 64 chunk sums in order
-__global__ __launch_bounds__(256) void k_sample_block_sums(const float* __restrict__ p, uint64_t S, uint64_t nb,
-                                                           double* __restrict__ bsum) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t b = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+// One wave per block of 4096 entries: 16-byte coalesced loads into an LDS tile of 64 rows (one per
+// chunk, padded to 65 columns so that lane c's row reads do not collide), then lane c sums chunk c
+// in index order (FP64, from 0.0; entries past S count as 0) and stores it, and every lane sums the
+// 64 chunk sums in order.
+constexpr int kSumWaves = 2;
+__global__ __launch_bounds__(64 * kSumWaves) void k_sample_block_sums(const float* __restrict__ p, uint64_t S, uint64_t nb,
+                                                                      double* __restrict__ bsum,
+                                                                      double* __restrict__ csum) {
+  __shared__ float tile[kSumWaves][kSampleChunk * (kSampleChunk + 1)];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t b = (uint64_t)blockIdx.x * kSumWaves + (uint64_t)w;
   if (b >= nb) return;
-  const double cs = chunk_sum(p, S, b * kSampleBlock + (uint64_t)lane * kSampleChunk);
+  float* t = tile[w];
+  const uint64_t base = b * kSampleBlock;
+#pragma unroll 4
+  for (int j = 0; j < kSampleBlock / 256; ++j) {
+    const int e0 = j * 256 + 4 * lane;  // entry of the block: row e0 / 64, columns e0 % 64 .. + 3
+    const uint64_t i0 = base + (uint64_t)e0;
+    float v[4];
+    if (i0 + 4 <= S) {
+      __builtin_memcpy(v, p + i0, 16);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] = i0 + e < S ? p[i0 + e] : 0.f;
+    }
+    float* row = t + (e0 >> 6) * (kSampleChunk + 1) + (e0 & 63);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) row[e] = v[e];
+  }
+  slos::wave_sync();
+  double cs = 0.0;
+  const float* mine = t + lane * (kSampleChunk + 1);
+  for (int k = 0; k < kSampleChunk; ++k) cs += (double)mine[k];
+  csum[b * 64 + (uint64_t)lane] = cs;
   double tot = 0.0;
   for (int c = 0; c < 64; ++c) tot += __shfl(cs, c, 64);
   if (lane == 0) bsum[b] = tot;
@@ -611,8 +629,9 @@ __global__ __launch_bounds__(1024) void k_sample_block_cdf(const double* __restr
 // one wave per shot: u = uniform * total; block = first cdf >= u (binary search), then the chunk
 // and the entry by in-order prefix sums; every lane computes the same values (lane 0 stores)
 __global__ __launch_bounds__(256) void k_sample_search(const float* __restrict__ p, uint64_t S, uint64_t nb,
-                                                       const double* __restrict__ cdf, const double* __restrict__ uni,
-                                                       int64_t shots, uint64_t* __restrict__ idx) {
+                                                       const double* __restrict__ cdf, const double* __restrict__ csum,
+                                                       const double* __restrict__ uni, int64_t shots,
+                                                       uint64_t* __restrict__ idx) {
   const int lane = threadIdx.x & 63;
   const int64_t s = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (s >= shots) return;
@@ -626,7 +645,7 @@ __global__ __launch_bounds__(256) void k_sample_search(const float* __restrict__
   }
   const uint64_t b = lo;
   double r = u - (b > 0 ? cdf[b - 1] : 0.0);
-  const double cs = chunk_sum(p, S, b * kSampleBlock + (uint64_t)lane * kSampleChunk);
+  const double cs = csum[b * 64 + (uint64_t)lane];  // the block pass's chunk sums
   int c = 63;
   double pre = 0.0;
   for (int q = 0; q < 64; ++q) {
@@ -641,9 +660,10 @@ __global__ __launch_bounds__(256) void k_sample_search(const float* __restrict__
   const uint64_t first = b * kSampleBlock + (uint64_t)c * kSampleChunk;
   int k = 63;
   double acc = 0.0;
+  const uint64_t il = first + (uint64_t)lane;
+  const float pl = il < S ? p[il] : 0.f;  // the chunk's entries, one per lane
   for (int q = 0; q < kSampleChunk; ++q) {
-    const uint64_t i = first + (uint64_t)q;
-    acc += i < S ? (double)p[i] : 0.0;
+    acc += (double)__shfl(pl, q, 64);
     if (acc >= r) {
       k = q;
       break;
@@ -1324,7 +1344,7 @@ int slos_sample_workspace(uint64_t count, size_t* bytes) {
   if (count < 1) return fail(SLOS_EINVAL, "count must be >= 1");
   const uint64_t nb = (count + kSampleBlock - 1) / kSampleBlock;
   if (nb > (uint64_t)kSampleSeg * 1024) return fail(SLOS_ERANGE, "too many categories for the sampler");
-  *bytes = 2 * align256(nb * sizeof(double));
+  *bytes = 2 * align256(nb * sizeof(double)) + align256(nb * 64 * sizeof(double));
   return SLOS_OK;
 }
 
@@ -1340,14 +1360,17 @@ int slos_sample(const float* probs, uint64_t count, const double* uniforms, int6
   const uint64_t nb = (count + kSampleBlock - 1) / kSampleBlock;
   double* bsum = static_cast<double*>(ws);
   double* cdf = reinterpret_cast<double*>(static_cast<unsigned char*>(ws) + align256(nb * sizeof(double)));
+  double* csum = reinterpret_cast<double*>(static_cast<unsigned char*>(ws) + 2 * align256(nb * sizeof(double)));
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_sample_block_sums, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, st, probs, count, nb, bsum);
+  hipLaunchKernelGGL(k_sample_block_sums, dim3((unsigned)((nb + kSumWaves - 1) / kSumWaves)), dim3(64 * kSumWaves), 0, st,
+                     probs, count, nb, bsum, csum);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_sample_block_cdf, dim3(1), dim3(1024), 0, st, bsum, nb, cdf);
   HIP_TRY(hipGetLastError());
   const uint64_t g = ((uint64_t)shots + 3) / 4;
   if (g > 0x7fffffffull) return fail(SLOS_ERANGE, "too many shots for one launch");
-  hipLaunchKernelGGL(k_sample_search, dim3((unsigned)g), dim3(256), 0, st, probs, count, nb, cdf, uniforms, shots, idx);
+  hipLaunchKernelGGL(k_sample_search, dim3((unsigned)g), dim3(256), 0, st, probs, count, nb, cdf, csum, uniforms, shots,
+                     idx);
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }

@@ -103,11 +103,53 @@ def species_probs(kernel, Ub: torch.Tensor, groups: Sequence[Sequence[int]], sin
     return q
 
 
+def _subset_terms(kernel, Ub: torch.Tensor, cols: Sequence[int], bads: Sequence[Sequence[int]]) -> torch.Tensor:
+    """sum over the subsets of species_probs(interfering = cols minus bad, singles = bad), for
+    subsets that all leave out len(bad) photons: [B, S_n]. The subsets form the batch dimension:
+    each row's U has the interfering columns first, so one forward with modes 0..g-1 covers all of
+    them; the singles then enter by per-row raises (slos_raise takes one weight row per batch row)."""
+    n, B, m = len(cols), Ub.shape[0], Ub.shape[-1]
+    C, nb = len(bads), len(bads[0])
+    dev = Ub.device
+    goods = [[cols[j] for j in range(n) if j not in bad] for bad in bads]
+    singles = [[cols[j] for j in bad] for bad in bads]
+    g = n - nb
+    if g >= 2:
+        gi = torch.as_tensor(goods, dtype=torch.int64, device=dev)                  # [C, g]
+        Up = torch.zeros((C, B, m, m), dtype=torch.complex64, device=dev)
+        Up[..., :g] = Ub[:, :, gi].permute(2, 0, 1, 3)                               # [C, B, m, g]
+        q = kernel._plan.forward(Up.reshape(C * B, m, m).contiguous(), modes=list(range(g)), layer=g,
+                                 out_kind=nat.OUT_PROBS, scale=1.0,
+                                 flags=nat.FLAG_COMPENSATED if kernel.compensated else 0)
+        norms = []
+        for good in goods:
+            counts: dict = {}
+            for c in good:
+                counts[c] = counts.get(c, 0) + 1
+            norms.append(1.0 / math.prod(math.factorial(v) for v in counts.values()))
+        if any(v != 1.0 for v in norms):                                             # 1 / prod s_i!
+            q = q * torch.as_tensor(norms, dtype=torch.float32, device=dev).repeat_interleave(B).unsqueeze(1)
+        k = g
+    else:
+        q = torch.ones((C * B, 1), dtype=torch.float32, device=dev)
+        singles = [sg + gd for sg, gd in zip(singles, goods)]
+        k = 0
+    for t in range(len(singles[0])):
+        ct = torch.as_tensor([sg[t] for sg in singles], dtype=torch.int64, device=dev)  # [C]
+        w = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(C * B, m).to(torch.float32).contiguous()
+        q = kernel._plan.raise_layer(k + 1, w, q.contiguous())
+        k += 1
+    if k != kernel.n:
+        raise ValueError(f"the partition covers {k} photons, the kernel has {kernel.n}")
+    return q.reshape(C, B, -1).sum(0)
+
+
 def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torch.Tensor:
     """The x-model distribution on the final layer ([S] or [B, S], float32): the sum over the
     interfering subsets G of the input photons of x^|G| (1 - x)^(n - |G|) species_probs(G, rest).
     `indist` a float, a Distinguishability(x=...) or a tensor x (differentiable); `order=b` keeps
-    the subsets with at most b distinguishable photons."""
+    the subsets with at most b distinguishable photons. Subsets with the same number of
+    distinguishable photons run together (chunks of at most 2 GB of float32 terms)."""
     d = spec(indist)
     if d.x is None:
         raise NotImplementedError("strong simulation: the x-model only (random species need a partition sum)")
@@ -117,12 +159,14 @@ def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torc
     cols = kernel.input_modes
     n = len(cols)
     b_max = n if order is None else max(0, min(int(order), n))
+    per = max(1, (2 << 30) // (4 * kernel.layers.size(n) * Ub.shape[0]))
     out = None
     for b in range(b_max + 1):
         w = x ** (n - b) * (1 - x) ** b
-        for bad in combinations(range(n), b):
-            good = [cols[j] for j in range(n) if j not in bad]
-            term = species_probs(kernel, Ub, [good] if len(good) >= 2 else [],
-                                 [cols[j] for j in bad] + (good if len(good) == 1 else []))
-            out = term * w if out is None else out + term * w
+        subsets = list(combinations(range(n), b))
+        acc = None
+        for i0 in range(0, len(subsets), per):
+            term = _subset_terms(kernel, Ub, cols, subsets[i0:i0 + per])
+            acc = term if acc is None else acc + term
+        out = acc * w if out is None else out + acc * w
     return kernel._out(out, batched)
