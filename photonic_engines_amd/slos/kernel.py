@@ -316,11 +316,17 @@ class SuperpositionSLOSKernel:
         w = w.to(dev)
         if w.numel() != self.K:
             raise ValueError(f"weights must have K = {self.K} entries")
-        amp = None
-        for k in range(self.K):
-            a = self._plan.forward(Ub, modes=self._modes[k], layer=self.n, out_kind=nat.OUT_AMPS,
-                                   scale=self._norms[k])
-            amp = a * w[k] if amp is None else amp.add_(a * w[k])
+        # all components in one batched forward: row (k, b) holds U_b with component k's input
+        # columns first, so modes 0..n-1 select them; the raw amplitudes then get the component's
+        # input norm and weight and are summed over k
+        K, B, m = self.K, Ub.shape[0], self.m
+        cols = torch.as_tensor(self._modes, dtype=torch.int64, device=dev)            # [K, n]
+        Up = torch.zeros((K, B, m, m), dtype=torch.complex64, device=dev)
+        Up[..., : self.n] = Ub[:, :, cols].permute(2, 0, 1, 3)
+        raw = self._plan.forward(Up.reshape(K * B, m, m).contiguous(), modes=list(range(self.n)), layer=self.n,
+                                 out_kind=nat.OUT_AMPS, scale=1.0)
+        coef = (w * torch.as_tensor(self._norms, dtype=torch.float32, device=dev)).to(torch.complex64)
+        amp = (raw.reshape(K, B, -1) * coef.view(K, 1, 1)).sum(0)
         amp = amp if batched else amp.squeeze(0)
         return amp.to(self.device) if self.device.type != "cuda" else amp
 
