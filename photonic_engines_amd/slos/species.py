@@ -107,7 +107,8 @@ def _subset_terms(kernel, Ub: torch.Tensor, cols: Sequence[int], bads: Sequence[
     """sum over the subsets of species_probs(interfering = cols minus bad, singles = bad), for
     subsets that all leave out len(bad) photons: [B, S_n]. The subsets form the batch dimension:
     each row's U has the interfering columns first, so one forward with modes 0..g-1 covers all of
-    them; the singles then enter by per-row raises (slos_raise takes one weight row per batch row)."""
+    them; the singles then enter by per-row raises (layer steps with the coefficient column
+    |U[j, col]|^2 of the row)."""
     n, B, m = len(cols), Ub.shape[0], Ub.shape[-1]
     C, nb = len(bads), len(bads[0])
     dev = Ub.device
@@ -134,11 +135,18 @@ def _subset_terms(kernel, Ub: torch.Tensor, cols: Sequence[int], bads: Sequence[
         q = torch.ones((C * B, 1), dtype=torch.float32, device=dev)
         singles = [sg + gd for sg, gd in zip(singles, goods)]
         k = 0
-    for t in range(len(singles[0])):
-        ct = torch.as_tensor([sg[t] for sg in singles], dtype=torch.int64, device=dev)  # [C]
-        w = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(C * B, m).to(torch.float32).contiguous()
-        q = kernel._plan.raise_layer(k + 1, w, q.contiguous())
-        k += 1
+    if singles[0]:
+        # The raises are the layer recurrence with a real coefficient column |U[j, col]|^2 on real
+        # data: run them as complex layer steps (imaginary parts stay exactly zero, the real parts are
+        # the same fused multiply-adds) so that the block kernel carries them.
+        qc = torch.complex(q, torch.zeros_like(q))
+        Uw = torch.zeros((C * B, m, m), dtype=torch.complex64, device=dev)
+        for t in range(len(singles[0])):
+            ct = torch.as_tensor([sg[t] for sg in singles], dtype=torch.int64, device=dev)  # [C]
+            Uw[:, :, 0] = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(C * B, m).to(torch.complex64)
+            qc = kernel._plan.layer_step(Uw, k + 1, 0, qc.contiguous(), nat.OUT_RAW)
+            k += 1
+        q = qc.real
     if k != kernel.n:
         raise ValueError(f"the partition covers {k} photons, the kernel has {kernel.n}")
     return q.reshape(C, B, -1).sum(0)

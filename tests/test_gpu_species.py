@@ -81,3 +81,25 @@ def test_u_gradient_is_refused_on_this_path():
     U = haar_unitary(5, 1).to(torch.complex64).requires_grad_(True)
     with pytest.raises(NotImplementedError):
         k.probs(U, indist=0.5)
+
+
+@pytest.mark.parametrize("n,m,mask,occ", [(6, 9, "fock", None), (7, 6, "fock", [2, 0, 1, 3, 1, 0]),
+                                          (6, 12, "unbunched", None), (6, 8, "tail_empty", None)])
+def test_raise_as_complex_layer_step_equals_raise_kernel(n, m, mask, occ):
+    """The x-model's raise (a layer step whose U column is |U[j, col]|^2, on real data) equals the
+    probability-level raise kernel slos_raise on every layer, per batch row."""
+    from helpers import occupation_list
+    from photonic_engines_amd import _native as nat
+    occ = occ or occupation_list(n, m, mask)
+    k = FockSLOSKernel(n, m, occ, mask=mask, device=DEV)
+    plan = k._plan
+    g = torch.Generator(device=DEV).manual_seed(n + m)
+    for kk in range(1, n + 1):
+        q = torch.rand((3, plan.sizes[kk - 1]), generator=g, device=DEV)
+        w = torch.rand((3, m), generator=g, device=DEV)
+        a = plan.raise_layer(kk, w, q)
+        Uw = torch.zeros((3, m, m), dtype=torch.complex64, device=DEV)
+        Uw[:, :, 0] = w.to(torch.complex64)
+        b = plan.layer_step(Uw, kk, 0, torch.complex(q, torch.zeros_like(q)), nat.OUT_RAW)
+        assert torch.all(b.imag == 0)
+        assert torch.allclose(b.real, a, rtol=2e-6, atol=0), kk
