@@ -335,7 +335,8 @@ class SuperpositionSLOSKernel:
         amp = self.amplitudes(U, weights=weights, chunk_size=chunk_size)
         p = amp.real ** 2 + amp.imag ** 2
         if not self._output_norm_is_one:
-            if self._prob_norm is None:
-                self._prob_norm = torch.as_tensor(self.layers.output_prob_norms()).to(dtype=torch.float32)
-            p = p * self._prob_norm.to(p.device)
+            if self._prob_norm is None or self._prob_norm.device != p.device:  # prod t_j!, kept where p lives
+                self._prob_norm = torch.as_tensor(self.layers.output_prob_norms()).to(device=p.device,
+                                                                                      dtype=torch.float32)
+            p = p * self._prob_norm
         return _maybe_thin(self.layers, p, loss)
