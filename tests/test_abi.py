@@ -112,3 +112,13 @@ def test_group_host_checks():
     assert L.slos_group_destroy(g) == nat.SLOS_OK
     for p in plans:
         assert L.slos_plan_destroy(p) == nat.SLOS_OK
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with status 2 and says so."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 2 and "one GPU" in r.stderr
