@@ -2,7 +2,7 @@
 // (the plan builds the suffix-tuple table with it; the CPU tests execute it unchanged).
 //
 // A child state of layer K is c = (Q, tau): Q = its first d = K - SUF sorted positions, tau = the
-// last SUF positions, all >= lo = lo[last(Q)] (lo = 0 when d = 0). The children of one Q are
+// last SUF positions (SUF = 4, or 5 for the big layers), all >= lo = lo[last(Q)] (lo = 0 when d = 0). The children of one Q are
 // contiguous (the "block" of Q, size cg[SUF][lo]), and tau runs over the SUF-tuples >= lo in
 // lex order, which is the tail of the lex order of all SUF-tuples of the space. So with u the
 // position of c inside its block:
@@ -11,7 +11,8 @@
 //  * dropping the suffix photon x_j gives (Q, tau - x_j) inside the block of Q in layer K-1
 //    (suffix length SUF-1): parent = LBp + lr_j, lr_j = global (SUF-1)-tuple index of tau - x_j.
 //    With cg[r][x] - cg[r-1][lo[x]] = cg[r][x+1] (the counting recurrence), the global index of a
-//    3-tuple (a, b, c) is cg[3][0] - cg[3][a+1] - cg[2][b+1] - cg[1][c]: three per-position terms.
+//    3-tuple (a, b, c) is cg[3][0] - cg[3][a+1] - cg[2][b+1] - cg[1][c]: three per-position terms
+//    (a 4-tuple (a, b, c, e): cg[4][0] - cg[4][a+1] - cg[3][b+1] - cg[2][c+1] - cg[1][e]).
 // Ranks use the counting table of space.hpp: with lo_0 = 0, lo_{i+1} = lo[q_i],
 //   T_i = cg[K-i][lo_i] - cg[K-i][q_i]        (child terms, photon count K)
 //   A_i = cg[K-1-i][lo_i] - cg[K-1-i][q_i]    (same position, one photon fewer)
@@ -26,28 +27,64 @@
 
 namespace slos {
 
-constexpr int kSuf = 4;
+constexpr int kSuf = 4;     // suffix length of the loss descent and the uneven-batch launches
+constexpr int kSufMax = 5;  // the forward layer kernel also runs with 5-position suffixes
 constexpr int kNoEdge = 255;
 
-// Suffix-tuple word (4 B) of a SUF-tuple tau, in lex order over all SUF-tuples: byte j is
+// Suffix-tuple word of a SUF-tuple tau, in lex order over all SUF-tuples: byte j is
 // x_j | mark_j << 6, where mark_j = 1 when x_j repeats x_{j-1} (a multiset's later copy of a mode,
-// which is not an edge). Positions are < 64 (kBlockModes, checked at plan creation). A byte is
-// also the row of the per-launch suffix table (128 rows): rows x hold U[x, col] and the rank terms
-// of position x, rows x | 64 the same rank terms with a zero coefficient.
-typedef uint32_t TupleWord;
+// which is not an edge); 4 B for SUF = 4, 8 B (bytes 5..7 zero) for SUF = 5. Positions are < 64
+// (kBlockModes, checked at plan creation). A byte is also the row of the per-launch suffix table
+// (128 rows): rows x hold U[x, col] and the rank terms of position x, rows x | 64 the same rank
+// terms with a zero coefficient.
+template <int SUF>
+struct SufWord;
+template <>
+struct SufWord<4> {
+  typedef uint32_t type;
+};
+template <>
+struct SufWord<5> {
+  typedef uint64_t type;
+};
+template <int SUF>
+using TupleWordT = typename SufWord<SUF>::type;
+typedef TupleWordT<4> TupleWord;
 constexpr int kBlockModes = 64;
 constexpr uint32_t kTupleMark = 64;
 constexpr int kSufRows = 128;
 constexpr int kNoEdgeRow = 127;  // a zero-coefficient row: inactive prefix-stream slots
 
-// the marks of x1, x2, x3 as a 3-bit code (bit j-1: x_j repeats x_{j-1})
-SLOS_HD uint32_t tuple_code(TupleWord w) { return ((w >> 14) & 1u) | ((w >> 21) & 2u) | ((w >> 28) & 4u); }
+// byte j of a tuple word
+template <class W>
+SLOS_HD uint32_t tw_byte(W w, int j) {
+  return (uint32_t)(w >> (8 * j)) & 0xffu;
+}
+// the marks of x_1 .. x_{SUF-1} as a (SUF-1)-bit code (bit j-1: x_j repeats x_{j-1})
+template <int SUF = kSuf>
+SLOS_HD uint32_t tuple_code(TupleWordT<SUF> w) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 1; j < SUF; ++j) c |= ((tw_byte(w, j) >> 6) & 1u) << (j - 1);
+  return c;
+}
 // copies of x0 in tau
-SLOS_HD int code_lead(uint32_t c) { return 1 + (int)(c & 1u) + ((c & 3u) == 3u) + (c == 7u); }
+template <int SUF = kSuf>
+SLOS_HD int code_lead(uint32_t c) {
+  int lead = 1;
+  bool run = true;
+#pragma unroll
+  for (int j = 0; j < SUF - 1; ++j) {
+    run = run && ((c >> j) & 1u);
+    lead += run ? 1 : 0;
+  }
+  return lead;
+}
 // prod of the run factorials of tau
+template <int SUF = kSuf>
 SLOS_HD int code_fct(uint32_t c) {
   int f = 1, run = 1;
-  for (int j = 0; j < kSuf - 1; ++j) {
+  for (int j = 0; j < SUF - 1; ++j) {
     run = ((c >> j) & 1u) ? run + 1 : 1;
     f *= run;
   }
@@ -67,10 +104,25 @@ SLOS_HD void suffix_parent_ranks(uint32_t C0, const uint32_t (&a3)[kSuf], const 
   lr[3] = t01 - b1[2];       // (x0, x1, x2)
 }
 
+// The same for SUF = 5: global 4-tuple indices of tau without x_j, j = 0..4, from
+// t4[j] = cg[4][x_j + 1], t3[j] = cg[3][x_j + 1], t2[j] = cg[2][x_j + 1], t1[j] = cg[1][x_j], C0 = cg[4][0].
+SLOS_HD void suffix_parent_ranks5(uint32_t C0, const uint32_t (&t4)[5], const uint32_t (&t3)[5], const uint32_t (&t2)[5],
+                                  const uint32_t (&t1)[5], uint32_t (&lr)[5]) {
+  const uint32_t s34 = t2[3] + t1[4];
+  const uint32_t c0 = C0 - t4[0];
+  const uint32_t c01 = c0 - t3[1];
+  const uint32_t c012 = c01 - t2[2];
+  lr[0] = C0 - t4[1] - t3[2] - s34;  // (x1, x2, x3, x4)
+  lr[1] = c0 - t3[2] - s34;          // (x0, x2, x3, x4)
+  lr[2] = c01 - s34;                 // (x0, x1, x3, x4)
+  lr[3] = c012 - t1[4];              // (x0, x1, x2, x4)
+  lr[4] = c012 - t1[3];              // (x0, x1, x2, x3)
+}
+
 struct BlockTables {
   const uint64_t* cg;        // (n + 2) x (me + 1)
   const int* lo;             // me
-  const TupleWord* tuples;   // cg[SUF][0] entries
+  const void* tuples;        // cg[SUF][0] tuple words of the launch's SUF
   int me, stride, multiset;
 };
 
@@ -95,8 +147,9 @@ struct Streams {
   bool active[MAXD];
 };
 
-template <int MAXD>
+template <int MAXD, int SUF = kSuf>
 SLOS_HD void block_info(const BlockTables& t, int K, int d, const int* q, BlockInfo& bi, Streams<MAXD>& st) {
+  constexpr int kSuf = SUF;  // the suffix length of this launch (shadows the default)
   const uint64_t* cg = t.cg;
   const int W = t.stride;
 #define SLOS_C(r, x) cg[(r) * W + (x)]
@@ -235,13 +288,14 @@ SLOS_HD uint64_t unrank_prefix(const BlockTables& t, int K, int d, uint64_t g, i
 }
 
 // the tuple word of one SUF-tuple (host side, at plan creation)
-inline TupleWord make_tuple_word(const SpaceView& sv, int multiset, uint64_t u) {
-  int x[kSuf];
-  sv.unrank(u, kSuf, x);
-  TupleWord w = 0;
-  for (int j = 0; j < kSuf; ++j) {
+template <int SUF = kSuf>
+inline TupleWordT<SUF> make_tuple_word(const SpaceView& sv, int multiset, uint64_t u) {
+  int x[SUF];
+  sv.unrank(u, SUF, x);
+  TupleWordT<SUF> w = 0;
+  for (int j = 0; j < SUF; ++j) {
     const bool mark = multiset && j > 0 && x[j] == x[j - 1];
-    w |= ((uint32_t)x[j] | (mark ? kTupleMark : 0u)) << (8 * j);
+    w |= (TupleWordT<SUF>)((uint32_t)x[j] | (mark ? kTupleMark : 0u)) << (8 * j);
   }
   return w;
 }
@@ -366,25 +420,27 @@ SLOS_HD int desc_tail_copies(const int (&s)[kSuf], int a, int lastP, int lastrun
   return 1 + (a == lastP ? lastrun : 0) + (s[0] == a) + (s[1] == a) + (s[2] == a) + (s[3] == a);
 }
 
-static_assert(kSuf <= 4, "lead_binom covers runs of up to 4 copies");
+static_assert(kSufMax <= 5, "lead_binom covers runs of up to 5 copies");
 
-// binom(r + lead, lead) for lead <= kSuf, exact in integers (each step divides exactly)
+// binom(r + lead, lead) for lead <= 5, exact in integers (each step divides exactly)
 SLOS_HD float lead_binom(int r, int lead) {
   uint32_t p = (uint32_t)r + 1u;
   const uint32_t p2 = p * ((uint32_t)r + 2u) / 2u;
   const uint32_t p3 = p2 * ((uint32_t)r + 3u) / 3u;
   const uint32_t p4 = p3 * ((uint32_t)r + 4u) / 4u;
-  p = lead >= 4 ? p4 : lead == 3 ? p3 : lead == 2 ? p2 : lead == 1 ? p : 1u;
+  const uint32_t p5 = p4 * ((uint32_t)r + 5u) / 5u;
+  p = lead >= 5 ? p5 : lead == 4 ? p4 : lead == 3 ? p3 : lead == 2 ? p2 : lead == 1 ? p : 1u;
   return (float)p;
 }
 
 // prod t_j! of the child (Q, tau) from the block's run data and the tuple word
-SLOS_HD float child_norm(const BlockInfo& bi, TupleWord w, int multiset) {
+template <int SUF = kSuf>
+SLOS_HD float child_norm(const BlockInfo& bi, TupleWordT<SUF> w, int multiset) {
   if (!multiset) return 1.f;
-  const uint32_t c = tuple_code(w);
-  float f = bi.fQ * (float)code_fct(c);
+  const uint32_t c = tuple_code<SUF>(w);
+  float f = bi.fQ * (float)code_fct<SUF>(c);
   // tau's leading run continues the last run of Q: (r + c)! = r! c! binom(r + c, c)
-  if (bi.skip0 && (int)(w & 0xffu) == bi.lo) f *= lead_binom(bi.lastrun, code_lead(c));
+  if (bi.skip0 && (int)tw_byte(w, 0) == bi.lo) f *= lead_binom(bi.lastrun, code_lead<SUF>(c));
   return f;
 }
 

@@ -252,7 +252,7 @@ __device__ __forceinline__ void descend_wave(const BlockTables& tg, const BlockT
       }
       const uint64_t avail = g_end - g;
       const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
-      DescCtx c{qk, qout, tg.tuples + toff, cgl, CBp, CB0, sb, scnt, tA0, tA1, tA2, tA3, u_stop, W, me, lastP, lastrun,
+      DescCtx c{qk, qout, static_cast<const TupleWord*>(tg.tuples) + toff, cgl, CBp, CB0, sb, scnt, tA0, tA1, tA2, tA3, u_stop, W, me, lastP, lastrun,
                 lane, invk, scale};
       uint32_t uc = (uint32_t)u0;
       for (; uc + 128u <= u_stop; uc += 128u) descend_step2<ME>(c, uc);

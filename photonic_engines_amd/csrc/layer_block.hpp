@@ -80,14 +80,17 @@ __device__ __forceinline__ void bk_cmac(BlockAcc& a, float2 u, float2 v) {
 }
 
 // One row of the per-launch suffix table (LDS, kSufRows rows, row = tuple-word byte): the U
-// coefficient of the position (zero for marked rows, x | 64) and its rank terms cg[3][x+1], cg[2][x+1]
-// and cg[1][x] (the last two packed as 16-bit halves).
+// coefficient of the position (zero for marked rows, x | 64) and its rank terms, packed:
+//   SUF 4: t0 = cg[3][x+1], t1 = cg[2][x+1] | cg[1][x] << 16;
+//   SUF 5: t0 = cg[4][x+1] | cg[1][x] << 24, t1 = cg[3][x+1] | cg[2][x+1] << 16
+// (the plan checks that every term fits its field).
 struct SufRow {
   float re, im;
-  uint32_t a3, b21;
+  uint32_t t0, t1;
 };
 
 // rows for U[b, :, col] (Ucol = &U[b][0][col], stride m), positions < me; cg: the global table
+template <int SUF>
 __device__ __forceinline__ void fill_suffix_rows(SufRow* rows, const float2* Ucol, int m, const uint64_t* cg, int stride,
                                                  int me) {
   for (int i = threadIdx.x; i < kSufRows; i += blockDim.x) {
@@ -96,36 +99,47 @@ __device__ __forceinline__ void fill_suffix_rows(SufRow* rows, const float2* Uco
     SufRow r;
     r.re = u.x;
     r.im = u.y;
-    r.a3 = x < me ? (uint32_t)cg[3 * stride + x + 1] : 0u;
-    r.b21 = x < me ? ((uint32_t)cg[2 * stride + x + 1] | ((uint32_t)cg[stride + x] << 16)) : 0u;
+    r.t0 = r.t1 = 0u;
+    if (x < me) {
+      if constexpr (SUF == 4) {
+        r.t0 = (uint32_t)cg[3 * stride + x + 1];
+        r.t1 = (uint32_t)cg[2 * stride + x + 1] | ((uint32_t)cg[stride + x] << 16);
+      } else {
+        r.t0 = (uint32_t)cg[4 * stride + x + 1] | ((uint32_t)cg[stride + x] << 24);
+        r.t1 = (uint32_t)cg[3 * stride + x + 1] | ((uint32_t)cg[2 * stride + x + 1] << 16);
+      }
+    }
     rows[i] = r;
   }
 }
 
-// Readout norms by (row, code): row 0 is prod of tau's run factorials for the 3-bit mark code; row
-// 1 + r multiplies it by binom(r + lead, lead) (tau's leading run joins a last prefix run of r
-// copies). kNormRows covers r <= 16, the widest prefix of the block kernel.
+// Readout norms by (row, code): row 0 is prod of tau's run factorials for the (SUF-1)-bit mark
+// code; row 1 + r multiplies it by binom(r + lead, lead) (tau's leading run joins a last prefix
+// run of r copies). kNormRows covers r <= 16, the widest prefix of the block kernel.
 constexpr int kNormRows = 18;
+constexpr int kNormCodes = 1 << (kSufMax - 1);
+template <int SUF>
 __device__ __forceinline__ void fill_norm_rows(float* rn) {
-  for (int i = threadIdx.x; i < kNormRows * 8; i += blockDim.x) {
-    const int row = i >> 3;
-    const uint32_t c = (uint32_t)(i & 7);
-    const float f = (float)code_fct(c);
-    rn[i] = row == 0 ? f : f * lead_binom(row - 1, code_lead(c));
+  constexpr int NC = 1 << (SUF - 1);
+  for (int i = threadIdx.x; i < kNormRows * NC; i += blockDim.x) {
+    const int row = i / NC;
+    const uint32_t c = (uint32_t)(i % NC);
+    const float f = (float)code_fct<SUF>(c);
+    rn[i] = row == 0 ? f : f * lead_binom(row - 1, code_lead<SUF>(c));
   }
 }
 
 // per-block state of the chunk sweep (all wave-uniform except `lane`)
-template <int MAXD>
+template <int MAXD, int SUF>
 struct ChunkCtx {
   const float2* const (&sp)[MAXD];  // prefix-stream bases (parent of child u at sp[s] + u)
   const float2 (&coef)[MAXD];       // their U coefficients (zero for inactive slots)
   const float2* lp;                 // local parents: lp + lr_j
-  const TupleWord* tb;              // tuple words of the block's children
+  const TupleWordT<SUF>* tb;        // tuple words of the block's children
   char* ob;                         // output of the block's first child
   const SufRow* rows;               // suffix table (LDS)
   const float* rn;                  // readout norms (LDS)
-  uint32_t C0;                      // cg[3][0]
+  uint32_t C0;                      // cg[SUF-1][0]
   uint32_t u_stop;
   int lane;
   float fQ;
@@ -137,36 +151,52 @@ struct ChunkCtx {
 
 // the suffix edges of one child from its tuple word: coefficients c[j] (edge 0 of a tau that
 // continues the prefix's last run is no edge: zero) and local-parent byte offsets off[j]
-template <int MAXD>
-__device__ __forceinline__ void suffix_edges(const ChunkCtx<MAXD>& cx, TupleWord w, float2 (&c)[kSuf],
-                                             uint32_t (&off)[kSuf]) {
-  uint32_t a3[kSuf], b2[kSuf], b1[kSuf], lr[kSuf];
+template <int MAXD, int SUF>
+__device__ __forceinline__ void suffix_edges(const ChunkCtx<MAXD, SUF>& cx, TupleWordT<SUF> w, float2 (&c)[SUF],
+                                             uint32_t (&off)[SUF]) {
+  uint32_t lr[SUF];
+  if constexpr (SUF == 4) {
+    uint32_t a3[4], b2[4], b1[4];
 #pragma unroll
-  for (int j = 0; j < kSuf; ++j) {
-    const SufRow r = cx.rows[(w >> (8 * j)) & 0xffu];
-    c[j] = make_float2(r.re, r.im);
-    a3[j] = r.a3;
-    b2[j] = r.b21 & 0xffffu;
-    b1[j] = r.b21 >> 16;
+    for (int j = 0; j < 4; ++j) {
+      const SufRow r = cx.rows[tw_byte(w, j)];
+      c[j] = make_float2(r.re, r.im);
+      a3[j] = r.t0;
+      b2[j] = r.t1 & 0xffffu;
+      b1[j] = r.t1 >> 16;
+    }
+    suffix_parent_ranks(cx.C0, a3, b2, b1, lr);
+  } else {
+    const uint32_t lo32 = (uint32_t)w, hi32 = (uint32_t)(w >> 32);
+    uint32_t t4[5], t3[5], t2[5], t1[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const SufRow r = cx.rows[j < 4 ? (lo32 >> (8 * j)) & 0xffu : hi32 & 0xffu];
+      c[j] = make_float2(r.re, r.im);
+      t4[j] = r.t0 & 0xffffffu;
+      t1[j] = r.t0 >> 24;
+      t3[j] = r.t1 & 0xffffu;
+      t2[j] = r.t1 >> 16;
+    }
+    suffix_parent_ranks5(cx.C0, t4, t3, t2, t1, lr);
   }
-  if (cx.skip0 && (int)(w & 0xffu) == cx.lo) c[0] = make_float2(0.f, 0.f);
-  suffix_parent_ranks(cx.C0, a3, b2, b1, lr);
+  if (cx.skip0 && (int)((uint32_t)w & 0xffu) == cx.lo) c[0] = make_float2(0.f, 0.f);
 #pragma unroll
-  for (int j = 0; j < kSuf; ++j) off[j] = lr[j] * 8u;
+  for (int j = 0; j < SUF; ++j) off[j] = lr[j] * 8u;
 }
 
 // prod t_j! of a child from its tuple word and the block's prefix runs
-template <int MAXD>
-__device__ __forceinline__ float child_norm_w(const ChunkCtx<MAXD>& cx, TupleWord w) {
+template <int MAXD, int SUF>
+__device__ __forceinline__ float child_norm_w(const ChunkCtx<MAXD, SUF>& cx, TupleWordT<SUF> w) {
   if (!cx.multiset) return 1.f;
-  const uint32_t row = (cx.skip0 && (int)(w & 0xffu) == cx.lo) ? (uint32_t)cx.lastrun + 1u : 0u;
-  return cx.fQ * cx.rn[row * 8u + tuple_code(w)];
+  const uint32_t row = (cx.skip0 && (int)((uint32_t)w & 0xffu) == cx.lo) ? (uint32_t)cx.lastrun + 1u : 0u;
+  return cx.fQ * cx.rn[row * (1u << (SUF - 1)) + tuple_code<SUF>(w)];
 }
 
 // NC consecutive 64-child chunks starting at uc: every load of the NC chunks is issued before any
 // result is needed. Lanes past u_stop re-read the last valid child and store nothing.
-template <int KIND, bool COMP, int MAXD, int NC>
-__device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
+template <int KIND, bool COMP, int MAXD, int NC, int SUF>
+__device__ __forceinline__ void sweep(const ChunkCtx<MAXD, SUF>& cx, uint32_t uc) {
   uint32_t u[NC];
   bool in[NC];
 #pragma unroll
@@ -175,21 +205,21 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
     in[c] = ur < cx.u_stop;
     u[c] = in[c] ? ur : cx.u_stop - 1u;
   }
-  TupleWord tw[NC];
+  TupleWordT<SUF> tw[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) tw[c] = at_bytes(cx.tb, u[c] * 4u);
+  for (int c = 0; c < NC; ++c) tw[c] = at_bytes(cx.tb, u[c] * (uint32_t)sizeof(TupleWordT<SUF>));
   float2 sv[NC][MAXD];
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
     for (int s = 0; s < MAXD; ++s) sv[c][s] = at_bytes(cx.sp[s], u[c] * 8u);
-  float2 lc[NC][kSuf], lv[NC][kSuf];
+  float2 lc[NC][SUF], lv[NC][SUF];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    uint32_t off[kSuf];
+    uint32_t off[SUF];
     suffix_edges(cx, tw[c], lc[c], off);
 #pragma unroll
-    for (int jj = 0; jj < kSuf; ++jj) lv[c][jj] = at_bytes(cx.lp, off[jj]);
+    for (int jj = 0; jj < SUF; ++jj) lv[c][jj] = at_bytes(cx.lp, off[jj]);
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -197,7 +227,7 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD>& cx, uint32_t uc) {
 #pragma unroll
     for (int s = 0; s < MAXD; ++s) bk_cmac<COMP>(acc, cx.coef[s], sv[c][s]);
 #pragma unroll
-    for (int jj = 0; jj < kSuf; ++jj) bk_cmac<COMP>(acc, lc[c][jj], lv[c][jj]);
+    for (int jj = 0; jj < SUF; ++jj) bk_cmac<COMP>(acc, lc[c][jj], lv[c][jj]);
     float re = acc.re + acc.cre, im = acc.im + acc.cim;
     if (KIND == 0 || (KIND == 3 && cx.kind == 0)) {
       if (in[c]) *reinterpret_cast<float2*>(cx.ob + u[c] * 8u) = make_float2(re, im);
@@ -222,11 +252,16 @@ __device__ __forceinline__ float4 load_pair(const float2* base, uint32_t off_byt
   return v;
 }
 
-// the tuple words of lane i's two adjacent children uc + 2i and uc + 2i + 1 (one 8-byte load)
-template <int MAXD>
-__device__ __forceinline__ uint2 load_tuple_pair(const ChunkCtx<MAXD>& cx, uint32_t uc) {
-  uint2 w;
-  __builtin_memcpy(&w, reinterpret_cast<const char*>(cx.tb) + (uc + 2u * (uint32_t)cx.lane) * 4u, 8);
+// the tuple words of lane i's two adjacent children uc + 2i and uc + 2i + 1 (one 8- or 16-byte load)
+template <int SUF>
+struct TuplePair {
+  TupleWordT<SUF> a, b;
+};
+template <int MAXD, int SUF>
+__device__ __forceinline__ TuplePair<SUF> load_tuple_pair(const ChunkCtx<MAXD, SUF>& cx, uint32_t uc) {
+  TuplePair<SUF> w;
+  __builtin_memcpy(&w, reinterpret_cast<const char*>(cx.tb) + (uc + 2u * (uint32_t)cx.lane) * sizeof(TupleWordT<SUF>),
+                   sizeof(w));
   return w;
 }
 
@@ -235,20 +270,20 @@ __device__ __forceinline__ uint2 load_tuple_pair(const ChunkCtx<MAXD>& cx, uint3
 // the store is one 8- (probabilities) or 16-byte (amplitudes) store per lane. `tn` holds this
 // step's tuple words on entry; if `next`, the next step's (uc + 128, inside the block) are loaded
 // into it behind this step's loads, so the suffix-parent loads of a step never wait for a tuple load.
-template <int KIND, bool COMP, int MAXD>
-__device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t uc, uint2& tn, bool next) {
+template <int KIND, bool COMP, int MAXD, int SUF>
+__device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD, SUF>& cx, uint32_t uc, TuplePair<SUF>& tn, bool next) {
   const uint32_t u = uc + 2u * (uint32_t)cx.lane;
-  const TupleWord tw[2] = {tn.x, tn.y};
+  const TupleWordT<SUF> tw[2] = {tn.a, tn.b};
   float4 sv[MAXD];
 #pragma unroll
   for (int s = 0; s < MAXD; ++s) sv[s] = load_pair(cx.sp[s], u * 8u);
-  float2 lc[2][kSuf], lv[2][kSuf];
+  float2 lc[2][SUF], lv[2][SUF];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    uint32_t off[kSuf];
+    uint32_t off[SUF];
     suffix_edges(cx, tw[c], lc[c], off);
 #pragma unroll
-    for (int jj = 0; jj < kSuf; ++jj) lv[c][jj] = at_bytes(cx.lp, off[jj]);
+    for (int jj = 0; jj < SUF; ++jj) lv[c][jj] = at_bytes(cx.lp, off[jj]);
   }
   if (next) tn = load_tuple_pair(cx, uc + 128u);
   float res[2][2];
@@ -260,7 +295,7 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t u
     for (int s = 0; s < MAXD; ++s)
       bk_cmac<COMP>(acc, cx.coef[s], c == 0 ? make_float2(sv[s].x, sv[s].y) : make_float2(sv[s].z, sv[s].w));
 #pragma unroll
-    for (int jj = 0; jj < kSuf; ++jj) bk_cmac<COMP>(acc, lc[c][jj], lv[c][jj]);
+    for (int jj = 0; jj < SUF; ++jj) bk_cmac<COMP>(acc, lc[c][jj], lv[c][jj]);
     res[c][0] = acc.re + acc.cre;
     res[c][1] = acc.im + acc.cim;
     if (!(KIND == 0 || (KIND == 3 && cx.kind == 0))) f[c] = child_norm_w(cx, tw[c]);
@@ -285,7 +320,7 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD>& cx, uint32_t u
 // Dynamic LDS of the block kernels: counting-table rows 0..K and lo[] (per-lane block data is
 // computed from them); the suffix table and the readout norms are static.
 __host__ __device__ inline size_t block_lds_bytes(int K, int me) { return (size_t)(K + 1) * (me + 1) * 8 + (size_t)me * 4; }
-constexpr size_t kBlockStaticLds = kSufRows * sizeof(SufRow) + kNormRows * 8 * sizeof(float);  // rows, rn
+constexpr size_t kBlockStaticLds = kSufRows * sizeof(SufRow) + kNormRows * kNormCodes * sizeof(float);  // rows, rn
 // Per-wave block table in LDS: the data of kBlockBatch consecutive blocks, one column per block,
 // computed by all lanes at once (lane i: block r + i) and then read back one block at a time with
 // wave-uniform LDS reads, so none of it occupies vector registers during the child sweeps. Words:
@@ -304,13 +339,64 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// the wave-uniform data of one block's sweep (see block_children)
+struct BlockSweep {
+  const float2* lp;     // local parents: lp + lr_j
+  const void* tb;       // tuple words of the block's children
+  char* ob;             // output of the block's first child
+  const SufRow* rows;
+  const float* rn;
+  uint32_t C0, u0, u_stop;
+  int lane;
+  float fQ;
+  int lastrun, skip0, lo, multiset;
+  float scale;
+  unsigned flags;
+  int kind;
+};
+
+// Children [u0, u_stop) of one block with its first NS stream slots of the block table column `bj`
+// (slots past the block's active count have zero coefficients).
+template <int KIND, bool COMP, int MAXD, int NS, int SUF>
+__device__ __forceinline__ void block_children(const uint32_t* bj, const float2* par, const BlockSweep& bs) {
+  constexpr int kBaseW = 8 + (MAXD + 3) / 4;
+  auto word = [&](int f) { return rfl(bj[f * kBlockBatch]); };
+  float2 coef[NS];
+  const float2* sp[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const uint32_t md = (word(8 + s / 4) >> (8 * (s % 4))) & 0xffu;
+    const SufRow rw = bs.rows[md];
+    coef[s] = uniform_f2(make_float2(rw.re, rw.im));
+    sp[s] = par + ((uint64_t)word(kBaseW + 2 * s) | ((uint64_t)word(kBaseW + 2 * s + 1) << 32));
+  }
+  ChunkCtx<NS, SUF> cx{sp,       coef,    bs.lp,       static_cast<const TupleWordT<SUF>*>(bs.tb),
+                       bs.ob,    bs.rows, bs.rn,       bs.C0,    bs.u_stop, bs.lane, bs.fQ, bs.lastrun,
+                       bs.skip0, bs.lo,   bs.multiset, bs.scale, bs.flags,  bs.kind};
+  const uint32_t u_stop = bs.u_stop;
+  uint32_t uc = bs.u0;
+  if constexpr (!COMP) {
+    // full groups of 128 children as adjacent pairs per lane, the rest 64 children per step
+    if (uc + 128u <= u_stop) {
+      TuplePair<SUF> tn = load_tuple_pair(cx, uc);
+      for (; uc + 128u <= u_stop; uc += 128u) sweep_pairs<KIND, COMP, NS, SUF>(cx, uc, tn, uc + 256u <= u_stop);
+    }
+    if (uc + 64u < u_stop) sweep<KIND, COMP, NS, 2, SUF>(cx, uc);
+    else if (uc < u_stop) sweep<KIND, COMP, NS, 1, SUF>(cx, uc);
+  } else {
+    // the compensated sums keep one child per lane (register pressure)
+    for (; uc + 64u < u_stop; uc += 128u) sweep<KIND, COMP, NS, 2, SUF>(cx, uc);
+    if (uc < u_stop) sweep<KIND, COMP, NS, 1, SUF>(cx, uc);
+  }
+}
+
 // One wave's share of a layer slice: the children [g, g + a.per_wave) of one instance, whose parent
 // layer is `par` and whose output layer starts at `out`. `rows` is the suffix table and `rn` the
 // readout norms (LDS), `tl` the counting tables in LDS, `tg` the same tables in global memory (for
 // the wave-uniform scalar unrank of g), `tbl` this wave's block table (kBlockBatch *
 // block_words(MAXD) words of LDS).
 // KIND 3 decides between probabilities (kind 1) and raw amplitudes (kind 0) at run time.
-template <int KIND, bool COMP, int MAXD>
+template <int KIND, bool COMP, int MAXD, int SUF>
 __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTables& tl, const float2* __restrict__ par,
                                            void* __restrict__ out, const SufRow* rows, const float* rn,
                                            uint32_t* tbl, const BlockLayer& a, uint64_t g, int kind = 0) {
@@ -321,11 +407,11 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
   const int me = tg.me, stride = tg.stride, multiset = tg.multiset;
   const uint64_t* cg_l = tl.cg;
   const int* lo_l = tl.lo;
-  const TupleWord* tuples = tg.tuples;
-  const uint32_t C0 = (uint32_t)tg.cg[3 * stride];
+  const TupleWordT<SUF>* tuples = static_cast<const TupleWordT<SUF>*>(tg.tuples);
+  const uint32_t C0 = (uint32_t)tg.cg[(SUF - 1) * stride];
   if (g >= a.S) return;
   const uint64_t g_end = (g + a.per_wave < a.S) ? g + a.per_wave : a.S;
-  const int d = K - kSuf;
+  const int d = K - SUF;
 
   // the block holding g: its prefix rank r and g's offset inside it (scalar path)
   uint64_t r, u0;
@@ -345,7 +431,7 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
       const SpaceView sv{cg_l, lo_l, me, stride};
       if (ri < a.S_pre) {
         sv.unrank(ri, d, q);
-        block_info<MAXD>(tl, K, d, q, bi, st);
+        block_info<MAXD, SUF>(tl, K, d, q, bi, st);
       } else {
         bi.L = 0;
         bi.CB = bi.LBp = 0;
@@ -369,26 +455,26 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
       col[3 * kBlockBatch] = (uint32_t)(bi.LBp >> 32);
       col[4 * kBlockBatch] = bi.L;
       col[5 * kBlockBatch] = bi.toff;
-      col[6 * kBlockBatch] = (uint32_t)bi.lo | ((uint32_t)bi.skip0 << 8) | ((uint32_t)bi.lastrun << 16);
-      col[7 * kBlockBatch] = __builtin_bit_cast(uint32_t, bi.fQ);
+      // active slots compacted to the front in ascending prefix position (the summation order of
+      // the thread-per-state kernel); slots past the active count carry slot 0's base and no edge
 #pragma unroll
-      for (int w = 0; w < kModeW; ++w) {
-        uint32_t word = 0;
+      for (int w = 0; w < kModeW; ++w) col[(8 + w) * kBlockBatch] = kNoEdgeRow * 0x01010101u;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int s = 4 * w + k;
-          const uint32_t md = (s < MAXD && st.active[s < MAXD ? s : 0]) ? (uint32_t)st.mode[s < MAXD ? s : 0]
-                                                                        : (uint32_t)kNoEdgeRow;
-          word |= (md & 0xffu) << (8 * k);
+      for (int t = 0; t < MAXD; ++t) {
+        col[(kBaseW + 2 * t) * kBlockBatch] = (uint32_t)st.base[0];
+        col[(kBaseW + 2 * t + 1) * kBlockBatch] = (uint32_t)(st.base[0] >> 32);
+      }
+      uint32_t nact = 0;
+#pragma unroll
+      for (int s = 0; s < MAXD; ++s)
+        if (st.active[s]) {
+          reinterpret_cast<uint8_t*>(col + (8 + (nact >> 2)) * kBlockBatch)[nact & 3u] = (uint8_t)st.mode[s];
+          col[(kBaseW + 2 * nact) * kBlockBatch] = (uint32_t)st.base[s];
+          col[(kBaseW + 2 * nact + 1) * kBlockBatch] = (uint32_t)(st.base[s] >> 32);
+          ++nact;
         }
-        col[(8 + w) * kBlockBatch] = word;
-      }
-#pragma unroll
-      for (int s = 0; s < MAXD; ++s) {
-        const uint64_t base = st.active[s] ? st.base[s] : st.base[0];
-        col[(kBaseW + 2 * s) * kBlockBatch] = (uint32_t)base;
-        col[(kBaseW + 2 * s + 1) * kBlockBatch] = (uint32_t)(base >> 32);
-      }
+      col[6 * kBlockBatch] = (uint32_t)bi.lo | ((uint32_t)bi.skip0 << 8) | ((uint32_t)bi.lastrun << 16) | (nact << 24);
+      col[7 * kBlockBatch] = __builtin_bit_cast(uint32_t, bi.fQ);
       wave_sync();
     }
     const int nb = (int)((a.S_pre - r < kBlockBatch) ? a.S_pre - r : kBlockBatch);
@@ -403,42 +489,27 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
       const uint32_t w6 = word(6);
       const int blo = (int)(w6 & 0xffu);
       const int skip0 = (int)((w6 >> 8) & 1u);
-      float2 coef[MAXD];
-      const float2* sp[MAXD];
-#pragma unroll
-      for (int s = 0; s < MAXD; ++s) {
-        const uint32_t md = (word(8 + s / 4) >> (8 * (s % 4))) & 0xffu;
-        const SufRow rw = rows[md];
-        coef[s] = uniform_f2(make_float2(rw.re, rw.im));
-        sp[s] = par + ((uint64_t)word(kBaseW + 2 * s) | ((uint64_t)word(kBaseW + 2 * s + 1) << 32));
-      }
+      const int nact = (int)(w6 >> 24);
       float fQ = 1.f;
       int lastrun = 0;
       if constexpr (KIND != 0) {
         fQ = __builtin_bit_cast(float, word(7));
-        lastrun = (int)(w6 >> 16);
+        lastrun = (int)((w6 >> 16) & 0xffu);
       }
-      const float2* lp = par + LBp;
-      const TupleWord* tb = tuples + toff;
       const bool probs = KIND == 1 || (KIND == 3 && kind == 1);
-      char* ob = reinterpret_cast<char*>(out) + CB * (probs ? 4 : 8);
       const uint64_t avail = g_end - g;
       const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
-      ChunkCtx<MAXD> cx{sp, coef, lp, tb, ob, rows, rn, C0, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale,
-                        a.flags, kind};
-      uint32_t uc = (uint32_t)u0;
-      if constexpr (!COMP) {
-        // full groups of 128 children as adjacent pairs per lane, the rest 64 children per step
-        if (uc + 128u <= u_stop) {
-          uint2 tn = load_tuple_pair(cx, uc);
-          for (; uc + 128u <= u_stop; uc += 128u) sweep_pairs<KIND, COMP, MAXD>(cx, uc, tn, uc + 256u <= u_stop);
-        }
-        if (uc + 64u < u_stop) sweep<KIND, COMP, MAXD, 2>(cx, uc);
-        else if (uc < u_stop) sweep<KIND, COMP, MAXD, 1>(cx, uc);
+      const BlockSweep bs{par + LBp, tuples + toff, reinterpret_cast<char*>(out) + CB * (probs ? 4 : 8), rows, rn, C0,
+                          (uint32_t)u0, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale, a.flags, kind};
+      // one sweep per count of active prefix streams, so inactive slots cost no loads or FMAs
+      if constexpr (MAXD > 8) {
+        block_children<KIND, COMP, MAXD, MAXD, SUF>(bj, par, bs);
       } else {
-        // the compensated sums keep one child per lane (register pressure)
-        for (; uc + 64u < u_stop; uc += 128u) sweep<KIND, COMP, MAXD, 2>(cx, uc);
-        if (uc < u_stop) sweep<KIND, COMP, MAXD, 1>(cx, uc);
+        constexpr int N0 = MAXD > 3 ? MAXD - 3 : 1;
+        if (nact <= N0) block_children<KIND, COMP, MAXD, N0, SUF>(bj, par, bs);
+        else if (nact == N0 + 1) block_children<KIND, COMP, MAXD, (N0 + 1 <= MAXD ? N0 + 1 : MAXD), SUF>(bj, par, bs);
+        else if (nact == N0 + 2) block_children<KIND, COMP, MAXD, (N0 + 2 <= MAXD ? N0 + 2 : MAXD), SUF>(bj, par, bs);
+        else block_children<KIND, COMP, MAXD, MAXD, SUF>(bj, par, bs);
       }
       g += u_stop - u0;
       u0 = 0;
@@ -448,17 +519,18 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
 }
 
 
-// KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes
-template <int KIND, bool COMP, int MAXD>
+// KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes;
+// SUF: suffix length (4, or 5 with the 8-byte tuple words)
+template <int KIND, bool COMP, int MAXD, int SUF>
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(COMP || MAXD > 8 ? 3 : 5))) void k_layer_block(const uint64_t* __restrict__ cg_t,
                                                                const int* __restrict__ lo_t,
-                                                               const TupleWord* __restrict__ tuples,
+                                                               const void* __restrict__ tuples,
                                                                int me, int stride, int multiset,
                                                                const float2* __restrict__ par_all,
                                                                void* __restrict__ out_all, BlockLayer a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ SufRow rows[kSufRows];
-  __shared__ float rn[kNormRows * 8];
+  __shared__ float rn[kNormRows * kNormCodes];
   const int K = a.K;
   const int ncg = (K + 1) * stride;
   uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
@@ -466,8 +538,8 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   const int64_t b = blockIdx.y;
   for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = cg_t[i];
   for (int i = threadIdx.x; i < me; i += blockDim.x) lo_l[i] = lo_t[i];
-  fill_suffix_rows(rows, a.U + (size_t)b * a.m * a.m + a.col, a.m, cg_t, stride, me);
-  fill_norm_rows(rn);
+  fill_suffix_rows<SUF>(rows, a.U + (size_t)b * a.m * a.m + a.col, a.m, cg_t, stride, me);
+  fill_norm_rows<SUF>(rn);
   __syncthreads();
   const BlockTables tg{cg_t, lo_t, tuples, me, stride, multiset};  // global: scalar loads
   const BlockTables tl{cg_l, lo_l, tuples, me, stride, multiset};  // LDS: per-lane lookups
@@ -481,7 +553,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   char* out_b = reinterpret_cast<char*>(out_all) + (size_t)b * a.S * (KIND == 1 ? 4 : 8);
   uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + block_table_offset(K, me)) +
                   (size_t)wv * kBlockBatch * block_words(MAXD);
-  block_wave<KIND, COMP, MAXD>(tg, tl, par_all + (size_t)b * a.S_par, out_b, rows, rn, tbl, a, g);
+  block_wave<KIND, COMP, MAXD, SUF>(tg, tl, par_all + (size_t)b * a.S_par, out_b, rows, rn, tbl, a, g);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -543,9 +615,9 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
 k_layer_block_group(GroupLaunch gl) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ SufRow rows[kSufRows];
-  __shared__ float rn[kNormRows * 8];
+  __shared__ float rn[kNormRows * kNormCodes];
   const int K = gl.K;
-  fill_norm_rows(rn);
+  fill_norm_rows<kSuf>(rn);
   const int wv = (int)rfl(threadIdx.x >> 6);
   const uint64_t* cur = nullptr;  // plan whose tables are in LDS
   for_each_item(gl.total, [&](uint32_t item) {
@@ -562,7 +634,7 @@ k_layer_block_group(GroupLaunch gl) {
       for (int i = threadIdx.x; i < it.me; i += blockDim.x) lo_l[i] = it.lo[i];
       cur = it.cg;
     }
-    fill_suffix_rows(rows, it.U + (size_t)b * it.m * it.m + it.col, it.m, it.cg, stride, it.me);
+    fill_suffix_rows<kSuf>(rows, it.U + (size_t)b * it.m * it.m + it.col, it.m, it.cg, stride, it.me);
     __syncthreads();
     const BlockTables tg{it.cg, it.lo, it.tuples, it.me, stride, it.multiset};
     const BlockTables tl{cg_l, lo_l, it.tuples, it.me, stride, it.multiset};
@@ -573,7 +645,7 @@ k_layer_block_group(GroupLaunch gl) {
     const float2* par = it.par + (size_t)b * it.S_par;
     char* out = reinterpret_cast<char*>(it.out) + (size_t)b * it.S * (it.kind == 1 ? 4 : 8);
     uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + gl.table_off) + (size_t)wv * kBlockBatch * block_words(MAXD);
-    block_wave<3, COMP, MAXD>(tg, tl, par, out, rows, rn, tbl, a, g, it.kind);
+    block_wave<3, COMP, MAXD, kSuf>(tg, tl, par, out, rows, rn, tbl, a, g, it.kind);
   });
 }
 

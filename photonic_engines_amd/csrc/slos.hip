@@ -48,6 +48,7 @@ struct DevTables {
   int* lo = nullptr;
   int* modes = nullptr;                    // the plan's input modes (spot permanents)
   slos::TupleWord* tuples = nullptr;       // block-kernel suffix-tuple words (may be absent)
+  uint64_t* tuples5 = nullptr;             // the same for 5-position suffixes (may be absent)
 };
 
 struct slos_plan {
@@ -57,6 +58,7 @@ struct slos_plan {
   std::vector<int> lo;
   double s_fact = 1.0;       // prod_i s_i!
   std::vector<slos::TupleWord> tuples;   // suffix-tuple words of the block kernel (empty: unused)
+  std::vector<uint64_t> tuples5;         // 5-position suffix-tuple words (empty: unused)
   std::vector<DevTables> dev;
   std::mutex mu;
   slos::SpaceView host_view() const { return slos::SpaceView{cg.data(), lo.data(), me, me + 1}; }
@@ -87,6 +89,10 @@ int device_tables(const slos_plan* cp, DevTables* out) {
     HIP_TRY(hipMalloc(&t.tuples, p->tuples.size() * sizeof(slos::TupleWord)));
     HIP_TRY(hipMemcpy(t.tuples, p->tuples.data(), p->tuples.size() * sizeof(slos::TupleWord),
                       hipMemcpyHostToDevice));
+  }
+  if (!p->tuples5.empty()) {
+    HIP_TRY(hipMalloc(&t.tuples5, p->tuples5.size() * sizeof(uint64_t)));
+    HIP_TRY(hipMemcpy(t.tuples5, p->tuples5.data(), p->tuples5.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   }
   p->dev.push_back(t);
   *out = t;
@@ -456,32 +462,49 @@ __global__ __launch_bounds__(64) void k_reduce_gu(const double2* __restrict__ pa
 // the stream-slot count instantiated for prefix width d (exact where the big layers live)
 int block_maxd(int d) { return d <= 4 ? 4 : d <= 8 ? d : 16; }
 
-// dynamic LDS of the single-plan block kernel for layer K, or 0 if it does not fit 64 KB
-size_t block_lds_fit(int K, int me) {
-  const size_t sh = slos::block_dyn_lds(K, me, block_maxd(K - slos::kSuf));
+// dynamic LDS of the single-plan block kernel for layer K with suffix length suf, or 0 if it
+// does not fit 64 KB
+size_t block_lds_fit(int K, int me, int suf) {
+  const size_t sh = slos::block_dyn_lds(K, me, block_maxd(K - suf));
   return sh + slos::kBlockStaticLds <= 64 * 1024 ? sh : 0;
 }
 
+// suffix length of the single-plan block kernel for layer K: 5-position suffixes where their table
+// exists and the prefix is 1..8 positions long (5.4x fewer, larger blocks than with 4 at
+// n = 11, m = 32: less per-block set-up and fewer part-filled sweep steps), else 4
+int block_suffix(const slos_plan* p, const DevTables& dt, int K) {
+  return (dt.tuples5 && K - 5 >= 1 && K - 5 <= 8 && block_lds_fit(K, p->me, 5)) ? 5 : slos::kSuf;
+}
+
 template <int KIND, bool COMP>
-void launch_block_kind(int maxd, dim3 grid, size_t sh, hipStream_t st, const DevTables& dt, const slos_plan* p,
-                       const float2* par, void* out, const slos::BlockLayer& L) {
-#define SLOS_BLK(D)                                                                                          \
-  hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, D>), grid, dim3(slos::kBlockThreads), sh, st, dt.cg, dt.lo, \
-                     dt.tuples, p->me, p->me + 1, p->multiset, par, out, L)
-  // one stream slot per prefix position: exact widths where the big layers live (d = K - 4)
-  if (maxd <= 4) SLOS_BLK(4);
-  else if (maxd == 5) SLOS_BLK(5);
-  else if (maxd == 6) SLOS_BLK(6);
-  else if (maxd == 7) SLOS_BLK(7);
-  else if (maxd == 8) SLOS_BLK(8);
-  else SLOS_BLK(16);
+void launch_block_kind(int maxd, int suf, dim3 grid, size_t sh, hipStream_t st, const DevTables& dt,
+                       const slos_plan* p, const float2* par, void* out, const slos::BlockLayer& L) {
+#define SLOS_BLK(D, S, T)                                                                                        \
+  hipLaunchKernelGGL((slos::k_layer_block<KIND, COMP, D, S>), grid, dim3(slos::kBlockThreads), sh, st, dt.cg, dt.lo, \
+                     T, p->me, p->me + 1, p->multiset, par, out, L)
+  // one stream slot per prefix position: exact widths where the big layers live (d = K - suf)
+  if (suf == 5) {
+    if (maxd <= 4) SLOS_BLK(4, 5, dt.tuples5);
+    else if (maxd == 5) SLOS_BLK(5, 5, dt.tuples5);
+    else if (maxd == 6) SLOS_BLK(6, 5, dt.tuples5);
+    else if (maxd == 7) SLOS_BLK(7, 5, dt.tuples5);
+    else SLOS_BLK(8, 5, dt.tuples5);
+  } else {
+    if (maxd <= 4) SLOS_BLK(4, 4, dt.tuples);
+    else if (maxd == 5) SLOS_BLK(5, 4, dt.tuples);
+    else if (maxd == 6) SLOS_BLK(6, 4, dt.tuples);
+    else if (maxd == 7) SLOS_BLK(7, 4, dt.tuples);
+    else if (maxd == 8) SLOS_BLK(8, 4, dt.tuples);
+    else SLOS_BLK(16, 4, dt.tuples);
+  }
 #undef SLOS_BLK
 }
 
 int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, int kind, bool comp, hipStream_t st) {
   slos::BlockLayer L;
   L.K = a.K; L.m = a.m; L.col = a.col; L.U = a.U; L.S_par = a.S_par; L.S = a.S;
-  L.S_pre = p->size(a.K - slos::kSuf);
+  const int suf = block_suffix(p, dt, a.K);
+  L.S_pre = p->size(a.K - suf);
   L.scale = a.scale; L.flags = a.flags;
   // children per wave: enough waves to fill the chip several times, but long ranges at scale
   uint64_t pw = (a.S / 16384 + 63) & ~63ull;
@@ -490,16 +513,16 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   const uint64_t wgs = (waves + slos::kWaves - 1) / slos::kWaves;
   if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
   if (a.B > 65535) return fail(SLOS_ERANGE, "batch > 65535");
-  const size_t sh = block_lds_fit(a.K, p->me);
+  const size_t sh = block_lds_fit(a.K, p->me, suf);
   if (!sh) return fail(SLOS_ERANGE, "block kernel LDS exceeds 64 KB");
   dim3 grid((unsigned)wgs, (unsigned)a.B);
-  const int maxd = a.K - slos::kSuf;
-  if (kind == 0) comp ? launch_block_kind<0, true>(maxd, grid, sh, st, dt, p, a.par, a.out, L)
-                      : launch_block_kind<0, false>(maxd, grid, sh, st, dt, p, a.par, a.out, L);
-  else if (kind == 1) comp ? launch_block_kind<1, true>(maxd, grid, sh, st, dt, p, a.par, a.out, L)
-                           : launch_block_kind<1, false>(maxd, grid, sh, st, dt, p, a.par, a.out, L);
-  else comp ? launch_block_kind<2, true>(maxd, grid, sh, st, dt, p, a.par, a.out, L)
-            : launch_block_kind<2, false>(maxd, grid, sh, st, dt, p, a.par, a.out, L);
+  const int maxd = a.K - suf;
+  if (kind == 0) comp ? launch_block_kind<0, true>(maxd, suf, grid, sh, st, dt, p, a.par, a.out, L)
+                      : launch_block_kind<0, false>(maxd, suf, grid, sh, st, dt, p, a.par, a.out, L);
+  else if (kind == 1) comp ? launch_block_kind<1, true>(maxd, suf, grid, sh, st, dt, p, a.par, a.out, L)
+                           : launch_block_kind<1, false>(maxd, suf, grid, sh, st, dt, p, a.par, a.out, L);
+  else comp ? launch_block_kind<2, true>(maxd, suf, grid, sh, st, dt, p, a.par, a.out, L)
+            : launch_block_kind<2, false>(maxd, suf, grid, sh, st, dt, p, a.par, a.out, L);
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -512,7 +535,8 @@ int launch_layer(const slos_plan* p, const DevTables& dt, int K, int col, const 
   a.U = U; a.cols = nullptr; a.par = par; a.S_par = S_par; a.out = out; a.S = S; a.B = B;
   a.scale = scale; a.flags = flags;
   const bool comp = flags & SLOS_FLAG_COMPENSATED;
-  if (!(flags & SLOS_FLAG_SIMPLE) && dt.tuples && K > slos::kSuf && K - slos::kSuf <= 16 && block_lds_fit(K, p->me))
+  if (!(flags & SLOS_FLAG_SIMPLE) && dt.tuples && K > slos::kSuf && K - slos::kSuf <= 16 &&
+      block_lds_fit(K, p->me, slos::kSuf))
     return launch_block(p, dt, a, kind, comp, st);
   const size_t sh = (((size_t)(K + 1) * (p->me + 1) * 8 + p->me * 4 + 15) & ~(size_t)15) + (size_t)p->m * 8;
   if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
@@ -965,6 +989,13 @@ void build_tuples(slos_plan* p) {
   const uint64_t N = p->size(slos::kSuf);
   p->tuples.resize(N);
   for (uint64_t u = 0; u < N; ++u) p->tuples[u] = slos::make_tuple_word(sv, p->multiset, u);
+  // 5-position suffixes: positions < 64 keep every rank term inside its packed field
+  // (cg[4][1] < 2^24, cg[3][1] and cg[2][1] < 2^16, cg[1][0] < 2^8)
+  if (p->n > 5 && p->size(5) <= (4u << 20)) {
+    const uint64_t N5 = p->size(5);
+    p->tuples5.resize(N5);
+    for (uint64_t u = 0; u < N5; ++u) p->tuples5[u] = slos::make_tuple_word<5>(sv, p->multiset, u);
+  }
 }
 
 extern "C" {
@@ -1035,6 +1066,7 @@ int slos_plan_destroy(slos_plan* p) {
     (void)hipFree(t.lo);
     (void)hipFree(t.modes);
     if (t.tuples) (void)hipFree(t.tuples);
+    if (t.tuples5) (void)hipFree(t.tuples5);
   }
   (void)hipSetDevice(cur);
   delete p;

@@ -20,6 +20,7 @@ struct Space {
   std::vector<uint64_t> cg;
   std::vector<int> lo;
   std::vector<slos::TupleWord> tuples;
+  std::vector<uint64_t> tuples5;  // 5-position suffix words (layers K >= 6)
 };
 
 bool build(int n, int m, int mask, Space& s) {
@@ -48,6 +49,11 @@ bool build(int n, int m, int mask, Space& s) {
     s.tuples.resize(N);
     for (uint64_t u = 0; u < N; ++u) s.tuples[u] = slos::make_tuple_word(sv, s.multiset, u);
   }
+  if (n > 5) {
+    const uint64_t N5 = s.cg[(size_t)5 * W];
+    s.tuples5.resize(N5);
+    for (uint64_t u = 0; u < N5; ++u) s.tuples5[u] = slos::make_tuple_word<5>(sv, s.multiset, u);
+  }
   return true;
 }
 
@@ -63,32 +69,82 @@ void layer_simple(const Space& s, int K, const cd* ucol, const std::vector<cd>& 
   }
 }
 
+// The device's suffix-table row x | mark << 6: the U coefficient (zero for marked rows) and the
+// per-position rank terms packed as the kernel packs them (SufRow in layer_block.hpp), unpacked as
+// it unpacks them: t[i] = cg[SUF-1-i][x+1] for i < SUF-2, t[SUF-2] = cg[1][x].
+template <int SUF>
+struct Row {
+  cd c;
+  uint32_t t[SUF - 1];
+};
+template <int SUF>
+Row<SUF> suffix_row(const Space& s, const std::vector<cd>& ucol, int i) {
+  const int W = s.me + 1;
+  const int x = i & (slos::kBlockModes - 1);
+  Row<SUF> r{};
+  if (i < slos::kBlockModes && x < (int)ucol.size()) r.c = ucol[x];
+  uint32_t t0 = 0, t1 = 0;
+  if (x < s.me) {
+    if constexpr (SUF == 4) {
+      t0 = (uint32_t)s.cg[(size_t)3 * W + x + 1];
+      t1 = (uint32_t)s.cg[(size_t)2 * W + x + 1] | ((uint32_t)s.cg[(size_t)W + x] << 16);
+    } else {
+      t0 = (uint32_t)s.cg[(size_t)4 * W + x + 1] | ((uint32_t)s.cg[(size_t)W + x] << 24);
+      t1 = (uint32_t)s.cg[(size_t)3 * W + x + 1] | ((uint32_t)s.cg[(size_t)2 * W + x + 1] << 16);
+    }
+  }
+  if constexpr (SUF == 4) {
+    r.t[0] = t0;
+    r.t[1] = t1 & 0xffffu;
+    r.t[2] = t1 >> 16;
+  } else {
+    r.t[0] = t0 & 0xffffffu;
+    r.t[1] = t1 & 0xffffu;
+    r.t[2] = t1 >> 16;
+    r.t[3] = t0 >> 24;
+  }
+  return r;
+}
+
+// local-parent ranks of one suffix word, through the kernel's functions
+template <int SUF>
+void local_ranks(uint32_t C0, const Row<SUF>* rw, uint32_t (&lr)[SUF]) {
+  if constexpr (SUF == 4) {
+    uint32_t a3[4], b2[4], b1[4];
+    for (int j = 0; j < 4; ++j) {
+      a3[j] = rw[j].t[0];
+      b2[j] = rw[j].t[1];
+      b1[j] = rw[j].t[2];
+    }
+    slos::suffix_parent_ranks(C0, a3, b2, b1, lr);
+  } else {
+    uint32_t t4[5], t3[5], t2[5], t1[5];
+    for (int j = 0; j < 5; ++j) {
+      t4[j] = rw[j].t[0];
+      t3[j] = rw[j].t[1];
+      t2[j] = rw[j].t[2];
+      t1[j] = rw[j].t[3];
+    }
+    slos::suffix_parent_ranks5(C0, t4, t3, t2, t1, lr);
+  }
+}
+
 // the block kernel's wave loop: batches of 64 consecutive prefixes (one per lane), then the blocks
 // one at a time, 64 children per step; writes[c] counts how often child c is written
-template <int MAXD>
+template <int MAXD, int SUF>
 void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const std::vector<cd>& par, std::vector<cd>& out,
                  std::vector<double>* norms, uint64_t per_wave, std::vector<int>& writes) {
-  slos::BlockTables t{s.cg.data(), s.lo.data(), s.tuples.data(), s.me, s.me + 1, s.multiset};
+  const void* tw = SUF == 4 ? (const void*)s.tuples.data() : (const void*)s.tuples5.data();
+  slos::BlockTables t{s.cg.data(), s.lo.data(), tw, s.me, s.me + 1, s.multiset};
+  const auto* words = static_cast<const slos::TupleWordT<SUF>*>(tw);
   const int W = s.me + 1;
   const uint64_t S = s.cg[(size_t)K * W];
-  const int d = K - slos::kSuf;
+  const int d = K - SUF;
   const uint64_t S_pre = s.cg[(size_t)d * W];
   slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, W};
-  // the device's suffix table: row x | mark << 6 = (U coefficient or zero, cg[3][x+1], cg[2][x+1], cg[1][x])
-  struct Row { cd c; uint32_t a3, b2, b1; };
-  std::vector<Row> rows(slos::kSufRows);
-  for (int i = 0; i < slos::kSufRows; ++i) {
-    const int x = i & (slos::kBlockModes - 1);
-    Row r{cd(0, 0), 0, 0, 0};
-    if (i < slos::kBlockModes && x < (int)ucol_in.size()) r.c = ucol_in[x];
-    if (x < s.me) {
-      r.a3 = (uint32_t)s.cg[(size_t)3 * W + x + 1];
-      r.b2 = (uint32_t)s.cg[(size_t)2 * W + x + 1];
-      r.b1 = (uint32_t)s.cg[(size_t)W + x];
-    }
-    rows[i] = r;
-  }
-  const uint32_t C0 = (uint32_t)s.cg[(size_t)3 * W];
+  std::vector<Row<SUF>> rows(slos::kSufRows);
+  for (int i = 0; i < slos::kSufRows; ++i) rows[i] = suffix_row<SUF>(s, ucol_in, i);
+  const uint32_t C0 = (uint32_t)s.cg[(size_t)(SUF - 1) * W];
   out.assign(S, 0);
   writes.assign(S, 0);
   if (norms) norms->assign(S, 0.0);
@@ -107,7 +163,7 @@ void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const st
         sv.unrank(r + j, d, q);
         slos::BlockInfo bi;
         slos::Streams<MAXD> st;
-        slos::block_info<MAXD>(t, K, d, q, bi, st);
+        slos::block_info<MAXD, SUF>(t, K, d, q, bi, st);
         if (bi.L == 0) continue;
         const uint64_t avail = g_end - g;
         const uint32_t u_stop = (uint32_t)(((uint64_t)bi.L - u0 < avail) ? (uint64_t)bi.L : u0 + avail);
@@ -118,24 +174,22 @@ void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const st
             cd acc = 0;
             for (int si = 0; si < MAXD; ++si)
               if (st.active[si]) acc += rows.at(st.mode[si]).c * par.at(st.base[si] + u);
-            const slos::TupleWord w = t.tuples[bi.toff + u];
-            uint32_t a3[slos::kSuf], b2[slos::kSuf], b1[slos::kSuf], lr[slos::kSuf];
-            cd coef[slos::kSuf];
-            for (int jj = 0; jj < slos::kSuf; ++jj) {
-              const Row& rw = rows.at((w >> (8 * jj)) & 0xffu);
-              coef[jj] = rw.c;
-              a3[jj] = rw.a3;
-              b2[jj] = rw.b2;
-              b1[jj] = rw.b1;
+            const slos::TupleWordT<SUF> w = words[bi.toff + u];
+            Row<SUF> rw[SUF];
+            uint32_t lr[SUF];
+            cd coef[SUF];
+            for (int jj = 0; jj < SUF; ++jj) {
+              rw[jj] = rows.at(slos::tw_byte(w, jj));
+              coef[jj] = rw[jj].c;
             }
             // tau's first copy of lo continuing the prefix's last run is no edge (edge 0 only)
-            if (bi.skip0 && (int)(w & 0xffu) == bi.lo) coef[0] = cd(0, 0);
-            slos::suffix_parent_ranks(C0, a3, b2, b1, lr);
-            for (int jj = 0; jj < slos::kSuf; ++jj) acc += coef[jj] * par.at(bi.LBp + lr[jj]);
+            if (bi.skip0 && (int)slos::tw_byte(w, 0) == bi.lo) coef[0] = cd(0, 0);
+            local_ranks<SUF>(C0, rw, lr);
+            for (int jj = 0; jj < SUF; ++jj) acc += coef[jj] * par.at(bi.LBp + lr[jj]);
             const uint64_t c = bi.CB + u;
             out.at(c) = acc;
             writes.at(c)++;
-            if (norms) (*norms)[c] = slos::child_norm(bi, w, t.multiset);
+            if (norms) (*norms)[c] = slos::child_norm<SUF>(bi, w, t.multiset);
           }
         g += u_stop - u0;
         u0 = 0;
@@ -149,10 +203,12 @@ void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const st
 
 extern "C" {
 
-// Forward pass from the vacuum to layer n; probs[S_n] = |amp * in_norm|^2 * prod t_j!.
-// Returns 0 on success, 2 if some child was not written exactly once, 1 on bad input.
-int emu_forward(int n, int m, int mask, const int* modes, const double* U, double in_norm, uint64_t per_wave,
-                double* probs) {
+// Forward pass from the vacuum to layer n; probs[S_n] = |amp * in_norm|^2 * prod t_j!. suf 5 runs
+// the layers with K - 5 in [1, 8] with 5-position suffixes (the device's choice where its table
+// exists), suf 4 every block layer with 4. Returns 0 on success, 2 if some child was not written
+// exactly once, 1 on bad input.
+int emu_forward_suf(int n, int m, int mask, const int* modes, const double* U, double in_norm, uint64_t per_wave,
+                    int suf, double* probs) {
   Space s;
   if (!build(n, m, mask, s) || n > 32) return 1;
   const cd* Uc = reinterpret_cast<const cd*>(U);
@@ -173,16 +229,26 @@ int emu_forward(int n, int m, int mask, const int* modes, const double* U, doubl
           norms[g] = slos::run_factorial(p, K, s.multiset);
         }
       }
+    } else if (suf == 5 && K - 5 >= 1 && K - 5 <= 8) {
+      const int d = K - 5;
+      auto* nm = last ? &norms : nullptr;
+      if (d <= 4) layer_block<4, 5>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 5) layer_block<5, 5>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 6) layer_block<6, 5>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 7) layer_block<7, 5>(s, K, ucol, par, out, nm, per_wave, writes);
+      else layer_block<8, 5>(s, K, ucol, par, out, nm, per_wave, writes);
+      for (int w : writes)
+        if (w != 1) return 2;
     } else {
       const int d = K - slos::kSuf;
       // the stream widths the device launcher instantiates: exact for d = 5..8
       auto* nm = last ? &norms : nullptr;
-      if (d <= 4) layer_block<4>(s, K, ucol, par, out, nm, per_wave, writes);
-      else if (d == 5) layer_block<5>(s, K, ucol, par, out, nm, per_wave, writes);
-      else if (d == 6) layer_block<6>(s, K, ucol, par, out, nm, per_wave, writes);
-      else if (d == 7) layer_block<7>(s, K, ucol, par, out, nm, per_wave, writes);
-      else if (d == 8) layer_block<8>(s, K, ucol, par, out, nm, per_wave, writes);
-      else layer_block<16>(s, K, ucol, par, out, nm, per_wave, writes);
+      if (d <= 4) layer_block<4, 4>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 5) layer_block<5, 4>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 6) layer_block<6, 4>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 7) layer_block<7, 4>(s, K, ucol, par, out, nm, per_wave, writes);
+      else if (d == 8) layer_block<8, 4>(s, K, ucol, par, out, nm, per_wave, writes);
+      else layer_block<16, 4>(s, K, ucol, par, out, nm, per_wave, writes);
       for (int w : writes)
         if (w != 1) return 2;
     }
@@ -190,6 +256,11 @@ int emu_forward(int n, int m, int mask, const int* modes, const double* U, doubl
   }
   for (uint64_t g = 0; g < par.size(); ++g) probs[g] = std::norm(par[g] * in_norm) * norms[g];
   return 0;
+}
+
+int emu_forward(int n, int m, int mask, const int* modes, const double* U, double in_norm, uint64_t per_wave,
+                double* probs) {
+  return emu_forward_suf(n, m, mask, modes, U, in_norm, per_wave, 4, probs);
 }
 
 // q_{k-1}(t) = sum_j q_k(t + e_j) count / k via slos::for_each_child (the descent kernel's code)
@@ -241,52 +312,65 @@ extern "C" {
 // every child rank that slos::for_each_child reports for parents [first, first + count) of layer
 // k-1 equals the rank of the child's own sorted positions and is < S_k; returns the number of
 // mismatches (0: all correct), -1 on bad input
-// Every suffix tuple of the space: the tuple word's positions and marks, the four local-parent ranks
-// from the per-position terms (slos::suffix_parent_ranks), and the run data from the mark code
-// against direct computation (unrank, rank of tau minus one position, runs). Returns the number of
+}  // extern "C"
+
+// Every SUF-tuple of the space: the tuple word's positions and marks, the suffix-table rows (packed
+// and unpacked as the kernel does), the SUF local-parent ranks from the per-position terms
+// (slos::suffix_parent_ranks / suffix_parent_ranks5), and the run data from the mark code against
+// direct computation (unrank, rank of tau minus one position, runs). Returns the number of
 // mismatches, or -1 for bad input.
-int64_t emu_check_tuple_words(int m, int mask) {
+template <int SUF>
+int64_t check_tuple_words(int m, int mask) {
   Space s;
-  if (!build(slos::kSuf + 1, m, mask, s)) return -1;
+  if (!build(SUF + 1, m, mask, s)) return -1;
   const int W = s.me + 1;
   slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, W};
-  const uint32_t C0 = (uint32_t)s.cg[(size_t)3 * W];
+  const uint32_t C0 = (uint32_t)s.cg[(size_t)(SUF - 1) * W];
+  const std::vector<cd> ucol(m, cd(1, 0));
+  const uint64_t N = s.cg[(size_t)SUF * W];
   int64_t bad = 0;
-  for (uint64_t u = 0; u < s.tuples.size(); ++u) {
-    int x[slos::kSuf];
-    sv.unrank(u, slos::kSuf, x);
-    const slos::TupleWord w = s.tuples[u];
-    uint32_t a3[slos::kSuf], b2[slos::kSuf], b1[slos::kSuf], lr[slos::kSuf];
-    for (int j = 0; j < slos::kSuf; ++j) {
-      const uint32_t byte = (w >> (8 * j)) & 0xffu;
+  for (uint64_t u = 0; u < N; ++u) {
+    int x[SUF];
+    sv.unrank(u, SUF, x);
+    const slos::TupleWordT<SUF> w = SUF == 4 ? (slos::TupleWordT<SUF>)s.tuples[u] : (slos::TupleWordT<SUF>)s.tuples5[u];
+    Row<SUF> rw[SUF];
+    uint32_t lr[SUF];
+    for (int j = 0; j < SUF; ++j) {
+      const uint32_t byte = slos::tw_byte(w, j);
       const int pos = (int)(byte & (slos::kBlockModes - 1));
       const bool mark = (byte & slos::kTupleMark) != 0;
       if (pos != x[j] || mark != (s.multiset && j > 0 && x[j] == x[j - 1]) || byte >= (uint32_t)slos::kSufRows) ++bad;
-      a3[j] = (uint32_t)s.cg[(size_t)3 * W + pos + 1];
-      b2[j] = (uint32_t)s.cg[(size_t)2 * W + pos + 1];
-      b1[j] = (uint32_t)s.cg[(size_t)W + pos];
+      rw[j] = suffix_row<SUF>(s, ucol, (int)byte);
+      // a marked row has a zero coefficient and the same rank terms
+      if ((std::abs(rw[j].c) != 0) == mark) ++bad;
     }
-    slos::suffix_parent_ranks(C0, a3, b2, b1, lr);
-    for (int j = 0; j < slos::kSuf; ++j) {
-      int r[slos::kSuf - 1], c = 0;
-      for (int i = 0; i < slos::kSuf; ++i)
+    if (SUF == 5 && (slos::tw_byte(w, 5) | slos::tw_byte(w, 6) | slos::tw_byte(w, 7))) ++bad;
+    local_ranks<SUF>(C0, rw, lr);
+    for (int j = 0; j < SUF; ++j) {
+      int r[SUF - 1], c = 0;
+      for (int i = 0; i < SUF; ++i)
         if (i != j) r[c++] = x[i];
-      if ((uint64_t)lr[j] != sv.rank(r, slos::kSuf - 1)) ++bad;
+      if ((uint64_t)lr[j] != sv.rank(r, SUF - 1)) ++bad;
     }
     int lead = 1, fct = 1, run = 1;
     if (s.multiset) {
-      while (lead < slos::kSuf && x[lead] == x[0]) ++lead;
-      for (int j = 1; j < slos::kSuf; ++j) {
+      while (lead < SUF && x[lead] == x[0]) ++lead;
+      for (int j = 1; j < SUF; ++j) {
         run = (x[j] == x[j - 1]) ? run + 1 : 1;
         fct *= run;
       }
     }
-    const uint32_t code = slos::tuple_code(w);
-    if (s.multiset && (slos::code_lead(code) != lead || slos::code_fct(code) != fct)) ++bad;
+    const uint32_t code = slos::tuple_code<SUF>(w);
+    if (s.multiset && (slos::code_lead<SUF>(code) != lead || slos::code_fct<SUF>(code) != fct)) ++bad;
     if (!s.multiset && code != 0) ++bad;
   }
   return bad;
 }
+
+extern "C" {
+
+int64_t emu_check_tuple_words(int m, int mask) { return check_tuple_words<4>(m, mask); }
+int64_t emu_check_tuple_words5(int m, int mask) { return check_tuple_words<5>(m, mask); }
 
 // The block descent's child of every parent in [first, first + count) of layer k-1 and every mode
 // (slos::descend_block_info, desc_lane_init, desc_tail_offset, desc_tail_copies, as the device

@@ -24,6 +24,7 @@ def emu(tmp_path_factory):
     L = ctypes.CDLL(str(out))
     L.emu_child_edges.restype = ctypes.c_int64
     L.emu_check_tuple_words.restype = ctypes.c_int64
+    L.emu_check_tuple_words5.restype = ctypes.c_int64
     return L
 
 
@@ -39,8 +40,13 @@ FWD = [(4, 6, "fock", None), (5, 7, "fock", None), (6, 6, "fock", [3, 0, 2, 0, 1
        (10, 20, "dual_rail", None), (5, 40, "unbunched", None)]
 
 
+@pytest.mark.parametrize("suf", [4, 5])
 @pytest.mark.parametrize("n,m,mask,occ", FWD)
-def test_block_kernel_decomposition(emu, n, m, mask, occ):
+def test_block_kernel_decomposition(emu, n, m, mask, occ, suf):
+    """The block kernel's wave loop with 4-position suffixes on every block layer, and with
+    5-position suffixes on the layers K = 6..13 (the device's choice where its table exists)."""
+    if suf == 5 and n < 6:
+        pytest.skip("no layer with a 5-position suffix block")
     occ = occ or occupation_list(n, m, mask)
     U = haar_unitary(m, n + 7 * m).numpy()
     ref = pull.forward_probs(n, m, occ, U, mask)
@@ -49,8 +55,8 @@ def test_block_kernel_decomposition(emu, n, m, mask, occ):
     innorm = 1 / math.sqrt(math.prod(math.factorial(c) for c in occ))
     for per_wave in (64, 100, 256, 1000, 4096):
         out = np.zeros(ref.shape[0])
-        rc = emu.emu_forward(n, m, pull.MASK_IDS[mask], _p(modes), _p(Ud), ctypes.c_double(innorm),
-                             ctypes.c_uint64(per_wave), _p(out))
+        rc = emu.emu_forward_suf(n, m, pull.MASK_IDS[mask], _p(modes), _p(Ud), ctypes.c_double(innorm),
+                                 ctypes.c_uint64(per_wave), suf, _p(out))
         assert rc == 0, "some child was not written exactly once" if rc == 2 else rc
         # the kernel forms prod t_j! in float32: exact below 2^24, one rounding (2^-24 relative) above
         assert np.all(np.abs(out - ref) <= 1e-14 + 2.4e-7 * np.abs(ref))
@@ -85,8 +91,10 @@ def test_descent_adjoint_and_child_edges(emu, n, m, mask):
 def test_tuple_words_and_local_parent_ranks(emu, m, mask):
     """Every suffix tuple: the 4-byte word's positions and marks, the four local-parent ranks from the
     per-position terms cg[3][x+1], cg[2][x+1], cg[1][x], and the run data from the mark code equal the
-    direct unrank / rank / run computation."""
+    direct unrank / rank / run computation. The same for every 5-tuple (8-byte words, terms
+    cg[4][x+1], cg[3][x+1], cg[2][x+1], cg[1][x] packed into the suffix table's two words)."""
     assert emu.emu_check_tuple_words(m, pull.MASK_IDS[mask]) == 0
+    assert emu.emu_check_tuple_words5(m, pull.MASK_IDS[mask]) == 0
 
 
 @pytest.mark.parametrize("n,m,mask,k,first", [(6, 5, "fock", 6, 0), (7, 6, "fock", 7, 0), (7, 6, "fock", 6, 0),
