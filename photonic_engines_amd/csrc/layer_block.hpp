@@ -520,9 +520,11 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
 
 
 // KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes;
-// SUF: suffix length (4, or 5 with the 8-byte tuple words)
+// SUF: suffix length (4, or 5 with the 8-byte tuple words). Occupancy: 5 waves per SIMD for the
+// plain 4-suffix sweep; 4 for the 5-suffix sweep (its widest prefixes would spill at 96 registers,
+// and 4 measured slightly faster at n = 11, m = 32 as well); 3 for compensated sums and wide prefixes.
 template <int KIND, bool COMP, int MAXD, int SUF>
-__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(COMP || MAXD > 8 ? 3 : 5))) void k_layer_block(const uint64_t* __restrict__ cg_t,
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(COMP || MAXD > 8 ? 3 : SUF == 5 ? 4 : 5))) void k_layer_block(const uint64_t* __restrict__ cg_t,
                                                                const int* __restrict__ lo_t,
                                                                const void* __restrict__ tuples,
                                                                int me, int stride, int multiset,
