@@ -31,6 +31,7 @@ struct BlockLayer {
   float scale;
   unsigned flags;
   uint64_t per_wave;     // children per wave
+  uint32_t xcd_chunk;    // work groups per XCD chunk (see k_layer_block); 1: plain launch order
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -546,12 +547,25 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   const BlockTables tg{cg_t, lo_t, tuples, me, stride, multiset};  // global: scalar loads
   const BlockTables tl{cg_l, lo_l, tuples, me, stride, multiset};  // LDS: per-lane lookups
 
-  // Work groups in launch order: the dispatcher deals them round-robin to the 8 XCDs, so all XCDs
-  // sweep the same region of the layer at once and the far prefix streams one XCD reads are found
-  // by the others in the shared last-level cache (measured faster than giving each XCD a contiguous
-  // eighth of the layer).
+  // Work groups in XCD chunks: the dispatcher deals work groups round-robin to the 8 XCDs, and work
+  // group i (i = 8 t + x, XCD x) takes child range 8 C (t / C) + C x + t % C. Each XCD then sweeps
+  // runs of C consecutive ranges (about 1M children: a block's local parents are fetched into one
+  // XCD's L2, not eight), while the eight XCDs stay in one region of the layer, so the far prefix
+  // streams one XCD reads are found by the others in the shared last-level cache. At n = 11,
+  // m = 32 this measured 28.7 ms for the last layer against 30.3 ms in plain launch order (C = 1)
+  // and 30.4 ms with C = 4096; contiguous eighths of the layer (the largest C) were slower still.
+  // The work groups past the last whole multiple of 8 C keep their own index.
+  uint32_t bid = blockIdx.x;
+  if (a.xcd_chunk > 1) {
+    const uint32_t C = a.xcd_chunk;
+    const uint32_t full = (gridDim.x / (8u * C)) * (8u * C);
+    if (bid < full) {
+      const uint32_t x = bid & 7u, t = bid >> 3;
+      bid = (t / C) * 8u * C + x * C + (t % C);
+    }
+  }
   const int wv = (int)rfl(threadIdx.x >> 6);
-  const uint64_t g = uniform_u64(((uint64_t)blockIdx.x * kWaves + wv) * a.per_wave);
+  const uint64_t g = uniform_u64(((uint64_t)bid * kWaves + wv) * a.per_wave);
   char* out_b = reinterpret_cast<char*>(out_all) + (size_t)b * a.S * (KIND == 1 ? 4 : 8);
   uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + block_table_offset(K, me)) +
                   (size_t)wv * kBlockBatch * block_words(MAXD);
@@ -642,7 +656,7 @@ k_layer_block_group(GroupLaunch gl) {
     const BlockTables tl{cg_l, lo_l, it.tuples, it.me, stride, it.multiset};
     BlockLayer a;
     a.K = K; a.m = it.m; a.col = it.col; a.U = it.U; a.S_par = it.S_par; a.S = it.S; a.S_pre = it.S_pre;
-    a.scale = it.scale; a.flags = it.flags; a.per_wave = it.per_wave;
+    a.scale = it.scale; a.flags = it.flags; a.per_wave = it.per_wave; a.xcd_chunk = 1;
     const uint64_t g = uniform_u64(((uint64_t)w * kWaves + wv) * it.per_wave);
     const float2* par = it.par + (size_t)b * it.S_par;
     char* out = reinterpret_cast<char*>(it.out) + (size_t)b * it.S * (it.kind == 1 ? 4 : 8);

@@ -509,6 +509,8 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   // children per wave: enough waves to fill the chip several times, but long ranges at scale
   uint64_t pw = (a.S / 16384 + 63) & ~63ull;
   L.per_wave = std::min<uint64_t>(std::max<uint64_t>(pw, 256), 4096);
+  // XCD chunks of about 2^20 children (64 work groups at 4096 children per wave)
+  L.xcd_chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 20) / ((uint64_t)slos::kWaves * L.per_wave));
   const uint64_t waves = (a.S + L.per_wave - 1) / L.per_wave;
   const uint64_t wgs = (waves + slos::kWaves - 1) / slos::kWaves;
   if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
