@@ -269,7 +269,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_descend_block(const uint64_t*
                                                                  const TupleWord* __restrict__ tuples, int me, int stride,
                                                                  int K, const float* __restrict__ qk_all, uint64_t Sk,
                                                                  float* __restrict__ qout_all, uint64_t Sp, uint64_t S_pre,
-                                                                 uint64_t per_wave, float scale) {
+                                                                 uint64_t per_wave, float scale, uint32_t xcd_chunk) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int ncg = (K + 1) * stride;
   uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
@@ -281,7 +281,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_descend_block(const uint64_t*
   const BlockTables tl{cg_l, lo_l, tuples, me, stride, 1};
   const int64_t b = blockIdx.y;
   const int wv = (int)rfl(threadIdx.x >> 6);
-  const uint64_t g = uniform_u64(((uint64_t)blockIdx.x * kWaves + wv) * per_wave);
+  // work groups in XCD chunks, as the forward block kernel
+  const uint64_t g = uniform_u64(((uint64_t)xcd_chunk_index(blockIdx.x, xcd_chunk) * kWaves + wv) * per_wave);
   uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + desc_table_offset(K, me)) + (size_t)wv * kBlockBatch * desc_words(MAXD);
   descend_wave<MAXD, ME>(tg, tl, K, qk_all + (size_t)b * Sk, qout_all + (size_t)b * Sp, Sp, S_pre, per_wave, tbl, g, scale);
 }

@@ -334,6 +334,18 @@ __host__ __device__ inline size_t block_dyn_lds(int K, int me, int maxd) {
   return block_table_offset(K, me) + (size_t)kWaves * kBlockBatch * block_words(maxd) * 4;
 }
 
+// Work group `bid` of a 1-D grid in XCD chunks of C: the dispatcher deals work groups
+// round-robin to the 8 XCDs, and work group i = 8 t + x (XCD x) takes range 8 C (t / C) + C x + t % C,
+// so each XCD sweeps runs of C consecutive ranges while the eight XCDs stay in one region. The work
+// groups past the last whole multiple of 8 C keep their own index (a bijection of [0, gridDim.x)).
+__device__ __forceinline__ uint32_t xcd_chunk_index(uint32_t bid, uint32_t C) {
+  if (C <= 1) return bid;
+  const uint32_t full = (gridDim.x / (8u * C)) * (8u * C);
+  if (bid >= full) return bid;
+  const uint32_t x = bid & 7u, t = bid >> 3;
+  return (t / C) * 8u * C + x * C + (t % C);
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -547,23 +559,12 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   const BlockTables tg{cg_t, lo_t, tuples, me, stride, multiset};  // global: scalar loads
   const BlockTables tl{cg_l, lo_l, tuples, me, stride, multiset};  // LDS: per-lane lookups
 
-  // Work groups in XCD chunks: the dispatcher deals work groups round-robin to the 8 XCDs, and work
-  // group i (i = 8 t + x, XCD x) takes child range 8 C (t / C) + C x + t % C. Each XCD then sweeps
-  // runs of C consecutive ranges (about 1M children: a block's local parents are fetched into one
-  // XCD's L2, not eight), while the eight XCDs stay in one region of the layer, so the far prefix
-  // streams one XCD reads are found by the others in the shared last-level cache. At n = 11,
-  // m = 32 this measured 28.7 ms for the last layer against 30.3 ms in plain launch order (C = 1)
-  // and 30.4 ms with C = 4096; contiguous eighths of the layer (the largest C) were slower still.
-  // The work groups past the last whole multiple of 8 C keep their own index.
-  uint32_t bid = blockIdx.x;
-  if (a.xcd_chunk > 1) {
-    const uint32_t C = a.xcd_chunk;
-    const uint32_t full = (gridDim.x / (8u * C)) * (8u * C);
-    if (bid < full) {
-      const uint32_t x = bid & 7u, t = bid >> 3;
-      bid = (t / C) * 8u * C + x * C + (t % C);
-    }
-  }
+  // Work groups in XCD chunks (xcd_chunk_index) of about 1M children: a block's local parents are
+  // fetched into one XCD's L2, not eight, while the eight XCDs stay in one region of the layer, so
+  // the far prefix streams one XCD reads are found by the others in the shared last-level cache.
+  // At n = 11, m = 32 this measured 28.7 ms for the last layer against 30.3 ms in plain launch
+  // order (C = 1) and 30.4 ms with C = 4096; contiguous eighths of the layer were slower still.
+  const uint32_t bid = xcd_chunk_index(blockIdx.x, a.xcd_chunk);
   const int wv = (int)rfl(threadIdx.x >> 6);
   const uint64_t g = uniform_u64(((uint64_t)bid * kWaves + wv) * a.per_wave);
   char* out_b = reinterpret_cast<char*>(out_all) + (size_t)b * a.S * (KIND == 1 ? 4 : 8);

@@ -508,7 +508,9 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   L.scale = a.scale; L.flags = a.flags;
   // children per wave: enough waves to fill the chip several times, but long ranges at scale
   uint64_t pw = (a.S / 16384 + 63) & ~63ull;
-  L.per_wave = std::min<uint64_t>(std::max<uint64_t>(pw, 256), 4096);
+  // (2048 at most for the complex64 intermediate layers, 4096 for the read-out layer: measured best
+  // at n = 11, m = 32)
+  L.per_wave = std::min<uint64_t>(std::max<uint64_t>(pw, 256), kind == 0 ? 2048 : 4096);
   // XCD chunks of about 2^20 children (64 work groups at 4096 children per wave)
   L.xcd_chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 20) / ((uint64_t)slos::kWaves * L.per_wave));
   const uint64_t waves = (a.S + L.per_wave - 1) / L.per_wave;
@@ -1257,6 +1259,9 @@ int slos_adjoint_layer(const slos_plan* p, const void* U, int64_t batch, int k, 
   return SLOS_OK;
 }
 
+// parents per XCD chunk of the block descent (see slos::xcd_chunk_index)
+constexpr uint64_t kDescChunkParents = 1ull << 18;
+
 int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1, int64_t batch, unsigned flags,
                       double scale, slos_stream_t stream) {
   if (!p || !q_k || !q_km1) return fail(SLOS_EINVAL, "NULL argument");
@@ -1276,11 +1281,12 @@ int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1,
       const uint64_t wgs = ((Sp + pw - 1) / pw + slos::kWaves - 1) / slos::kWaves;
       if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
       const dim3 grid((unsigned)wgs, (unsigned)batch);
+      const uint32_t xc = (uint32_t)std::max<uint64_t>(1, kDescChunkParents / ((uint64_t)slos::kWaves * pw));
       const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
       const uint64_t S_pre = p->size(dp);
 #define SLOS_DESC(D, E)                                                                                                  \
   hipLaunchKernelGGL((slos::k_descend_block<D, E>), grid, dim3(slos::kBlockThreads), shb, st, dt.cg, dt.lo, dt.tuples, \
-                     p->me, p->me + 1, k, q_k, Sk, q_km1, Sp, S_pre, pw, (float)scale)
+                     p->me, p->me + 1, k, q_k, Sk, q_km1, Sp, S_pre, pw, (float)scale, xc)
 #define SLOS_DESC_E(D)          \
   if (p->me <= 16) SLOS_DESC(D, 16); \
   else if (p->me <= 32) SLOS_DESC(D, 32); \
