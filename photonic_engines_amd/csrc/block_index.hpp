@@ -300,6 +300,38 @@ inline TupleWordT<SUF> make_tuple_word(const SpaceView& sv, int multiset, uint64
   return w;
 }
 
+// ---- the last layer in group order (5-position suffixes, multiset spaces) ----------------------
+// A block Q = (a, R) of the last layer drops its first photon into the parent block R of layer
+// K-1 (same tuple offsets u, since lo[last(Q)] = lo[last(R)]). The blocks (a, R), a = 0..R_0,
+// form the group of R: they share that parent block, their tuple words and their local-parent
+// ranks. A work item is (rank of R among the (d-1)-prefixes, u_begin, u_end): a range of suffix
+// positions of every block of the group, at most about 2048 children in all (measured best at
+// n = 11, m = 32: 27.5 ms for the last layer against 28.2 ms with 4096, 28.9 ms with 1024).
+constexpr uint32_t kGroupItemChildren = 2048;
+SLOS_HD uint32_t group_item_width(int G) {
+  const uint32_t w = (kGroupItemChildren / (uint32_t)G) & ~127u;
+  return w < 128u ? 128u : w;
+}
+
+// the work items of layer K (prefix length d = K - 5 >= 2), 4 words each (r, u_b, u_e, 0)
+template <class Vec>
+inline void group_items(const SpaceView& sv, int K, Vec& out) {
+  const int d = K - 5;
+  const uint64_t nR = sv.C(d - 1, 0);
+  int R[64];
+  for (uint64_t r = 0; r < nR; ++r) {
+    sv.unrank(r, d - 1, R);
+    const uint32_t L = (uint32_t)sv.C(5, sv.lo[R[d - 2]]);
+    const uint32_t w = group_item_width(R[0] + 1);
+    for (uint32_t ub = 0; ub < L; ub += w) {
+      out.push_back((uint32_t)r);
+      out.push_back(ub);
+      out.push_back(L - ub < w ? L : ub + w);
+      out.push_back(0u);
+    }
+  }
+}
+
 // ---- loss descent on the block structure (pull over children, multiset spaces: lo[x] = x) -----
 // A parent of layer D = K - 1 is (P, s): P its first dp = D - SUF positions, s = (s0..s3) its last
 // four, u its offset in the block of P. Inserting mode a gives the child (K photons):

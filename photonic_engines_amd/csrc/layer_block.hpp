@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "block_index.hpp"
 
 namespace slos {
@@ -50,6 +52,13 @@ __device__ __forceinline__ float2 uniform_f2(float2 v) {
 template <class T>
 __device__ __forceinline__ const T& at_bytes(const T* base, uint32_t off) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+
+// |amp|^2 * f with each product and the sum rounded on their own (no fused multiply-add), so that
+// every kernel writes the same float for the same amplitude whatever the compiler fuses elsewhere
+__device__ __forceinline__ float prob_readout(float re, float im, float f) {
+#pragma clang fp contract(off)
+  return (re * re + im * im) * f;
 }
 
 struct BlockAcc {
@@ -237,7 +246,7 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD, SUF>& cx, uint32_t uc
       re *= cx.scale;
       im *= cx.scale;
       if constexpr (KIND == 1 || KIND == 3) {
-        if (in[c]) *reinterpret_cast<float*>(cx.ob + u[c] * 4u) = (re * re + im * im) * f;
+        if (in[c]) *reinterpret_cast<float*>(cx.ob + u[c] * 4u) = prob_readout(re, im, f);
       } else {
         const float s = (cx.flags & 2u) ? sqrtf(f) : 1.f;
         if (in[c]) *reinterpret_cast<float2*>(cx.ob + u[c] * 8u) = make_float2(re * s, im * s);
@@ -308,7 +317,7 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD, SUF>& cx, uint3
     float re0 = res[0][0] * cx.scale, im0 = res[0][1] * cx.scale;
     float re1 = res[1][0] * cx.scale, im1 = res[1][1] * cx.scale;
     if constexpr (KIND == 1 || KIND == 3) {
-      const float2 v = make_float2((re0 * re0 + im0 * im0) * f[0], (re1 * re1 + im1 * im1) * f[1]);
+      const float2 v = make_float2(prob_readout(re0, im0, f[0]), prob_readout(re1, im1, f[1]));
       __builtin_memcpy(reinterpret_cast<float*>(cx.ob) + u, &v, 8);
     } else {
       const float s0 = (cx.flags & 2u) ? sqrtf(f[0]) : 1.f, s1 = (cx.flags & 2u) ? sqrtf(f[1]) : 1.f;
@@ -571,6 +580,245 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(C
   uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + block_table_offset(K, me)) +
                   (size_t)wv * kBlockBatch * block_words(MAXD);
   block_wave<KIND, COMP, MAXD, SUF>(tg, tl, par_all + (size_t)b * a.S_par, out_b, rows, rn, tbl, a, g);
+}
+
+// ------------------------------------------------------------------------------------------
+// The last layer in group order (5-position suffixes, multiset spaces, prefixes of 2..6 positions;
+// items from group_items in block_index.hpp). A wave takes one item: a group R and a range of
+// suffix positions u. Lane a computes block (a, R) into the wave's group table (32 words a block,
+// read back with wave-uniform 16-byte LDS reads). Each 128-child step then loads the parent block
+// R's pair of values (the prefix stream that drops the first photon, whose reuse distance spans
+// the layer), the tuple words and the local-parent offsets once, and sweeps the step for every
+// block of the group with its own prefix streams, local parents, coefficients and readout. Every
+// child sums its parents in the order of the block kernel (ascending photon position), so both
+// give bit-identical probabilities.
+// ------------------------------------------------------------------------------------------
+constexpr int kGroupWords = 32;  // CB(2) LBp(2) w6 fQ base0(2) coef0(2) 5 x {base(2), coef(2)} toff L
+__host__ __device__ inline size_t group_table_offset(int K, int me) { return (block_lds_bytes(K, me) + 15) & ~(size_t)15; }
+__host__ __device__ inline size_t group_dyn_lds(int K, int me) {
+  return group_table_offset(K, me) + (size_t)kWaves * me * kGroupWords * 4;
+}
+
+// one block of the group for the pair step: NS stream slots (slot 0 shared), both children
+template <int NS>
+__device__ __forceinline__ void group_block_pairs(const uint4 (&bw)[8], const float2* par, const float4& v0,
+                                                  const float2 (&lc)[2][5], const uint32_t (&off)[2][5], uint32_t u,
+                                                  const uint32_t (&code)[2], const bool (&lead)[2], const float* rn,
+                                                  float scale, float* out) {
+  const uint64_t CB = (uint64_t)bw[0].x | ((uint64_t)bw[0].y << 32);
+  const uint64_t LBp = (uint64_t)bw[0].z | ((uint64_t)bw[0].w << 32);
+  const uint32_t w6 = bw[1].x;
+  const float fQ = __builtin_bit_cast(float, bw[1].y);
+  const uint32_t* wd = reinterpret_cast<const uint32_t*>(bw);
+  float4 sv[NS];
+  float2 coef[NS];
+  sv[0] = v0;
+  coef[0] = make_float2(__builtin_bit_cast(float, wd[8]), __builtin_bit_cast(float, wd[9]));
+#pragma unroll
+  for (int s = 1; s < NS; ++s) {
+    const uint64_t base = (uint64_t)wd[10 + 4 * (s - 1)] | ((uint64_t)wd[11 + 4 * (s - 1)] << 32);
+    sv[s] = load_pair(par + base, u * 8u);
+    coef[s] = make_float2(__builtin_bit_cast(float, wd[12 + 4 * (s - 1)]), __builtin_bit_cast(float, wd[13 + 4 * (s - 1)]));
+  }
+  const float2* lp = par + LBp;
+  float2 lv[2][5];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) lv[c][j] = at_bytes(lp, off[c][j]);
+  const uint32_t lastrow = ((w6 >> 16) & 0xffu) + 1u;
+  float pr[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    BlockAcc acc{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      bk_cmac<false>(acc, coef[s], c == 0 ? make_float2(sv[s].x, sv[s].y) : make_float2(sv[s].z, sv[s].w));
+#pragma unroll
+    for (int j = 0; j < 5; ++j) bk_cmac<false>(acc, lc[c][j], lv[c][j]);
+    const float f = fQ * rn[(lead[c] ? lastrow : 0u) * 16u + code[c]];
+    const float re = (acc.re + acc.cre) * scale, im = (acc.im + acc.cim) * scale;
+    pr[c] = prob_readout(re, im, f);
+  }
+  const float2 v = make_float2(pr[0], pr[1]);
+  __builtin_memcpy(out + CB + u, &v, 8);
+}
+
+// the same for one child per lane (the item's last partial step)
+template <int NS>
+__device__ __forceinline__ void group_block_one(const uint4 (&bw)[8], const float2* par, const float2& v0,
+                                                const float2 (&lc)[5], const uint32_t (&off)[5], uint32_t u, bool in,
+                                                uint32_t code, bool lead, const float* rn, float scale, float* out) {
+  const uint64_t CB = (uint64_t)bw[0].x | ((uint64_t)bw[0].y << 32);
+  const uint64_t LBp = (uint64_t)bw[0].z | ((uint64_t)bw[0].w << 32);
+  const uint32_t w6 = bw[1].x;
+  const float fQ = __builtin_bit_cast(float, bw[1].y);
+  const uint32_t* wd = reinterpret_cast<const uint32_t*>(bw);
+  float2 sv[NS], coef[NS];
+  sv[0] = v0;
+  coef[0] = make_float2(__builtin_bit_cast(float, wd[8]), __builtin_bit_cast(float, wd[9]));
+#pragma unroll
+  for (int s = 1; s < NS; ++s) {
+    const uint64_t base = (uint64_t)wd[10 + 4 * (s - 1)] | ((uint64_t)wd[11 + 4 * (s - 1)] << 32);
+    sv[s] = at_bytes(par + base, u * 8u);
+    coef[s] = make_float2(__builtin_bit_cast(float, wd[12 + 4 * (s - 1)]), __builtin_bit_cast(float, wd[13 + 4 * (s - 1)]));
+  }
+  const float2* lp = par + LBp;
+  float2 lv[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) lv[j] = at_bytes(lp, off[j]);
+  BlockAcc acc{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < NS; ++s) bk_cmac<false>(acc, coef[s], sv[s]);
+#pragma unroll
+  for (int j = 0; j < 5; ++j) bk_cmac<false>(acc, lc[j], lv[j]);
+  const float f = fQ * rn[(lead ? ((w6 >> 16) & 0xffu) + 1u : 0u) * 16u + code];
+  const float re = (acc.re + acc.cre) * scale, im = (acc.im + acc.cim) * scale;
+  if (in) out[CB + u] = prob_readout(re, im, f);
+}
+
+// the group table entry of block g (wave-uniform 16-byte LDS reads)
+__device__ __forceinline__ void group_words(const uint32_t* tbl, int g, uint4 (&bw)[8]) {
+  const uint4* src = reinterpret_cast<const uint4*>(tbl + g * kGroupWords);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) bw[i] = src[i];
+}
+
+template <int MAXD, class F>
+__device__ __forceinline__ void group_dispatch(uint32_t nact, F&& f) {
+  constexpr int N0 = MAXD > 3 ? MAXD - 3 : 1;
+  if (nact <= (uint32_t)N0) f(std::integral_constant<int, N0>{});
+  else if (nact == (uint32_t)N0 + 1) f(std::integral_constant<int, (N0 + 1 <= MAXD ? N0 + 1 : MAXD)>{});
+  else if (nact == (uint32_t)N0 + 2) f(std::integral_constant<int, (N0 + 2 <= MAXD ? N0 + 2 : MAXD)>{});
+  else f(std::integral_constant<int, MAXD>{});
+}
+
+template <int MAXD>
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_layer_group(
+    const uint64_t* __restrict__ cg_t, const int* __restrict__ lo_t, const uint64_t* __restrict__ tuples5, int me,
+    int stride, const float2* __restrict__ par_all, float* __restrict__ out_all, BlockLayer a,
+    const uint4* __restrict__ items, uint32_t n_items) {
+  static_assert(MAXD >= 2 && MAXD <= 6, "group table holds 6 stream slots");
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ SufRow rows[kSufRows];
+  __shared__ float rn[kNormRows * kNormCodes];
+  const int K = a.K;
+  const int d = K - 5;
+  const int ncg = (K + 1) * stride;
+  uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
+  int* lo_l = reinterpret_cast<int*>(cg_l + ncg);
+  const int64_t b = blockIdx.y;
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = cg_t[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo_l[i] = lo_t[i];
+  fill_suffix_rows<5>(rows, a.U + (size_t)b * a.m * a.m + a.col, a.m, cg_t, stride, me);
+  fill_norm_rows<5>(rn);
+  __syncthreads();
+  const int wv = (int)rfl(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const uint32_t item = xcd_chunk_index(blockIdx.x, a.xcd_chunk) * kWaves + (uint32_t)wv;
+  if (item >= n_items) return;
+  const uint4 it = items[item];
+  const uint32_t r = rfl(it.x), u_b = rfl(it.y), u_e = rfl(it.z);
+  const BlockTables tl{cg_l, lo_l, tuples5, me, stride, 1};
+  const SpaceView sv{cg_l, lo_l, me, stride};
+  int R[MAXD];
+  sv.unrank(r, d - 1, R);
+  const int G = (int)rfl((uint32_t)R[0]) + 1;  // blocks (a, R), a = 0..R_0 (lo[x] = x)
+  uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + group_table_offset(K, me)) + (size_t)wv * me * kGroupWords;
+  const float2* par = par_all + (size_t)b * a.S_par;
+  float* out = out_all + (size_t)b * a.S;
+  if (lane < G) {
+    int q[MAXD];
+    q[0] = lane;
+#pragma unroll
+    for (int i = 1; i < MAXD; ++i) q[i] = i < d ? R[i - 1] : 0;
+    BlockInfo bi;
+    Streams<MAXD> st;
+    block_info<MAXD, 5>(tl, K, d, q, bi, st);
+    uint32_t* col = tbl + lane * kGroupWords;
+    col[0] = (uint32_t)bi.CB;
+    col[1] = (uint32_t)(bi.CB >> 32);
+    col[2] = (uint32_t)bi.LBp;
+    col[3] = (uint32_t)(bi.LBp >> 32);
+    col[5] = __builtin_bit_cast(uint32_t, bi.fQ);
+    col[6] = (uint32_t)st.base[0];
+    col[7] = (uint32_t)(st.base[0] >> 32);
+    const SufRow r0 = rows[st.mode[0]];
+    col[8] = __builtin_bit_cast(uint32_t, r0.re);
+    col[9] = __builtin_bit_cast(uint32_t, r0.im);
+    // slots 1..5: the other active streams in ascending position, then slot 0's base, no edge
+#pragma unroll
+    for (int t = 1; t < 6; ++t) {
+      col[10 + 4 * (t - 1)] = (uint32_t)st.base[0];
+      col[11 + 4 * (t - 1)] = (uint32_t)(st.base[0] >> 32);
+      col[12 + 4 * (t - 1)] = 0u;
+      col[13 + 4 * (t - 1)] = 0u;
+    }
+    uint32_t nact = 1;
+#pragma unroll
+    for (int s = 1; s < MAXD; ++s)
+      if (st.active[s]) {
+        const SufRow rw = rows[st.mode[s]];
+        col[10 + 4 * (nact - 1)] = (uint32_t)st.base[s];
+        col[11 + 4 * (nact - 1)] = (uint32_t)(st.base[s] >> 32);
+        col[12 + 4 * (nact - 1)] = __builtin_bit_cast(uint32_t, rw.re);
+        col[13 + 4 * (nact - 1)] = __builtin_bit_cast(uint32_t, rw.im);
+        ++nact;
+      }
+    col[4] = (uint32_t)bi.lo | ((uint32_t)bi.skip0 << 8) | ((uint32_t)bi.lastrun << 16) | (nact << 24);
+    col[30] = bi.toff;
+    col[31] = bi.L;
+  }
+  wave_sync();
+  // shared by the group: the parent block R (slot 0's base), the tuple range, lo, skip0
+  const uint64_t base0 = (uint64_t)rfl(tbl[6]) | ((uint64_t)rfl(tbl[7]) << 32);
+  const uint32_t toff = rfl(tbl[30]);
+  const uint32_t w60 = rfl(tbl[4]);
+  const int lo = (int)(w60 & 0xffu), skip0 = (int)((w60 >> 8) & 1u);
+  const float2* sp0 = par + base0;
+  const uint64_t* tb = tuples5 + toff;
+  const float2* const dsp[1] = {sp0};
+  const float2 dcoef[1] = {make_float2(0.f, 0.f)};
+  const ChunkCtx<1, 5> cx{dsp, dcoef, par, tb, nullptr, rows, rn, (uint32_t)cg_t[4 * stride], u_e, lane, 1.f, 0, skip0, lo,
+                          1, a.scale, 0u, 1};
+  uint32_t uc = u_b;
+  for (; uc + 128u <= u_e; uc += 128u) {
+    const uint32_t u = uc + 2u * (uint32_t)lane;
+    const TuplePair<5> tp = load_tuple_pair(cx, uc);
+    const float4 v0 = load_pair(sp0, u * 8u);
+    float2 lc[2][5];
+    uint32_t off[2][5];
+    suffix_edges(cx, tp.a, lc[0], off[0]);
+    suffix_edges(cx, tp.b, lc[1], off[1]);
+    const uint32_t code[2] = {tuple_code<5>(tp.a), tuple_code<5>(tp.b)};
+    const bool lead[2] = {skip0 && (int)((uint32_t)tp.a & 0xffu) == lo, skip0 && (int)((uint32_t)tp.b & 0xffu) == lo};
+    for (int g = 0; g < G; ++g) {
+      uint4 bw[8];
+      group_words(tbl, g, bw);
+      group_dispatch<MAXD>(bw[1].x >> 24, [&](auto ns) {
+        group_block_pairs<decltype(ns)::value>(bw, par, v0, lc, off, u, code, lead, rn, a.scale, out);
+      });
+    }
+  }
+  for (; uc < u_e; uc += 64u) {
+    const uint32_t ur = uc + (uint32_t)lane;
+    const bool in = ur < u_e;
+    const uint32_t u = in ? ur : u_e - 1u;
+    const uint64_t w = tb[u];
+    const float2 v0 = at_bytes(sp0, u * 8u);
+    float2 lc[5];
+    uint32_t off[5];
+    suffix_edges(cx, w, lc, off);
+    const uint32_t code = tuple_code<5>(w);
+    const bool lead = skip0 && (int)((uint32_t)w & 0xffu) == lo;
+    for (int g = 0; g < G; ++g) {
+      uint4 bw[8];
+      group_words(tbl, g, bw);
+      group_dispatch<MAXD>(bw[1].x >> 24, [&](auto ns) {
+        group_block_one<decltype(ns)::value>(bw, par, v0, lc, off, u, in, code, lead, rn, a.scale, out);
+      });
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------

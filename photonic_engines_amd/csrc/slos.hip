@@ -49,6 +49,8 @@ struct DevTables {
   int* modes = nullptr;                    // the plan's input modes (spot permanents)
   slos::TupleWord* tuples = nullptr;       // block-kernel suffix-tuple words (may be absent)
   uint64_t* tuples5 = nullptr;             // the same for 5-position suffixes (may be absent)
+  uint4* items = nullptr;                  // group-order work items of the last layer (may be absent)
+  uint32_t n_items = 0;
 };
 
 struct slos_plan {
@@ -59,6 +61,7 @@ struct slos_plan {
   double s_fact = 1.0;       // prod_i s_i!
   std::vector<slos::TupleWord> tuples;   // suffix-tuple words of the block kernel (empty: unused)
   std::vector<uint64_t> tuples5;         // 5-position suffix-tuple words (empty: unused)
+  std::vector<uint32_t> items;           // group-order work items of layer n, 4 words each (empty: unused)
   std::vector<DevTables> dev;
   std::mutex mu;
   slos::SpaceView host_view() const { return slos::SpaceView{cg.data(), lo.data(), me, me + 1}; }
@@ -93,6 +96,11 @@ int device_tables(const slos_plan* cp, DevTables* out) {
   if (!p->tuples5.empty()) {
     HIP_TRY(hipMalloc(&t.tuples5, p->tuples5.size() * sizeof(uint64_t)));
     HIP_TRY(hipMemcpy(t.tuples5, p->tuples5.data(), p->tuples5.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  }
+  if (!p->items.empty()) {
+    HIP_TRY(hipMalloc(&t.items, p->items.size() * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(t.items, p->items.data(), p->items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    t.n_items = (uint32_t)(p->items.size() / 4);
   }
   p->dev.push_back(t);
   *out = t;
@@ -180,7 +188,7 @@ __global__ __launch_bounds__(256) void k_layer_simple(LayerArgs a) {
         re *= a.scale;
         im *= a.scale;
         if constexpr (KIND == 1) {
-          reinterpret_cast<float*>(a.out)[(size_t)b * a.S + g] = (re * re + im * im) * f;
+          reinterpret_cast<float*>(a.out)[(size_t)b * a.S + g] = slos::prob_readout(re, im, f);
         } else {
           const float s = (a.flags & SLOS_FLAG_IMMEDIATE) ? sqrtf(f) : 1.f;
           reinterpret_cast<float2*>(a.out)[(size_t)b * a.S + g] = make_float2(re * s, im * s);
@@ -517,6 +525,27 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   const uint64_t wgs = (waves + slos::kWaves - 1) / slos::kWaves;
   if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
   if (a.B > 65535) return fail(SLOS_ERANGE, "batch > 65535");
+  // the last layer in group order where its work items exist (probabilities, plain sums)
+  if (kind == 1 && !comp && suf == 5 && dt.items && a.K == p->n) {
+    const size_t shg = slos::group_dyn_lds(a.K, p->me);
+    const uint64_t gwgs = ((uint64_t)dt.n_items + slos::kWaves - 1) / slos::kWaves;
+    if (shg + slos::kBlockStaticLds <= 64 * 1024 && gwgs <= 0x7fffffffull) {
+      L.xcd_chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 20) / ((uint64_t)slos::kWaves * slos::kGroupItemChildren));
+      const dim3 ggrid((unsigned)gwgs, (unsigned)a.B);
+      const int maxd = a.K - suf;
+#define SLOS_GRP(D)                                                                                                 \
+  hipLaunchKernelGGL((slos::k_layer_group<D>), ggrid, dim3(slos::kBlockThreads), shg, st, dt.cg, dt.lo, dt.tuples5, \
+                     p->me, p->me + 1, a.par, static_cast<float*>(a.out), L, dt.items, dt.n_items)
+      if (maxd <= 2) SLOS_GRP(2);
+      else if (maxd == 3) SLOS_GRP(3);
+      else if (maxd == 4) SLOS_GRP(4);
+      else if (maxd == 5) SLOS_GRP(5);
+      else SLOS_GRP(6);
+#undef SLOS_GRP
+      HIP_TRY(hipGetLastError());
+      return SLOS_OK;
+    }
+  }
   const size_t sh = block_lds_fit(a.K, p->me, suf);
   if (!sh) return fail(SLOS_ERANGE, "block kernel LDS exceeds 64 KB");
   dim3 grid((unsigned)wgs, (unsigned)a.B);
@@ -775,7 +804,7 @@ __global__ __launch_bounds__(256) void k_layer_simple_group(slos::GroupLaunch gl
         const float f = slos::run_factorial(pos, K, it.multiset);
         re *= it.scale;
         im *= it.scale;
-        reinterpret_cast<float*>(it.out)[(size_t)b * it.S + g] = (re * re + im * im) * f;
+        reinterpret_cast<float*>(it.out)[(size_t)b * it.S + g] = slos::prob_readout(re, im, f);
       } else {
         reinterpret_cast<float2*>(it.out)[(size_t)b * it.S + g] = make_float2(re, im);
       }
@@ -999,6 +1028,11 @@ void build_tuples(slos_plan* p) {
     const uint64_t N5 = p->size(5);
     p->tuples5.resize(N5);
     for (uint64_t u = 0; u < N5; ++u) p->tuples5[u] = slos::make_tuple_word<5>(sv, p->multiset, u);
+    // the last layer in group order: multiset spaces, prefixes of 2..6 positions
+    // (item ranks are 32-bit; at most about 32M items)
+    if (p->multiset && p->n - 5 >= 2 && p->n - 5 <= 6 && p->size(p->n - 6) <= 0xffffffffull &&
+        p->size(p->n) / slos::kGroupItemChildren + p->size(p->n - 6) <= (32u << 20))
+      slos::group_items(sv, p->n, p->items);
   }
 }
 
@@ -1071,6 +1105,7 @@ int slos_plan_destroy(slos_plan* p) {
     (void)hipFree(t.modes);
     if (t.tuples) (void)hipFree(t.tuples);
     if (t.tuples5) (void)hipFree(t.tuples5);
+    if (t.items) (void)hipFree(t.items);
   }
   (void)hipSetDevice(cur);
   delete p;

@@ -199,14 +199,82 @@ void layer_block(const Space& s, int K, const std::vector<cd>& ucol_in, const st
   }
 }
 
+// The last layer in group order (the device's k_layer_group): the work items of
+// slos::group_items, and per item the blocks (a, R), a = 0..R_0, from slos::block_info. Checks the
+// sharing the kernel relies on (every block of a group has the same first stream base, block size,
+// tuple offset, lo and skip0) and returns false if it fails.
+template <int MAXD>
+bool layer_group(const Space& s, int K, const std::vector<cd>& ucol_in, const std::vector<cd>& par, std::vector<cd>& out,
+                 std::vector<double>* norms, std::vector<int>& writes) {
+  slos::BlockTables t{s.cg.data(), s.lo.data(), s.tuples5.data(), s.me, s.me + 1, s.multiset};
+  const int W = s.me + 1;
+  const uint64_t S = s.cg[(size_t)K * W];
+  const int d = K - 5;
+  slos::SpaceView sv{s.cg.data(), s.lo.data(), s.me, W};
+  std::vector<Row<5>> rows(slos::kSufRows);
+  for (int i = 0; i < slos::kSufRows; ++i) rows[i] = suffix_row<5>(s, ucol_in, i);
+  const uint32_t C0 = (uint32_t)s.cg[(size_t)4 * W];
+  std::vector<uint32_t> items;
+  slos::group_items(sv, K, items);
+  out.assign(S, 0);
+  writes.assign(S, 0);
+  if (norms) norms->assign(S, 0.0);
+  for (size_t it = 0; it < items.size(); it += 4) {
+    int R[64];
+    sv.unrank(items[it], d - 1, R);
+    const uint32_t ub = items[it + 1], ue = items[it + 2];
+    const int G = R[0] + 1;
+    slos::BlockInfo b0{};
+    uint64_t base0 = 0;
+    for (int a = 0; a < G; ++a) {
+      int q[MAXD];
+      q[0] = a;
+      for (int i = 1; i < MAXD; ++i) q[i] = i < d ? R[i - 1] : 0;
+      slos::BlockInfo bi;
+      slos::Streams<MAXD> st;
+      slos::block_info<MAXD, 5>(t, K, d, q, bi, st);
+      if (!st.active[0]) return false;
+      if (a == 0) {
+        b0 = bi;
+        base0 = st.base[0];
+      } else if (st.base[0] != base0 || bi.L != b0.L || bi.toff != b0.toff || bi.lo != b0.lo || bi.skip0 != b0.skip0) {
+        return false;
+      }
+      if (ue > bi.L || ub >= ue) return false;
+      for (uint32_t u = ub; u < ue; ++u) {
+        cd acc = rows.at(st.mode[0]).c * par.at(base0 + u);
+        for (int si = 1; si < MAXD; ++si)
+          if (st.active[si]) acc += rows.at(st.mode[si]).c * par.at(st.base[si] + u);
+        const slos::TupleWordT<5> w = s.tuples5.at(b0.toff + u);
+        Row<5> rw[5];
+        uint32_t lr[5];
+        cd coef[5];
+        for (int jj = 0; jj < 5; ++jj) {
+          rw[jj] = rows.at(slos::tw_byte(w, jj));
+          coef[jj] = rw[jj].c;
+        }
+        if (b0.skip0 && (int)slos::tw_byte(w, 0) == b0.lo) coef[0] = cd(0, 0);
+        local_ranks<5>(C0, rw, lr);
+        for (int jj = 0; jj < 5; ++jj) acc += coef[jj] * par.at(bi.LBp + lr[jj]);
+        const uint64_t c = bi.CB + u;
+        out.at(c) = acc;
+        writes.at(c)++;
+        if (norms) (*norms)[c] = slos::child_norm<5>(bi, w, t.multiset);
+      }
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
 
 // Forward pass from the vacuum to layer n; probs[S_n] = |amp * in_norm|^2 * prod t_j!. suf 5 runs
 // the layers with K - 5 in [1, 8] with 5-position suffixes (the device's choice where its table
-// exists), suf 4 every block layer with 4. Returns 0 on success, 2 if some child was not written
-// exactly once, 1 on bad input.
+// exists), suf 4 every block layer with 4, suf 6 as 5 but the last layer of a multiset space with
+// a prefix of 2..6 positions in group order. Returns 0 on success, 2 if some child was not written
+// exactly once, 3 if a group's blocks do not share what the group kernel shares, 1 on bad input.
 int emu_forward_suf(int n, int m, int mask, const int* modes, const double* U, double in_norm, uint64_t per_wave,
                     int suf, double* probs) {
   Space s;
@@ -229,7 +297,18 @@ int emu_forward_suf(int n, int m, int mask, const int* modes, const double* U, d
           norms[g] = slos::run_factorial(p, K, s.multiset);
         }
       }
-    } else if (suf == 5 && K - 5 >= 1 && K - 5 <= 8) {
+    } else if (suf == 6 && last && s.multiset && K - 5 >= 2 && K - 5 <= 6) {
+      const int d = K - 5;
+      bool ok;
+      if (d == 2) ok = layer_group<2>(s, K, ucol, par, out, &norms, writes);
+      else if (d == 3) ok = layer_group<3>(s, K, ucol, par, out, &norms, writes);
+      else if (d == 4) ok = layer_group<4>(s, K, ucol, par, out, &norms, writes);
+      else if (d == 5) ok = layer_group<5>(s, K, ucol, par, out, &norms, writes);
+      else ok = layer_group<6>(s, K, ucol, par, out, &norms, writes);
+      if (!ok) return 3;
+      for (int w : writes)
+        if (w != 1) return 2;
+    } else if (suf >= 5 && K - 5 >= 1 && K - 5 <= 8) {
       const int d = K - 5;
       auto* nm = last ? &norms : nullptr;
       if (d <= 4) layer_block<4, 5>(s, K, ucol, par, out, nm, per_wave, writes);
