@@ -373,7 +373,8 @@ def main() -> None:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": f"k_layer_block (layer {n})", "kernel_ms": last_ms,
+                     "kernel": (f"k_layer_group (layer {n}, group order)" if 7 <= n <= 11 and math.comb(m + 4, 5) <= 4 << 20
+                                else f"k_layer_block (layer {n})"), "kernel_ms": last_ms,
                      "alg_bytes_per_launch": last_bytes},
         "whole_forward": {"alg_bytes": bytes_alg(n, m), "alg_GBps": bytes_alg(n, m) / dt / 1e9,
                           "alg_frac": bytes_alg(n, m) / dt / 1e9 / PEAK_HBM_GBS},
