@@ -782,10 +782,21 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4
   const ChunkCtx<1, 5> cx{dsp, dcoef, par, tb, nullptr, rows, rn, (uint32_t)cg_t[4 * stride], u_e, lane, 1.f, 0, skip0, lo,
                           1, a.scale, 0u, 1};
   uint32_t uc = u_b;
+  // the shared tuple words and first-stream pair are loaded one step ahead
+  TuplePair<5> tpn{};
+  float4 v0n{};
+  if (uc + 128u <= u_e) {
+    tpn = load_tuple_pair(cx, uc);
+    v0n = load_pair(sp0, (uc + 2u * (uint32_t)lane) * 8u);
+  }
   for (; uc + 128u <= u_e; uc += 128u) {
     const uint32_t u = uc + 2u * (uint32_t)lane;
-    const TuplePair<5> tp = load_tuple_pair(cx, uc);
-    const float4 v0 = load_pair(sp0, u * 8u);
+    const TuplePair<5> tp = tpn;
+    const float4 v0 = v0n;
+    if (uc + 256u <= u_e) {
+      tpn = load_tuple_pair(cx, uc + 128u);
+      v0n = load_pair(sp0, (u + 128u) * 8u);
+    }
     float2 lc[2][5];
     uint32_t off[2][5];
     suffix_edges(cx, tp.a, lc[0], off[0]);
