@@ -599,12 +599,13 @@ __host__ __device__ inline size_t group_dyn_lds(int K, int me) {
   return group_table_offset(K, me) + (size_t)kWaves * me * kGroupWords * 4;
 }
 
-// one block of the group for the pair step: NS stream slots (slot 0 shared), both children
-template <int NS>
+// one block of the group for the pair step: NS stream slots (slot 0 shared), both children;
+// KIND 1 writes probabilities (float32), KIND 0 the raw complex64 amplitudes of a middle layer
+template <int NS, int KIND>
 __device__ __forceinline__ void group_block_pairs(const uint4 (&bw)[8], const float2* par, const float4& v0,
                                                   const float2 (&lc)[2][5], const uint32_t (&off)[2][5], uint32_t u,
                                                   const uint32_t (&code)[2], const bool (&lead)[2], const float* rn,
-                                                  float scale, float* out) {
+                                                  float scale, void* out) {
   const uint64_t CB = (uint64_t)bw[0].x | ((uint64_t)bw[0].y << 32);
   const uint64_t LBp = (uint64_t)bw[0].z | ((uint64_t)bw[0].w << 32);
   const uint32_t w6 = bw[1].x;
@@ -627,7 +628,7 @@ __device__ __forceinline__ void group_block_pairs(const uint4 (&bw)[8], const fl
 #pragma unroll
     for (int j = 0; j < 5; ++j) lv[c][j] = at_bytes(lp, off[c][j]);
   const uint32_t lastrow = ((w6 >> 16) & 0xffu) + 1u;
-  float pr[2];
+  float res[2][2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     BlockAcc acc{0.f, 0.f, 0.f, 0.f};
@@ -636,19 +637,29 @@ __device__ __forceinline__ void group_block_pairs(const uint4 (&bw)[8], const fl
       bk_cmac<false>(acc, coef[s], c == 0 ? make_float2(sv[s].x, sv[s].y) : make_float2(sv[s].z, sv[s].w));
 #pragma unroll
     for (int j = 0; j < 5; ++j) bk_cmac<false>(acc, lc[c][j], lv[c][j]);
-    const float f = fQ * rn[(lead[c] ? lastrow : 0u) * 16u + code[c]];
-    const float re = (acc.re + acc.cre) * scale, im = (acc.im + acc.cim) * scale;
-    pr[c] = prob_readout(re, im, f);
+    res[c][0] = acc.re + acc.cre;
+    res[c][1] = acc.im + acc.cim;
   }
-  const float2 v = make_float2(pr[0], pr[1]);
-  __builtin_memcpy(out + CB + u, &v, 8);
+  if constexpr (KIND == 0) {
+    const float4 v = make_float4(res[0][0], res[0][1], res[1][0], res[1][1]);
+    __builtin_memcpy(static_cast<float2*>(out) + CB + u, &v, 16);
+  } else {
+    float pr[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const float f = fQ * rn[(lead[c] ? lastrow : 0u) * 16u + code[c]];
+      pr[c] = prob_readout(res[c][0] * scale, res[c][1] * scale, f);
+    }
+    const float2 v = make_float2(pr[0], pr[1]);
+    __builtin_memcpy(static_cast<float*>(out) + CB + u, &v, 8);
+  }
 }
 
 // the same for one child per lane (the item's last partial step)
-template <int NS>
+template <int NS, int KIND>
 __device__ __forceinline__ void group_block_one(const uint4 (&bw)[8], const float2* par, const float2& v0,
                                                 const float2 (&lc)[5], const uint32_t (&off)[5], uint32_t u, bool in,
-                                                uint32_t code, bool lead, const float* rn, float scale, float* out) {
+                                                uint32_t code, bool lead, const float* rn, float scale, void* out) {
   const uint64_t CB = (uint64_t)bw[0].x | ((uint64_t)bw[0].y << 32);
   const uint64_t LBp = (uint64_t)bw[0].z | ((uint64_t)bw[0].w << 32);
   const uint32_t w6 = bw[1].x;
@@ -672,9 +683,13 @@ __device__ __forceinline__ void group_block_one(const uint4 (&bw)[8], const floa
   for (int s = 0; s < NS; ++s) bk_cmac<false>(acc, coef[s], sv[s]);
 #pragma unroll
   for (int j = 0; j < 5; ++j) bk_cmac<false>(acc, lc[j], lv[j]);
-  const float f = fQ * rn[(lead ? ((w6 >> 16) & 0xffu) + 1u : 0u) * 16u + code];
-  const float re = (acc.re + acc.cre) * scale, im = (acc.im + acc.cim) * scale;
-  if (in) out[CB + u] = prob_readout(re, im, f);
+  const float re = acc.re + acc.cre, im = acc.im + acc.cim;
+  if constexpr (KIND == 0) {
+    if (in) static_cast<float2*>(out)[CB + u] = make_float2(re, im);
+  } else {
+    const float f = fQ * rn[(lead ? ((w6 >> 16) & 0xffu) + 1u : 0u) * 16u + code];
+    if (in) static_cast<float*>(out)[CB + u] = prob_readout(re * scale, im * scale, f);
+  }
 }
 
 // the group table entry of block g (wave-uniform 16-byte LDS reads)
@@ -693,10 +708,10 @@ __device__ __forceinline__ void group_dispatch(uint32_t nact, F&& f) {
   else f(std::integral_constant<int, MAXD>{});
 }
 
-template <int MAXD>
+template <int MAXD, int KIND>
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_layer_group(
     const uint64_t* __restrict__ cg_t, const int* __restrict__ lo_t, const uint64_t* __restrict__ tuples5, int me,
-    int stride, const float2* __restrict__ par_all, float* __restrict__ out_all, BlockLayer a,
+    int stride, const float2* __restrict__ par_all, void* __restrict__ out_all, BlockLayer a,
     const uint4* __restrict__ items, uint32_t n_items) {
   static_assert(MAXD >= 2 && MAXD <= 6, "group table holds 6 stream slots");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -726,7 +741,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4
   const int G = (int)rfl((uint32_t)R[0]) + 1;  // blocks (a, R), a = 0..R_0 (lo[x] = x)
   uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + group_table_offset(K, me)) + (size_t)wv * me * kGroupWords;
   const float2* par = par_all + (size_t)b * a.S_par;
-  float* out = out_all + (size_t)b * a.S;
+  void* out = static_cast<char*>(out_all) + (size_t)b * a.S * (KIND == 0 ? 8 : 4);
   if (lane < G) {
     int q[MAXD];
     q[0] = lane;
@@ -807,7 +822,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4
       uint4 bw[8];
       group_words(tbl, g, bw);
       group_dispatch<MAXD>(bw[1].x >> 24, [&](auto ns) {
-        group_block_pairs<decltype(ns)::value>(bw, par, v0, lc, off, u, code, lead, rn, a.scale, out);
+        group_block_pairs<decltype(ns)::value, KIND>(bw, par, v0, lc, off, u, code, lead, rn, a.scale, out);
       });
     }
   }
@@ -826,7 +841,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4
       uint4 bw[8];
       group_words(tbl, g, bw);
       group_dispatch<MAXD>(bw[1].x >> 24, [&](auto ns) {
-        group_block_one<decltype(ns)::value>(bw, par, v0, lc, off, u, in, code, lead, rn, a.scale, out);
+        group_block_one<decltype(ns)::value, KIND>(bw, par, v0, lc, off, u, in, code, lead, rn, a.scale, out);
       });
     }
   }

@@ -49,8 +49,10 @@ struct DevTables {
   int* modes = nullptr;                    // the plan's input modes (spot permanents)
   slos::TupleWord* tuples = nullptr;       // block-kernel suffix-tuple words (may be absent)
   uint64_t* tuples5 = nullptr;             // the same for 5-position suffixes (may be absent)
-  uint4* items = nullptr;                  // group-order work items of the last layer (may be absent)
+  uint4* items = nullptr;                  // group-order work items of layer n (may be absent)
   uint32_t n_items = 0;
+  uint4* items_prev = nullptr;             // the same for layer n - 1 (may be absent)
+  uint32_t n_items_prev = 0;
 };
 
 struct slos_plan {
@@ -62,6 +64,7 @@ struct slos_plan {
   std::vector<slos::TupleWord> tuples;   // suffix-tuple words of the block kernel (empty: unused)
   std::vector<uint64_t> tuples5;         // 5-position suffix-tuple words (empty: unused)
   std::vector<uint32_t> items;           // group-order work items of layer n, 4 words each (empty: unused)
+  std::vector<uint32_t> items_prev;      // the same for layer n - 1
   std::vector<DevTables> dev;
   std::mutex mu;
   slos::SpaceView host_view() const { return slos::SpaceView{cg.data(), lo.data(), me, me + 1}; }
@@ -101,6 +104,11 @@ int device_tables(const slos_plan* cp, DevTables* out) {
     HIP_TRY(hipMalloc(&t.items, p->items.size() * sizeof(uint32_t)));
     HIP_TRY(hipMemcpy(t.items, p->items.data(), p->items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     t.n_items = (uint32_t)(p->items.size() / 4);
+  }
+  if (!p->items_prev.empty()) {
+    HIP_TRY(hipMalloc(&t.items_prev, p->items_prev.size() * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(t.items_prev, p->items_prev.data(), p->items_prev.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    t.n_items_prev = (uint32_t)(p->items_prev.size() / 4);
   }
   p->dev.push_back(t);
   *out = t;
@@ -525,22 +533,31 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   const uint64_t wgs = (waves + slos::kWaves - 1) / slos::kWaves;
   if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
   if (a.B > 65535) return fail(SLOS_ERANGE, "batch > 65535");
-  // the last layer in group order where its work items exist (probabilities, plain sums)
-  if (kind == 1 && !comp && suf == 5 && dt.items && a.K == p->n) {
+  // the last layer (probabilities) and the layer before it (complex64) in group order where their
+  // work items exist (plain sums)
+  const bool grp_last = kind == 1 && a.K == p->n && dt.items;
+  const bool grp_prev = kind == 0 && a.K == p->n - 1 && dt.items_prev;
+  if (!comp && suf == 5 && (grp_last || grp_prev)) {
+    const uint4* gitems = grp_last ? dt.items : dt.items_prev;
+    const uint32_t gn = grp_last ? dt.n_items : dt.n_items_prev;
     const size_t shg = slos::group_dyn_lds(a.K, p->me);
-    const uint64_t gwgs = ((uint64_t)dt.n_items + slos::kWaves - 1) / slos::kWaves;
+    const uint64_t gwgs = ((uint64_t)gn + slos::kWaves - 1) / slos::kWaves;
     if (shg + slos::kBlockStaticLds <= 64 * 1024 && gwgs <= 0x7fffffffull) {
       L.xcd_chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 20) / ((uint64_t)slos::kWaves * slos::kGroupItemChildren));
       const dim3 ggrid((unsigned)gwgs, (unsigned)a.B);
       const int maxd = a.K - suf;
-#define SLOS_GRP(D)                                                                                                 \
-  hipLaunchKernelGGL((slos::k_layer_group<D>), ggrid, dim3(slos::kBlockThreads), shg, st, dt.cg, dt.lo, dt.tuples5, \
-                     p->me, p->me + 1, a.par, static_cast<float*>(a.out), L, dt.items, dt.n_items)
-      if (maxd <= 2) SLOS_GRP(2);
-      else if (maxd == 3) SLOS_GRP(3);
-      else if (maxd == 4) SLOS_GRP(4);
-      else if (maxd == 5) SLOS_GRP(5);
-      else SLOS_GRP(6);
+#define SLOS_GRP(D)                                                                                                    \
+  if (grp_last)                                                                                                        \
+    hipLaunchKernelGGL((slos::k_layer_group<D, 1>), ggrid, dim3(slos::kBlockThreads), shg, st, dt.cg, dt.lo, dt.tuples5, \
+                       p->me, p->me + 1, a.par, a.out, L, gitems, gn);                                                  \
+  else                                                                                                                 \
+    hipLaunchKernelGGL((slos::k_layer_group<D, 0>), ggrid, dim3(slos::kBlockThreads), shg, st, dt.cg, dt.lo, dt.tuples5, \
+                       p->me, p->me + 1, a.par, a.out, L, gitems, gn)
+      if (maxd <= 2) { SLOS_GRP(2); }
+      else if (maxd == 3) { SLOS_GRP(3); }
+      else if (maxd == 4) { SLOS_GRP(4); }
+      else if (maxd == 5) { SLOS_GRP(5); }
+      else { SLOS_GRP(6); }
 #undef SLOS_GRP
       HIP_TRY(hipGetLastError());
       return SLOS_OK;
@@ -1029,10 +1046,11 @@ void build_tuples(slos_plan* p) {
     p->tuples5.resize(N5);
     for (uint64_t u = 0; u < N5; ++u) p->tuples5[u] = slos::make_tuple_word<5>(sv, p->multiset, u);
     // the last layer in group order: multiset spaces, prefixes of 2..6 positions
-    // (item ranks are 32-bit; at most about 32M items)
-    if (p->multiset && p->n - 5 >= 2 && p->n - 5 <= 6 && p->size(p->n - 6) <= 0xffffffffull &&
-        p->size(p->n) / slos::kGroupItemChildren + p->size(p->n - 6) <= (32u << 20))
-      slos::group_items(sv, p->n, p->items);
+    // (item ranks are 32-bit; at most about 32M items a layer)
+    for (int K = p->n - 1; K <= p->n; ++K)
+      if (p->multiset && K - 5 >= 2 && K - 5 <= 6 && p->size(K - 6) <= 0xffffffffull &&
+          p->size(K) / slos::kGroupItemChildren + p->size(K - 6) <= (32u << 20))
+        slos::group_items(sv, K, K == p->n ? p->items : p->items_prev);
   }
 }
 
@@ -1106,6 +1124,7 @@ int slos_plan_destroy(slos_plan* p) {
     if (t.tuples) (void)hipFree(t.tuples);
     if (t.tuples5) (void)hipFree(t.tuples5);
     if (t.items) (void)hipFree(t.items);
+    if (t.items_prev) (void)hipFree(t.items_prev);
   }
   (void)hipSetDevice(cur);
   delete p;

@@ -272,8 +272,8 @@ extern "C" {
 
 // Forward pass from the vacuum to layer n; probs[S_n] = |amp * in_norm|^2 * prod t_j!. suf 5 runs
 // the layers with K - 5 in [1, 8] with 5-position suffixes (the device's choice where its table
-// exists), suf 4 every block layer with 4, suf 6 as 5 but the last layer of a multiset space with
-// a prefix of 2..6 positions in group order. Returns 0 on success, 2 if some child was not written
+// exists), suf 4 every block layer with 4, suf 6 as 5 but the last two layers of a multiset space
+// that have a prefix of 2..6 positions in group order. Returns 0 on success, 2 if some child was not written
 // exactly once, 3 if a group's blocks do not share what the group kernel shares, 1 on bad input.
 int emu_forward_suf(int n, int m, int mask, const int* modes, const double* U, double in_norm, uint64_t per_wave,
                     int suf, double* probs) {
@@ -297,14 +297,15 @@ int emu_forward_suf(int n, int m, int mask, const int* modes, const double* U, d
           norms[g] = slos::run_factorial(p, K, s.multiset);
         }
       }
-    } else if (suf == 6 && last && s.multiset && K - 5 >= 2 && K - 5 <= 6) {
+    } else if (suf == 6 && K >= n - 1 && s.multiset && K - 5 >= 2 && K - 5 <= 6) {
       const int d = K - 5;
+      auto* nm = last ? &norms : nullptr;
       bool ok;
-      if (d == 2) ok = layer_group<2>(s, K, ucol, par, out, &norms, writes);
-      else if (d == 3) ok = layer_group<3>(s, K, ucol, par, out, &norms, writes);
-      else if (d == 4) ok = layer_group<4>(s, K, ucol, par, out, &norms, writes);
-      else if (d == 5) ok = layer_group<5>(s, K, ucol, par, out, &norms, writes);
-      else ok = layer_group<6>(s, K, ucol, par, out, &norms, writes);
+      if (d == 2) ok = layer_group<2>(s, K, ucol, par, out, nm, writes);
+      else if (d == 3) ok = layer_group<3>(s, K, ucol, par, out, nm, writes);
+      else if (d == 4) ok = layer_group<4>(s, K, ucol, par, out, nm, writes);
+      else if (d == 5) ok = layer_group<5>(s, K, ucol, par, out, nm, writes);
+      else ok = layer_group<6>(s, K, ucol, par, out, nm, writes);
       if (!ok) return 3;
       for (int w : writes)
         if (w != 1) return 2;
