@@ -49,10 +49,8 @@ struct DevTables {
   int* modes = nullptr;                    // the plan's input modes (spot permanents)
   slos::TupleWord* tuples = nullptr;       // block-kernel suffix-tuple words (may be absent)
   uint64_t* tuples5 = nullptr;             // the same for 5-position suffixes (may be absent)
-  uint4* items = nullptr;                  // group-order work items of layer n (may be absent)
-  uint32_t n_items = 0;
-  uint4* items_prev = nullptr;             // the same for layer n - 1 (may be absent)
-  uint32_t n_items_prev = 0;
+  uint4* items[33] = {};                   // group-order work items of layer K (may be absent)
+  uint32_t n_items[33] = {};
 };
 
 struct slos_plan {
@@ -63,8 +61,7 @@ struct slos_plan {
   double s_fact = 1.0;       // prod_i s_i!
   std::vector<slos::TupleWord> tuples;   // suffix-tuple words of the block kernel (empty: unused)
   std::vector<uint64_t> tuples5;         // 5-position suffix-tuple words (empty: unused)
-  std::vector<uint32_t> items;           // group-order work items of layer n, 4 words each (empty: unused)
-  std::vector<uint32_t> items_prev;      // the same for layer n - 1
+  std::vector<uint32_t> items[33];       // group-order work items of layer K, 4 words each (empty: unused)
   std::vector<DevTables> dev;
   std::mutex mu;
   slos::SpaceView host_view() const { return slos::SpaceView{cg.data(), lo.data(), me, me + 1}; }
@@ -100,16 +97,12 @@ int device_tables(const slos_plan* cp, DevTables* out) {
     HIP_TRY(hipMalloc(&t.tuples5, p->tuples5.size() * sizeof(uint64_t)));
     HIP_TRY(hipMemcpy(t.tuples5, p->tuples5.data(), p->tuples5.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   }
-  if (!p->items.empty()) {
-    HIP_TRY(hipMalloc(&t.items, p->items.size() * sizeof(uint32_t)));
-    HIP_TRY(hipMemcpy(t.items, p->items.data(), p->items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    t.n_items = (uint32_t)(p->items.size() / 4);
-  }
-  if (!p->items_prev.empty()) {
-    HIP_TRY(hipMalloc(&t.items_prev, p->items_prev.size() * sizeof(uint32_t)));
-    HIP_TRY(hipMemcpy(t.items_prev, p->items_prev.data(), p->items_prev.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    t.n_items_prev = (uint32_t)(p->items_prev.size() / 4);
-  }
+  for (int K = 0; K <= p->n; ++K)
+    if (!p->items[K].empty()) {
+      HIP_TRY(hipMalloc(&t.items[K], p->items[K].size() * sizeof(uint32_t)));
+      HIP_TRY(hipMemcpy(t.items[K], p->items[K].data(), p->items[K].size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      t.n_items[K] = (uint32_t)(p->items[K].size() / 4);
+    }
   p->dev.push_back(t);
   *out = t;
   return SLOS_OK;
@@ -533,13 +526,11 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   const uint64_t wgs = (waves + slos::kWaves - 1) / slos::kWaves;
   if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
   if (a.B > 65535) return fail(SLOS_ERANGE, "batch > 65535");
-  // the last layer (probabilities) and the layer before it (complex64) in group order where their
-  // work items exist (plain sums)
-  const bool grp_last = kind == 1 && a.K == p->n && dt.items;
-  const bool grp_prev = kind == 0 && a.K == p->n - 1 && dt.items_prev;
-  if (!comp && suf == 5 && (grp_last || grp_prev)) {
-    const uint4* gitems = grp_last ? dt.items : dt.items_prev;
-    const uint32_t gn = grp_last ? dt.n_items : dt.n_items_prev;
+  // layers in group order where their work items exist (plain sums; probabilities or complex64)
+  const bool grp_last = kind == 1;
+  if (!comp && suf == 5 && kind <= 1 && dt.items[a.K]) {
+    const uint4* gitems = dt.items[a.K];
+    const uint32_t gn = dt.n_items[a.K];
     const size_t shg = slos::group_dyn_lds(a.K, p->me);
     const uint64_t gwgs = ((uint64_t)gn + slos::kWaves - 1) / slos::kWaves;
     if (shg + slos::kBlockStaticLds <= 64 * 1024 && gwgs <= 0x7fffffffull) {
@@ -1047,10 +1038,10 @@ void build_tuples(slos_plan* p) {
     for (uint64_t u = 0; u < N5; ++u) p->tuples5[u] = slos::make_tuple_word<5>(sv, p->multiset, u);
     // the last layer in group order: multiset spaces, prefixes of 2..6 positions
     // (item ranks are 32-bit; at most about 32M items a layer)
-    for (int K = p->n - 1; K <= p->n; ++K)
-      if (p->multiset && K - 5 >= 2 && K - 5 <= 6 && p->size(K - 6) <= 0xffffffffull &&
+    for (int K = 7; K <= p->n && K <= 11; ++K)
+      if (p->multiset && p->size(K - 6) <= 0xffffffffull &&
           p->size(K) / slos::kGroupItemChildren + p->size(K - 6) <= (32u << 20))
-        slos::group_items(sv, K, K == p->n ? p->items : p->items_prev);
+        slos::group_items(sv, K, p->items[K]);
   }
 }
 
@@ -1123,8 +1114,8 @@ int slos_plan_destroy(slos_plan* p) {
     (void)hipFree(t.modes);
     if (t.tuples) (void)hipFree(t.tuples);
     if (t.tuples5) (void)hipFree(t.tuples5);
-    if (t.items) (void)hipFree(t.items);
-    if (t.items_prev) (void)hipFree(t.items_prev);
+    for (auto* it : t.items)
+      if (it) (void)hipFree(it);
   }
   (void)hipSetDevice(cur);
   delete p;

@@ -272,8 +272,8 @@ extern "C" {
 
 // Forward pass from the vacuum to layer n; probs[S_n] = |amp * in_norm|^2 * prod t_j!. suf 5 runs
 // the layers with K - 5 in [1, 8] with 5-position suffixes (the device's choice where its table
-// exists), suf 4 every block layer with 4, suf 6 as 5 but the last two layers of a multiset space
-// that have a prefix of 2..6 positions in group order. Returns 0 on success, 2 if some child was not written
+// exists), suf 4 every block layer with 4, suf 6 as 5 but the layers of a multiset space that
+// have a prefix of 2..6 positions (K = 7..11) in group order. Returns 0 on success, 2 if some child was not written
 // exactly once, 3 if a group's blocks do not share what the group kernel shares, 1 on bad input.
 int emu_forward_suf(int n, int m, int mask, const int* modes, const double* U, double in_norm, uint64_t per_wave,
                     int suf, double* probs) {
@@ -297,7 +297,7 @@ int emu_forward_suf(int n, int m, int mask, const int* modes, const double* U, d
           norms[g] = slos::run_factorial(p, K, s.multiset);
         }
       }
-    } else if (suf == 6 && K >= n - 1 && s.multiset && K - 5 >= 2 && K - 5 <= 6) {
+    } else if (suf == 6 && s.multiset && K - 5 >= 2 && K - 5 <= 6) {
       const int d = K - 5;
       auto* nm = last ? &norms : nullptr;
       bool ok;
