@@ -2,21 +2,25 @@
 
     python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batched] [--workload c5|c2]
 
-A step is one full FockSLOSKernel.probs(U) pass (11 photon layers, the last one fused with the
-|amp|^2 * prod t! readout, float32 output) over one seeded Haar unitary already resident in HBM.
-`value` = output probabilities per second (S_11 / step time). The dominant kernel is the last
-layer; its launches are bracketed by HIP events on the stream they run on, and `roofline.achieved`
-= its algorithmic bytes (read layer 10 once: 8 S_10, write the probabilities: 4 S_11) / its mean
-duration. Accuracy of the same output: max|dp| and max relative error over 10,000 seeded outputs
-against FP64 Ryser permanents computed on the GPU, and the FP64 total mass.
-`cpu_baseline`: the oracle's C++/OpenMP pull-form restatement on this host's cores (a bounded
-sample: layers 1..10 complete plus the first 2^27 states of layer 11, the full-run time
-extrapolated linearly in layer-11 states).
+A step is one FockSLOSKernel.probs(U) call (11 photon layers, the last one fused with the
+|amp|^2 * prod t! readout, float32 output) on one seeded Haar unitary already resident in HBM: the
+drop-in op itself, timed with perf_counter around `steps` calls between two device
+synchronisations. `value` = output probabilities per second (S_11 / step time). The dominant
+kernel is the last layer: after the timed steps the same launch (same inputs) runs 5 times
+bracketed by HIP events on the stream it runs on, and `roofline.achieved` = its algorithmic bytes
+(read layer 10 once: 8 S_10, write the probabilities: 4 S_11) / its mean duration. Accuracy of
+the timed output: max|dp| and max relative error over 10,000 seeded outputs against FP64 Ryser
+permanents computed on the GPU, and the FP64 total mass.
+`cpu_baseline`: the oracle's C++/OpenMP pull-form restatement of the same workload on this host's
+cores, every layer complete (no extrapolation). `cpu_reference_algorithm`: the reference's own
+stored-edge algorithm restated in torch on the CPU at n=8, m=24 (its edges do not fit at n=11,
+m=32), beside the GPU engine on the same input.
 `variants`: the compensated-accumulation forward of the same workload; the lossy output
 (probs(U, loss=0.9): the forward with the top binomial weight in its readout and the 11 descent
 steps, with the k=11 descent's time and roofline and the FP64 mass of every photon-number block
 against the binomial law); the gradient step of n=6, m=16 (forward, L = -mean log p over 1,024
-outputs drawn from the exact distribution, dL/dU through the hand-written adjoint); and the
+outputs drawn from the exact distribution, dL/dU through the hand-written adjoint) and the same
+at n=8, m=24 with the time and roofline of its top (block-structured) adjoint layer; and the
 uneven batch of 512 unitaries ((4,12), (6,16), (8,20)) as a single-U loop, per-shape batches, one
 group launch per layer, and the autotuned group forward replayed as a hipGraph; and the components
 built on the path (`next_rows`): device sampling of 10^6 patterns from the n=11, m=32
@@ -188,6 +192,62 @@ def bench_gradient(steps: int = 20) -> dict:
             "ms_per_step": dt * 1e3, "grad_nonzero_columns": int((g.abs().sum(0) > 0).sum())}
 
 
+def bench_adjoint(steps: int = 5) -> dict:
+    """n=8, m=24 (7.9M outputs): one gradient step (probs(U) with U requiring a gradient, L = -mean
+    log p over 1,024 outputs drawn from the exact distribution, L.backward() through the adjoint
+    kernels), and the top adjoint layer (layer 8 -> 7, the block-structured adjoint) alone,
+    bracketed by HIP events on its stream. Its algorithmic bytes: read g_8 and psi_7, write g_7:
+    8 S_8 + 16 S_7."""
+    from photonic_engines_amd import _native as nat
+    from photonic_engines_amd.slos import FockSLOSKernel, haar_unitary
+    n, m = 8, 24
+    k = FockSLOSKernel(n, m, [1] * n + [0] * (m - n), device="cuda")
+    U0 = haar_unitary(m, 8024).to(torch.complex64).cuda()
+    with torch.no_grad():
+        p = k.probs(U0)
+    idx = torch.multinomial(p.double(), 1024, replacement=True, generator=torch.Generator(device="cuda").manual_seed(8))
+
+    def step():
+        U = U0.clone().requires_grad_(True)
+        loss = -torch.log(k.probs(U)[idx]).mean()
+        loss.backward()
+        return U.grad
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    plan, modes = k._plan, k.input_modes
+    S = plan.sizes
+    U = U0.unsqueeze(0).contiguous()
+    psi = torch.ones((1, 1), dtype=torch.complex64, device="cuda")
+    for kk in range(1, n):
+        psi = plan.layer_step(U, kk, modes[kk - 1], psi, nat.OUT_RAW)
+    gk = torch.randn((1, S[n]), dtype=torch.complex64, device="cuda")
+    gU = torch.zeros_like(U)
+    stream = torch.cuda.current_stream()
+    plan.adjoint_layer(U, n, modes[n - 1], psi, gk, gU)
+    ms = []
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        plan.adjoint_layer(U, n, modes[n - 1], psi, gk, gU)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    a_ms = float(np.median(ms))
+    a_bytes = 8 * S[n] + 16 * S[n - 1]
+    return {"workload": "n=8 m=24, first 8 modes, L = -mean log p over 1,024 sampled outputs", "states": int(S[n]),
+            "ms_per_step": dt * 1e3, "grad_nonzero_columns": int((g.abs().sum(0) > 0).sum()),
+            "adjoint_top": {"kernel": "k_adjoint_block (layer 8 -> 7, with dL/dU partials and their FP64 sum)",
+                            "kernel_ms": a_ms, "alg_bytes": a_bytes,
+                            "achieved_GBps": a_bytes / (a_ms * 1e-3) / 1e9,
+                            "frac": a_bytes / (a_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}}
+
+
 def bench_next_rows(kern, U: torch.Tensor) -> dict:
     """Timings of the components built on the path (sampling, x-model, superposition, grouped
     probabilities, backend lookups); their parity is in the GPU tests."""
@@ -255,6 +315,46 @@ def bench_next_rows(kern, U: torch.Tensor) -> dict:
     return out
 
 
+def cpu_stored_edge(threads: int, reps: int = 3) -> dict:
+    """The reference's own algorithm on the CPU, where its edge tables fit: the torch-CPU
+    restatement of the stored-edge mode-slab forward (oracle/stored_edge.py) at n=8, m=24 (7.9M
+    outputs, 63M edges), setup timed apart, then one warm-up and the median of `reps` probs(U)
+    calls; beside it the GPU engine's probs(U) on the same U. At n=11, m=32 its edges alone would
+    take 377 GB, so there it is not run."""
+    from oracle.stored_edge import StoredEdgeSLOS
+    from photonic_engines_amd.slos import FockSLOSKernel, haar_unitary
+    n, m = 8, 24
+    occ = [1] * n + [0] * (m - n)
+    U128 = haar_unitary(m, 8024)
+    torch.set_num_threads(threads)
+    ref = StoredEdgeSLOS(n, m, occ)
+    Uc = U128.to(torch.complex64)
+    ref.probs(Uc)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        pc = ref.probs(Uc)
+        ts.append(time.perf_counter() - t0)
+    t_cpu = float(np.median(ts))
+    kg = FockSLOSKernel(n, m, occ, device="cuda")
+    Ug = Uc.cuda()
+    kg.probs(Ug)
+    torch.cuda.synchronize()
+    gs = []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        pg = kg.probs(Ug)
+        torch.cuda.synchronize()
+        gs.append(time.perf_counter() - t0)
+    t_gpu = float(np.median(gs))
+    S = pc.numel()
+    return {"workload": "n=8 m=24 (7,888,725 outputs), input [1]*8+[0]*16, seed 8024, complex64",
+            "kind": "torch-CPU restatement of the reference's stored-edge mode-slab forward",
+            "cores": threads, "setup_s": ref.setup_s, "cpu_ms": t_cpu * 1e3, "cpu_probs_per_s": S / t_cpu,
+            "gpu_ms": t_gpu * 1e3, "gpu_probs_per_s": S / t_gpu,
+            "max_abs_dp_gpu_vs_cpu": float((pg.cpu() - pc).abs().max())}
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -272,7 +372,10 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-last", type=int, default=1 << 27, help="layer-n states in the CPU sample")
+    ap.add_argument("--cpu-last", type=int, default=0,
+                    help="layer-n states in the CPU sample (0: the whole layer, no extrapolation)")
+    ap.add_argument("--no-cpu-reference-algorithm", action="store_true",
+                    help="skip the torch-CPU stored-edge restatement at n=8, m=24")
     ap.add_argument("--no-compensated", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 Ryser spot checks (profiling passes)")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batch measurement")
@@ -299,36 +402,40 @@ def main() -> None:
     kern = FockSLOSKernel(n, m, occ, device=str(dev))
     plan, modes = kern._plan, kern.input_modes
     U128 = haar_unitary(m, wl["seed"])
-    U = U128.to(torch.complex64).to(dev).unsqueeze(0).contiguous()
+    U2 = U128.to(torch.complex64).to(dev).contiguous()        # [m, m], resident in HBM
+    U = U2.unsqueeze(0)
     S = plan.sizes
-    ones = torch.ones((1, 1), dtype=torch.complex64, device=dev)
     stream = torch.cuda.current_stream(dev)
 
-    def step(flags: int, events=None):
-        psi = ones
-        for k in range(1, n):
-            psi = plan.layer_step(U, k, modes[k - 1], psi, nat.OUT_RAW, flags=flags)
-        if events is not None:
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-        p = plan.layer_step(U, n, modes[n - 1], psi, nat.OUT_PROBS, scale=1.0, flags=flags)
-        if events is not None:
-            e1.record(stream)
-            events.append((e0, e1))
-        return p
-
-    def timed(flags: int):
+    def timed(k):
+        """`args.steps` calls of k.probs(U) (the drop-in op itself), after `args.warmup` untimed."""
         for _ in range(args.warmup):
-            p = step(flags)
+            p = k.probs(U2)
         torch.cuda.synchronize()
-        events = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            p = step(flags, events)
+            p = k.probs(U2)
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
-        last_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
-        return p, dt, last_ms
+        return p, (time.perf_counter() - t0) / args.steps
+
+    def last_layer_ms(flags: int, reps: int = 5) -> float:
+        """The last-layer launch alone (the same kernel probs(U) ends with, on the same inputs):
+        layer n-1 computed once, then `reps` launches of layer n bracketed by HIP events on the
+        stream they run on; the mean duration."""
+        psi = torch.ones((1, 1), dtype=torch.complex64, device=dev)
+        for k in range(1, n):
+            psi = plan.layer_step(U, k, modes[k - 1], psi, nat.OUT_RAW, flags=flags)
+        q = plan.layer_step(U, n, modes[n - 1], psi, nat.OUT_PROBS, scale=1.0, flags=flags)
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            q = plan.layer_step(U, n, modes[n - 1], psi, nat.OUT_PROBS, scale=1.0, flags=flags, out=q)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        del psi, q
+        return float(np.mean(ms))
 
     if args.copy_calibration_gb > 0:
         nbytes = int(args.copy_calibration_gb * 1e9) // 4 * 4
@@ -337,7 +444,7 @@ def main() -> None:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    p, dt, last_ms = timed(0)
+    p, dt = timed(kern)
     S_n = S[n]
     # accuracy of this output: 10,000 seeded outputs against FP64 Ryser on the GPU, FP64 mass
     rng = np.random.default_rng(wl["seed"])
@@ -346,10 +453,13 @@ def main() -> None:
     max_abs = max_rel = mass = None
     if not args.no_accuracy:
         ref = kern.spot_probs(U128, idx_t).cpu().numpy()
-        got = p[0, idx_t].double().cpu().numpy()
+        got = p[idx_t].double().cpu().numpy()
         max_abs = float(np.abs(got - ref).max())
         max_rel = float((np.abs(got - ref) / np.maximum(ref, 1e-300)).max())
-        mass = float(kern.total_mass(p)[0])
+        mass = float(kern.total_mass(p))
+    del p
+    torch.cuda.empty_cache()
+    last_ms = last_layer_ms(0)
 
     last_bytes = 8 * S[n - 1] + 4 * S[n]
     achieved = last_bytes / (last_ms * 1e-3) / 1e9
@@ -368,7 +478,8 @@ def main() -> None:
         "dtype": "c64",
         "data": "synthetic",
         "config": {"workload": f"SLOS fock n={n} m={m}, input [1]*{n}+[0]*{m - n}, one seeded Haar U "
-                               f"(seed {wl['seed']}), lossless, float32 probabilities",
+                               f"(seed {wl['seed']}), lossless, float32 probabilities; a step is one "
+                               f"FockSLOSKernel.probs(U) call",
                    "n": n, "m": m, "states": S_n, "global_batch": 1, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
@@ -381,20 +492,24 @@ def main() -> None:
         "accuracy": {"max_abs_dp": max_abs, "max_rel": max_rel, "spot_outputs": int(idx.size),
                      "oracle": "FP64 Ryser on the GPU", "mass_fp64": mass},
     }
-    del p
     if not args.no_compensated and not args.no_accuracy:
-        pc, dtc, last_c = timed(1)
-        gc = pc[0, idx_t].double().cpu().numpy()
+        kc = FockSLOSKernel(n, m, occ, device=str(dev), compensated=True)
+        pc, dtc = timed(kc)
+        gc = pc[idx_t].double().cpu().numpy()
+        del pc
+        torch.cuda.empty_cache()
+        last_c = last_layer_ms(nat.FLAG_COMPENSATED)
         out["variants"] = {"compensated": {"ms_per_step": dtc * 1e3, "value": S_n / dtc, "kernel_ms": last_c,
                                            "max_abs_dp": float(np.abs(gc - ref).max()),
                                            "max_rel": float((np.abs(gc - ref) / np.maximum(ref, 1e-300)).max())}}
-        del pc
     torch.cuda.empty_cache()
     if not args.no_lossy:
         out.setdefault("variants", {})["lossy"] = bench_lossy(kern, U[0])
         torch.cuda.empty_cache()
     if not args.no_gradient:
         out.setdefault("variants", {})["gradient_n6_m16"] = bench_gradient()
+        out["variants"]["gradient_n8_m24"] = bench_adjoint()
+        torch.cuda.empty_cache()
     if not args.no_next_rows:
         out.setdefault("variants", {})["next_rows"] = bench_next_rows(kern, U[0])
         torch.cuda.empty_cache()
@@ -404,12 +519,17 @@ def main() -> None:
     if not args.no_cpu_baseline:
         from oracle import pull
         threads = len(os.sched_getaffinity(0))
-        t_pre, t_last, done, _ = pull.forward_sample(n, m, occ, U128.numpy(), args.cpu_last, prec=64, threads=threads)
+        last = S_n if args.cpu_last <= 0 else min(args.cpu_last, S_n)
+        t_pre, t_last, done, _ = pull.forward_sample(n, m, occ, U128.numpy(), last, prec=64, threads=threads)
         full = t_pre + t_last * S_n / done
+        how = (f"layers 1..{n} complete ({t_pre + t_last:.2f} s)" if done == S_n else
+               f"layers 1..{n - 1} complete ({t_pre:.2f} s) + first {done} of {S_n} layer-{n} states "
+               f"({t_last:.2f} s), extrapolated to {full:.1f} s")
         out["cpu_baseline"] = {"value": S_n / full, "unit": "probs/s", "cores": threads, "kind": "port",
-                               "sample": f"oracle C++/OpenMP pull form, complex64, layers 1..{n - 1} complete "
-                                         f"({t_pre:.2f} s) + first {done} of {S_n} layer-{n} states "
-                                         f"({t_last:.2f} s), extrapolated to {full:.1f} s; CPU {cpu_model()}"}
+                               "sample": f"oracle C++/OpenMP pull form of the same workload, complex64, {how}; "
+                                         f"CPU {cpu_model()}"}
+        if not args.no_cpu_reference_algorithm:
+            out["cpu_reference_algorithm"] = cpu_stored_edge(threads)
     print(json.dumps(out))
 
 

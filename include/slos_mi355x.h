@@ -64,6 +64,11 @@ enum {
 #define SLOS_FLAG_IMMEDIATE 2u   /* amplitudes in the per-edge sqrt normalisation (kernel.py:239-244):
                                     raw amplitude * sqrt(prod_j t_j!) */
 #define SLOS_FLAG_SIMPLE 4u      /* force the thread-per-state kernel (ablation / debugging) */
+#define SLOS_FLAG_HILO 8u        /* slos_layer_step with SLOS_FLAG_COMPENSATED: the raw layers in
+                                    (psi_prev) and out (kind 2) are hi/lo pairs [2][batch][S], the
+                                    high parts then the low-order parts; the low parts enter every
+                                    child sum and the stored layer keeps its rounding error
+                                    (slos_forward with SLOS_FLAG_COMPENSATED does this internally) */
 
 /* Version string of the library. */
 const char* slos_version(void);
@@ -99,8 +104,9 @@ int slos_unrank(const slos_plan* plan, int k, const uint64_t* idx, int64_t count
 int slos_prob_norms(const slos_plan* plan, int k, uint64_t first, int64_t count, double* out);
 
 /* Device bytes of the workspace slos_forward needs for `batch` unitaries when it stops at layer
- * `layer` (layer = n for the full run). */
-int slos_forward_workspace(const slos_plan* plan, int64_t batch, int layer, size_t* bytes);
+ * `layer` (layer = n for the full run) with these flags (SLOS_FLAG_COMPENSATED doubles it: the
+ * intermediate layers then carry their low-order parts). */
+int slos_forward_workspace(const slos_plan* plan, int64_t batch, int layer, unsigned flags, size_t* bytes);
 
 /* The layer recurrence psi_{k+1}[c] = sum_{j distinct in c} U[b, j, i_{k+1}] psi_k[c - e_j]
  * (deferred normalisation), run from the vacuum to layer `layer` with input columns
@@ -205,7 +211,7 @@ int slos_unrank_device(const slos_plan* plan, int k, const uint64_t* idx, int64_
  * [batch[g]][S_n] (device), scale[g] the input norm 1/sqrt(prod s_i!). Plans must outlive the group. */
 int slos_group_create(int count, const slos_plan* const* plans, slos_group** out);
 int slos_group_destroy(slos_group* group);
-int slos_group_workspace(const slos_group* group, const int64_t* batch, size_t* bytes);
+int slos_group_workspace(const slos_group* group, const int64_t* batch, unsigned flags, size_t* bytes);
 int slos_group_forward(slos_group* group, const void* const* U, const int64_t* batch, float* const* out,
                        const double* scale, unsigned flags, void* workspace, size_t workspace_bytes,
                        slos_stream_t stream);

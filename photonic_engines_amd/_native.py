@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("SLOS_MI355X_LIB") or os.path.join(_HERE, "_build", "l
 SLOS_OK, SLOS_EINVAL, SLOS_ENOMEM, SLOS_EHIP, SLOS_ERANGE, SLOS_ENODEV = range(6)
 MASK_IDS = {"fock": 0, "unbunched": 1, "dual_rail": 2, "sparse_1in4": 3, "tail_empty": 4}
 OUT_PROBS, OUT_AMPS, OUT_RAW = 0, 1, 2
-FLAG_COMPENSATED, FLAG_IMMEDIATE, FLAG_SIMPLE = 1, 2, 4
+FLAG_COMPENSATED, FLAG_IMMEDIATE, FLAG_SIMPLE, FLAG_HILO = 1, 2, 4, 8
 
 # every symbol include/slos_mi355x.h declares: (name, restype, argtypes)
 _c = ctypes
@@ -32,7 +32,7 @@ SIGNATURES = {
     "slos_rank": (_c.c_int, [_vp, _c.c_int, _i32p, _c.c_int64, _u64p]),
     "slos_unrank": (_c.c_int, [_vp, _c.c_int, _u64p, _c.c_int64, _i32p]),
     "slos_prob_norms": (_c.c_int, [_vp, _c.c_int, _c.c_uint64, _c.c_int64, _f64p]),
-    "slos_forward_workspace": (_c.c_int, [_vp, _c.c_int64, _c.c_int, _c.POINTER(_c.c_size_t)]),
+    "slos_forward_workspace": (_c.c_int, [_vp, _c.c_int64, _c.c_int, _c.c_uint, _c.POINTER(_c.c_size_t)]),
     "slos_forward": (_c.c_int, [_vp, _vp, _c.c_int64, _i32p, _c.c_int, _c.c_int, _vp, _c.c_double, _c.c_uint,
                                 _vp, _c.c_size_t, _vp]),
     "slos_layer_step": (_c.c_int, [_vp, _vp, _c.c_int64, _c.c_int, _c.c_int, _vp, _c.c_int, _vp, _c.c_double,
@@ -52,7 +52,7 @@ SIGNATURES = {
     "slos_unrank_device": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_int64, _vp, _vp]),
     "slos_group_create": (_c.c_int, [_c.c_int, _c.POINTER(_vp), _c.POINTER(_vp)]),
     "slos_group_destroy": (_c.c_int, [_vp]),
-    "slos_group_workspace": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_size_t)]),
+    "slos_group_workspace": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.c_uint, _c.POINTER(_c.c_size_t)]),
     "slos_group_forward": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_c.c_int64), _c.POINTER(_vp), _f64p,
                                       _c.c_uint, _vp, _c.c_size_t, _vp]),
     "slos_group_tune": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_c.c_int64), _c.POINTER(_vp), _f64p,

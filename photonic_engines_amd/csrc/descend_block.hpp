@@ -131,12 +131,14 @@ __device__ __forceinline__ void descend_step2(const DescCtx& c, uint32_t uc) {
   __builtin_memcpy(c.qout + c.CBp + u, &o, 8);
 }
 
-// ME: a compile-time bound on the mode count (me <= ME <= 64), so that the mode loops unroll and
-// every child load of a step is issued before the first sum.
-template <int MAXD, int ME>
-__device__ __forceinline__ void descend_wave(const BlockTables& tg, const BlockTables& tl, int K, const float* __restrict__ qk,
-                                             float* __restrict__ qout, uint64_t Sp, uint64_t S_pre, uint64_t per_wave,
-                                             uint32_t* tbl, uint64_t g, float scale) {
+// The block walk of the pulls over children: the wave's parents [g, g + per_wave) of layer K-1,
+// block by block (64 blocks' data computed at once into the wave's table `tbl`, then read back one
+// block at a time); for each block, sweep(c, u0, u_stop) runs over the block's parents
+// [u0, u_stop) with the block's child-index context c.
+template <int MAXD, class Sweep>
+__device__ __forceinline__ void desc_block_walk(const BlockTables& tg, const BlockTables& tl, int K, const float* qk,
+                                                float* qout, uint64_t Sp, uint64_t S_pre, uint64_t per_wave,
+                                                uint32_t* tbl, uint64_t g, float scale, Sweep&& sweep) {
   constexpr int kPW = (MAXD + 3) / 4;
   constexpr int kBW = 7 + 2 * kPW;
   const int lane = threadIdx.x & 63;
@@ -254,14 +256,25 @@ __device__ __forceinline__ void descend_wave(const BlockTables& tg, const BlockT
       const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
       DescCtx c{qk, qout, static_cast<const TupleWord*>(tg.tuples) + toff, cgl, CBp, CB0, sb, scnt, tA0, tA1, tA2, tA3, u_stop, W, me, lastP, lastrun,
                 lane, invk, scale};
-      uint32_t uc = (uint32_t)u0;
-      for (; uc + 128u <= u_stop; uc += 128u) descend_step2<ME>(c, uc);
-      for (; uc < u_stop; uc += 64u) descend_step1<ME>(c, uc);
+      sweep(c, (uint32_t)u0, u_stop);
       g += u_stop - u0;
       u0 = 0;
     }
     r += kBlockBatch;
   }
+}
+
+// ME: a compile-time bound on the mode count (me <= ME <= 64), so that the mode loops unroll and
+// every child load of a step is issued before the first sum.
+template <int MAXD, int ME>
+__device__ __forceinline__ void descend_wave(const BlockTables& tg, const BlockTables& tl, int K, const float* __restrict__ qk,
+                                             float* __restrict__ qout, uint64_t Sp, uint64_t S_pre, uint64_t per_wave,
+                                             uint32_t* tbl, uint64_t g, float scale) {
+  desc_block_walk<MAXD>(tg, tl, K, qk, qout, Sp, S_pre, per_wave, tbl, g, scale,
+                        [](const DescCtx& c, uint32_t uc, uint32_t u_stop) {
+                          for (; uc + 128u <= u_stop; uc += 128u) descend_step2<ME>(c, uc);
+                          for (; uc < u_stop; uc += 64u) descend_step1<ME>(c, uc);
+                        });
 }
 
 template <int MAXD, int ME>
@@ -285,6 +298,137 @@ __global__ __launch_bounds__(kBlockThreads) void k_descend_block(const uint64_t*
   const uint64_t g = uniform_u64(((uint64_t)xcd_chunk_index(blockIdx.x, xcd_chunk) * kWaves + wv) * per_wave);
   uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + desc_table_offset(K, me)) + (size_t)wv * kBlockBatch * desc_words(MAXD);
   descend_wave<MAXD, ME>(tg, tl, K, qk_all + (size_t)b * Sk, qout_all + (size_t)b * Sp, Sp, S_pre, per_wave, tbl, g, scale);
+}
+
+
+// ------------------------------------------------------------------------------------------
+// Adjoint of one forward layer on the same block structure (the backward of the recurrence,
+// photonic_engines/slos/kernel.py:63-80, as a pull over children): for every parent t of layer
+// K-1 and every mode a, the child t + e_a is an edge with coefficient U[a, col], so
+//   grad_prev[t] = sum_a conj(U[a, col]) g_K[t + e_a]          (ascending a)
+//   dL/dU[a, col] += sum_t conj(psi_{K-1}[t]) g_K[t + e_a].
+// One parent per lane; the children are found exactly as in the loss descent. The U-gradient is
+// kept per lane and per mode in registers (mode a is wave-uniform in the unrolled mode loop),
+// summed over the wave's lanes in FP64 and over the work group's waves, and written as one FP64
+// partial per work group and mode; a second kernel sums the partials in FP64. No atomics.
+// ------------------------------------------------------------------------------------------
+struct AdjCtx {
+  const float2* gk;     // g_K (child layer)
+  float2* gout;         // grad_prev (parent layer), or nullptr
+  const float2* psi;    // psi_{K-1} (parent layer)
+  uint32_t ure, uim;    // lane a: U[a, col]
+};
+
+template <int ME>
+__device__ __forceinline__ void adjoint_step1(const DescCtx& c, const AdjCtx& x, uint32_t uc, float2 (&gacc)[ME]) {
+  const uint32_t u = uc + (uint32_t)c.lane;
+  const bool in = u < c.u_stop;
+  const uint32_t uu = in ? u : c.u_stop - 1u;
+  const TupleWord w = c.tb[uu];
+  const int sx[kSuf] = {(int)(w & 63u), (int)((w >> 8) & 63u), (int)((w >> 16) & 63u), (int)((w >> 24) & 63u)};
+  DescLane dl;
+  desc_lane_init(c.cgl, c.W, c.lastP, sx, dl);
+  const float2 ps = in ? x.psi[c.CBp + uu] : make_float2(0.f, 0.f);  // lanes past the block add nothing
+  float gr = 0.f, gi = 0.f;
+  constexpr int kChunk = 8;
+#pragma unroll
+  for (int a0 = 0; a0 < ME; a0 += kChunk) {
+    float2 gv[kChunk];
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      const int a = a0 + i;
+      gv[i] = make_float2(0.f, 0.f);
+      if (a < c.me) {
+        const uint64_t sba = rl64(c.sb, a);
+        uint64_t idx;
+        if (a < c.lastP) {
+          idx = sba + uu;
+        } else {
+          const uint32_t loc = desc_local_offset(dl, sx, a, rl(c.tA0, a), rl(c.tA1, a), rl(c.tA2, a), rl(c.tA3, a));
+          idx = a <= sx[0] ? sba + uu : c.CB0 + loc;
+        }
+        gv[i] = x.gk[idx];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      const int a = a0 + i;
+      if (a < c.me) {
+        const float ur = __builtin_bit_cast(float, rl(x.ure, a)), ui = __builtin_bit_cast(float, rl(x.uim, a));
+        gr = fmaf(ur, gv[i].x, gr);   // conj(u) * g
+        gr = fmaf(ui, gv[i].y, gr);
+        gi = fmaf(ur, gv[i].y, gi);
+        gi = fmaf(-ui, gv[i].x, gi);
+        gacc[a].x = fmaf(ps.x, gv[i].x, gacc[a].x);   // conj(psi) * g
+        gacc[a].x = fmaf(ps.y, gv[i].y, gacc[a].x);
+        gacc[a].y = fmaf(ps.x, gv[i].y, gacc[a].y);
+        gacc[a].y = fmaf(-ps.y, gv[i].x, gacc[a].y);
+      }
+    }
+  }
+  if (x.gout && in) x.gout[c.CBp + u] = make_float2(gr, gi);
+}
+
+// partial[b][work group][a] (double2), the U-gradient sums of the work group's parents
+template <int MAXD, int ME>
+__global__ __launch_bounds__(kBlockThreads) void k_adjoint_block(const uint64_t* __restrict__ cg_t, const int* __restrict__ lo_t,
+                                                                 const TupleWord* __restrict__ tuples, int me, int stride,
+                                                                 int K, int m, int col, const float2* __restrict__ U,
+                                                                 const float2* __restrict__ psi_all,
+                                                                 const float2* __restrict__ gk_all, uint64_t Sk,
+                                                                 float2* __restrict__ gout_all, uint64_t Sp, uint64_t S_pre,
+                                                                 uint64_t per_wave, uint32_t xcd_chunk,
+                                                                 double2* __restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double2 wsum[kWaves][ME];
+  const int ncg = (K + 1) * stride;
+  uint64_t* cg_l = reinterpret_cast<uint64_t*>(smem);
+  int* lo_l = reinterpret_cast<int*>(cg_l + ncg);
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg_l[i] = cg_t[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo_l[i] = lo_t[i];
+  __syncthreads();
+  const BlockTables tg{cg_t, lo_t, tuples, me, stride, 1};
+  const BlockTables tl{cg_l, lo_l, tuples, me, stride, 1};
+  const int64_t b = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int wv = (int)rfl(threadIdx.x >> 6);
+  AdjCtx x;
+  x.gk = gk_all + (size_t)b * Sk;
+  x.gout = gout_all ? gout_all + (size_t)b * Sp : nullptr;
+  x.psi = psi_all + (size_t)b * Sp;
+  const float2 ua = lane < m ? U[(size_t)b * m * m + (size_t)lane * m + col] : make_float2(0.f, 0.f);
+  x.ure = __builtin_bit_cast(uint32_t, ua.x);
+  x.uim = __builtin_bit_cast(uint32_t, ua.y);
+  float2 gacc[ME];
+#pragma unroll
+  for (int a = 0; a < ME; ++a) gacc[a] = make_float2(0.f, 0.f);
+  const uint64_t g = uniform_u64(((uint64_t)xcd_chunk_index(blockIdx.x, xcd_chunk) * kWaves + wv) * per_wave);
+  uint32_t* tbl = reinterpret_cast<uint32_t*>(smem + desc_table_offset(K, me)) + (size_t)wv * kBlockBatch * desc_words(MAXD);
+  desc_block_walk<MAXD>(tg, tl, K, nullptr, nullptr, Sp, S_pre, per_wave, tbl, g, 1.f,
+                        [&](const DescCtx& c, uint32_t uc, uint32_t u_stop) {
+                          for (; uc < u_stop; uc += 64u) adjoint_step1<ME>(c, x, uc, gacc);
+                        });
+  // wave sums in FP64 (butterfly over the 64 lanes), then the work group's four waves in order
+#pragma unroll
+  for (int a = 0; a < ME; ++a) {
+    double sr = (double)gacc[a].x, si = (double)gacc[a].y;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      sr += __shfl_xor(sr, o, 64);
+      si += __shfl_xor(si, o, 64);
+    }
+    if (lane == 0) wsum[wv][a] = make_double2(sr, si);
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < me; a += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      sr += wsum[w][a].x;
+      si += wsum[w][a].y;
+    }
+    partial[((size_t)b * gridDim.x + blockIdx.x) * me + a] = make_double2(sr, si);
+  }
 }
 
 }  // namespace slos

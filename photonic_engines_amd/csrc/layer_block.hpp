@@ -34,6 +34,9 @@ struct BlockLayer {
   unsigned flags;
   uint64_t per_wave;     // children per wave
   uint32_t xcd_chunk;    // work groups per XCD chunk (see k_layer_block); 1: plain launch order
+  // compensated sums only: element offsets of the low-order parts of the parent layer and of a
+  // complex64 output layer from their high parts (0: none; a hi/lo layer is [2][B][S])
+  int64_t par_lo, out_lo;
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -65,7 +68,10 @@ struct BlockAcc {
   float re, im, cre, cim;
 };
 
+// Compensated arithmetic must not be contracted: a product fused into the next addition would make
+// the two-sum's error terms wrong, so these bodies turn contraction off.
 __device__ __forceinline__ void bk_two_sum(float& s, float& c, float x) {
+#pragma clang fp contract(off)
   const float t = s + x;
   const float bp = t - s;
   c += (s - (t - bp)) + (x - bp);
@@ -81,12 +87,21 @@ __device__ __forceinline__ void bk_cmac(BlockAcc& a, float2 u, float2 v) {
     a.im = fmaf(u.x, v.y, a.im);
     a.im = fmaf(u.y, v.x, a.im);
   } else {
+#pragma clang fp contract(off)
     float p, e;
     p = u.x * v.x; e = fmaf(u.x, v.x, -p); bk_two_sum(a.re, a.cre, p); a.cre += e;
     p = -u.y * v.y; e = fmaf(-u.y, v.y, -p); bk_two_sum(a.re, a.cre, p); a.cre += e;
     p = u.x * v.y; e = fmaf(u.x, v.y, -p); bk_two_sum(a.im, a.cim, p); a.cim += e;
     p = u.y * v.x; e = fmaf(u.y, v.x, -p); bk_two_sum(a.im, a.cim, p); a.cim += e;
   }
+}
+
+// the low-order part w of a hi/lo parent value: acc.c += u * w (plain products into the correction)
+__device__ __forceinline__ void bk_lo_mac(BlockAcc& a, float2 u, float2 w) {
+  a.cre = fmaf(u.x, w.x, a.cre);
+  a.cre = fmaf(-u.y, w.y, a.cre);
+  a.cim = fmaf(u.x, w.y, a.cim);
+  a.cim = fmaf(u.y, w.x, a.cim);
 }
 
 // One row of the per-launch suffix table (LDS, kSufRows rows, row = tuple-word byte): the U
@@ -157,6 +172,7 @@ struct ChunkCtx {
   float scale;
   unsigned flags;
   int kind;  // KIND 3 only: 1 probabilities, 0 raw amplitudes
+  int64_t plo = 0, olo = 0;  // compensated sums: low-order parts of the parents / the output (elements)
 };
 
 // the suffix edges of one child from its tuple word: coefficients c[j] (edge 0 of a tau that
@@ -224,12 +240,12 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD, SUF>& cx, uint32_t uc
 #pragma unroll
     for (int s = 0; s < MAXD; ++s) sv[c][s] = at_bytes(cx.sp[s], u[c] * 8u);
   float2 lc[NC][SUF], lv[NC][SUF];
+  uint32_t loff[NC][SUF];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    uint32_t off[SUF];
-    suffix_edges(cx, tw[c], lc[c], off);
+    suffix_edges(cx, tw[c], lc[c], loff[c]);
 #pragma unroll
-    for (int jj = 0; jj < SUF; ++jj) lv[c][jj] = at_bytes(cx.lp, off[jj]);
+    for (int jj = 0; jj < SUF; ++jj) lv[c][jj] = at_bytes(cx.lp, loff[c][jj]);
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -238,9 +254,24 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD, SUF>& cx, uint32_t uc
     for (int s = 0; s < MAXD; ++s) bk_cmac<COMP>(acc, cx.coef[s], sv[c][s]);
 #pragma unroll
     for (int jj = 0; jj < SUF; ++jj) bk_cmac<COMP>(acc, lc[c][jj], lv[c][jj]);
+    if constexpr (COMP) {
+      if (cx.plo) {  // the parents' low-order parts, into the correction terms (plain products)
+#pragma unroll
+        for (int s = 0; s < MAXD; ++s) bk_lo_mac(acc, cx.coef[s], at_bytes(cx.sp[s] + cx.plo, u[c] * 8u));
+#pragma unroll
+        for (int jj = 0; jj < SUF; ++jj) bk_lo_mac(acc, lc[c][jj], at_bytes(cx.lp + cx.plo, loff[c][jj]));
+      }
+    }
     float re = acc.re + acc.cre, im = acc.im + acc.cim;
     if (KIND == 0 || (KIND == 3 && cx.kind == 0)) {
       if (in[c]) *reinterpret_cast<float2*>(cx.ob + u[c] * 8u) = make_float2(re, im);
+      if constexpr (COMP) {
+        // the rounding error of the stored value (fast two-sum: |acc.re| >= |acc.cre| as the
+        // correction is the accumulated rounding error of the sum)
+        if (cx.olo && in[c])
+          *reinterpret_cast<float2*>(cx.ob + u[c] * 8u + cx.olo * 8) =
+              make_float2(acc.cre - (re - acc.re), acc.cim - (im - acc.im));
+      }
     } else {
       const float f = child_norm_w(cx, tw[c]);
       re *= cx.scale;
@@ -375,6 +406,7 @@ struct BlockSweep {
   float scale;
   unsigned flags;
   int kind;
+  int64_t plo, olo;
 };
 
 // Children [u0, u_stop) of one block with its first NS stream slots of the block table column `bj`
@@ -394,7 +426,7 @@ __device__ __forceinline__ void block_children(const uint32_t* bj, const float2*
   }
   ChunkCtx<NS, SUF> cx{sp,       coef,    bs.lp,       static_cast<const TupleWordT<SUF>*>(bs.tb),
                        bs.ob,    bs.rows, bs.rn,       bs.C0,    bs.u_stop, bs.lane, bs.fQ, bs.lastrun,
-                       bs.skip0, bs.lo,   bs.multiset, bs.scale, bs.flags,  bs.kind};
+                       bs.skip0, bs.lo,   bs.multiset, bs.scale, bs.flags,  bs.kind, bs.plo, bs.olo};
   const uint32_t u_stop = bs.u_stop;
   uint32_t uc = bs.u0;
   if constexpr (!COMP) {
@@ -522,7 +554,8 @@ __device__ __forceinline__ void block_wave(const BlockTables& tg, const BlockTab
       const uint64_t avail = g_end - g;
       const uint32_t u_stop = (uint32_t)(((uint64_t)L - u0 < avail) ? (uint64_t)L : u0 + avail);
       const BlockSweep bs{par + LBp, tuples + toff, reinterpret_cast<char*>(out) + CB * (probs ? 4 : 8), rows, rn, C0,
-                          (uint32_t)u0, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale, a.flags, kind};
+                          (uint32_t)u0, u_stop, lane, fQ, lastrun, skip0, blo, multiset, a.scale, a.flags, kind,
+                          COMP ? a.par_lo : 0, COMP ? a.out_lo : 0};
       // one sweep per count of active prefix streams, so inactive slots cost no loads or FMAs
       if constexpr (MAXD > 8) {
         block_children<KIND, COMP, MAXD, MAXD, SUF>(bj, par, bs);
@@ -870,6 +903,7 @@ struct GroupItem {
   unsigned flags;
   int me, multiset, m, col, kind;
   uint32_t wg_begin, wgs_per_inst;
+  int64_t par_lo, out_lo;      // compensated sums: element offsets of the low-order parts (0: none)
 };
 
 struct GroupLaunch {
@@ -932,6 +966,7 @@ k_layer_block_group(GroupLaunch gl) {
     BlockLayer a;
     a.K = K; a.m = it.m; a.col = it.col; a.U = it.U; a.S_par = it.S_par; a.S = it.S; a.S_pre = it.S_pre;
     a.scale = it.scale; a.flags = it.flags; a.per_wave = it.per_wave; a.xcd_chunk = 1;
+    a.par_lo = it.par_lo; a.out_lo = it.out_lo;
     const uint64_t g = uniform_u64(((uint64_t)w * kWaves + wv) * it.per_wave);
     const float2* par = it.par + (size_t)b * it.S_par;
     char* out = reinterpret_cast<char*>(it.out) + (size_t)b * it.S * (it.kind == 1 ? 4 : 8);

@@ -116,7 +116,9 @@ struct Acc {
   float re = 0.f, im = 0.f, cre = 0.f, cim = 0.f;
 };
 
+// (compensated arithmetic: no contraction, see bk_two_sum)
 __device__ __forceinline__ void two_sum(float& s, float& c, float x) {
+#pragma clang fp contract(off)
   const float t = s + x;
   const float bp = t - s;
   c += (s - (t - bp)) + (x - bp);
@@ -132,6 +134,7 @@ __device__ __forceinline__ void cmac(Acc& a, float2 u, float2 v) {
     a.im = fmaf(u.x, v.y, a.im);
     a.im = fmaf(u.y, v.x, a.im);
   } else {
+#pragma clang fp contract(off)
     float p, e;
     p = u.x * v.x; e = fmaf(u.x, v.x, -p); two_sum(a.re, a.cre, p); a.cre += e;
     p = -u.y * v.y; e = fmaf(-u.y, v.y, -p); two_sum(a.re, a.cre, p); a.cre += e;
@@ -153,6 +156,7 @@ struct LayerArgs {
   int64_t B;
   float scale;
   unsigned flags;
+  int64_t par_lo, out_lo;  // compensated sums: element offsets of the low-order parts (0: none)
 };
 
 // KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes
@@ -179,9 +183,24 @@ __global__ __launch_bounds__(256) void k_layer_simple(LayerArgs a) {
       Acc acc;
       slos::for_each_parent(sv, a.multiset, K, g,
                             [&](uint64_t pr, int mode, int) { cmac<COMP>(acc, ucol[mode], pb[pr]); });
+      if constexpr (COMP) {
+        if (a.par_lo)  // the parents' low-order parts, into the correction terms
+          slos::for_each_parent(sv, a.multiset, K, g, [&](uint64_t pr, int mode, int) {
+            const float2 u = ucol[mode], w = pb[a.par_lo + pr];
+            acc.cre = fmaf(u.x, w.x, acc.cre);
+            acc.cre = fmaf(-u.y, w.y, acc.cre);
+            acc.cim = fmaf(u.x, w.y, acc.cim);
+            acc.cim = fmaf(u.y, w.x, acc.cim);
+          });
+      }
       float re = acc.re + acc.cre, im = acc.im + acc.cim;
       if constexpr (KIND == 0) {
         reinterpret_cast<float2*>(a.out)[(size_t)b * a.S + g] = make_float2(re, im);
+        if constexpr (COMP) {
+          if (a.out_lo)  // the rounding error of the stored value (fast two-sum)
+            reinterpret_cast<float2*>(a.out)[a.out_lo + (size_t)b * a.S + g] =
+                make_float2(acc.cre - (re - acc.re), acc.cim - (im - acc.im));
+        }
       } else {
         int p[kMaxPhotons];
         sv.unrank(g, K, p);
@@ -512,6 +531,7 @@ void launch_block_kind(int maxd, int suf, dim3 grid, size_t sh, hipStream_t st, 
 int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, int kind, bool comp, hipStream_t st) {
   slos::BlockLayer L;
   L.K = a.K; L.m = a.m; L.col = a.col; L.U = a.U; L.S_par = a.S_par; L.S = a.S;
+  L.par_lo = a.par_lo; L.out_lo = a.out_lo;
   const int suf = block_suffix(p, dt, a.K);
   L.S_pre = p->size(a.K - suf);
   L.scale = a.scale; L.flags = a.flags;
@@ -568,14 +588,18 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   return SLOS_OK;
 }
 
+// par_hilo / out_hilo (compensated sums only): the parent layer / the complex64 output layer is a
+// hi/lo pair [2][B][S], the low-order parts after the high ones
 int launch_layer(const slos_plan* p, const DevTables& dt, int K, int col, const float2* U, int64_t B,
                  const float2* par, uint64_t S_par, void* out, uint64_t S, int kind, float scale,
-                 unsigned flags, hipStream_t st) {
+                 unsigned flags, hipStream_t st, bool par_hilo = false, bool out_hilo = false) {
   LayerArgs a;
   a.cg = dt.cg; a.lo = dt.lo; a.me = p->me; a.multiset = p->multiset; a.K = K; a.m = p->m; a.col = col;
   a.U = U; a.cols = nullptr; a.par = par; a.S_par = S_par; a.out = out; a.S = S; a.B = B;
   a.scale = scale; a.flags = flags;
   const bool comp = flags & SLOS_FLAG_COMPENSATED;
+  a.par_lo = (comp && par_hilo) ? B * (int64_t)S_par : 0;
+  a.out_lo = (comp && out_hilo && kind == 0) ? B * (int64_t)S : 0;
   if (!(flags & SLOS_FLAG_SIMPLE) && dt.tuples && K > slos::kSuf && K - slos::kSuf <= 16 &&
       block_lds_fit(K, p->me, slos::kSuf))
     return launch_block(p, dt, a, kind, comp, st);
@@ -601,11 +625,14 @@ struct WsLayout {
   size_t ones = 0, odd = 0, even = 0, total = 0;
 };
 
-int forward_layout(const slos_plan* p, int64_t B, int layer, WsLayout* w) {
+// hilo: the intermediate layers are hi/lo pairs [2][B][S] (compensated sums carry their rounding
+// errors from layer to layer), twice the bytes
+int forward_layout(const slos_plan* p, int64_t B, int layer, WsLayout* w, bool hilo = false) {
   uint64_t maxodd = 0, maxeven = 0;
   for (int k = 1; k < layer; ++k) (k & 1 ? maxodd : maxeven) = std::max(k & 1 ? maxodd : maxeven, p->size(k));
   uint64_t a = 0, b = 0;
-  if (mul_overflow(maxodd, (uint64_t)B * 8, &a) || mul_overflow(maxeven, (uint64_t)B * 8, &b))
+  const uint64_t per = (uint64_t)B * 8 * (hilo ? 2 : 1);
+  if (mul_overflow(maxodd, per, &a) || mul_overflow(maxeven, per, &b))
     return fail(SLOS_ERANGE, "workspace size overflows");
   w->ones = 0;
   w->odd = align256((size_t)B * 8);
@@ -805,6 +832,16 @@ __global__ __launch_bounds__(256) void k_layer_simple_group(slos::GroupLaunch gl
       Acc acc;
       slos::for_each_parent(sv, it.multiset, K, g,
                             [&](uint64_t pr, int mode, int) { cmac<COMP>(acc, ucol[mode], pb[pr]); });
+      if constexpr (COMP) {
+        if (it.par_lo)  // the parents' low-order parts, into the correction terms
+          slos::for_each_parent(sv, it.multiset, K, g, [&](uint64_t pr, int mode, int) {
+            const float2 u = ucol[mode], v = pb[it.par_lo + pr];
+            acc.cre = fmaf(u.x, v.x, acc.cre);
+            acc.cre = fmaf(-u.y, v.y, acc.cre);
+            acc.cim = fmaf(u.x, v.y, acc.cim);
+            acc.cim = fmaf(u.y, v.x, acc.cim);
+          });
+      }
       float re = acc.re + acc.cre, im = acc.im + acc.cim;
       if (it.kind == 1) {
         int pos[kMaxPhotons];
@@ -815,6 +852,11 @@ __global__ __launch_bounds__(256) void k_layer_simple_group(slos::GroupLaunch gl
         reinterpret_cast<float*>(it.out)[(size_t)b * it.S + g] = slos::prob_readout(re, im, f);
       } else {
         reinterpret_cast<float2*>(it.out)[(size_t)b * it.S + g] = make_float2(re, im);
+        if constexpr (COMP) {
+          if (it.out_lo)
+            reinterpret_cast<float2*>(it.out)[it.out_lo + (size_t)b * it.S + g] =
+                make_float2(acc.cre - (re - acc.re), acc.cim - (im - acc.im));
+        }
       }
     }
   });
@@ -859,14 +901,16 @@ struct GroupWs {
   size_t total = 0;
 };
 
-int group_layout(const slos_group* G, const int64_t* B, GroupWs* w) {
+// hilo (compensated sums): every plan's intermediate layers are hi/lo pairs [2][B][S]
+int group_layout(const slos_group* G, const int64_t* B, GroupWs* w, bool hilo) {
   size_t off = 0;
   for (size_t g = 0; g < G->plans.size(); ++g) {
     const slos_plan* p = G->plans[g];
     if (B[g] < 1 || B[g] > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
     uint64_t maxS = 0, bytes = 0;
     for (int k = 1; k < p->n; ++k) maxS = std::max(maxS, p->size(k));
-    if (mul_overflow(maxS, (uint64_t)B[g] * 8, &bytes)) return fail(SLOS_ERANGE, "workspace size overflows");
+    if (mul_overflow(maxS, (uint64_t)B[g] * 8 * (hilo ? 2 : 1), &bytes))
+      return fail(SLOS_ERANGE, "workspace size overflows");
     w->ones.push_back(off);
     off += align256((size_t)B[g] * 8);
     w->buf0.push_back(off);
@@ -881,7 +925,7 @@ int group_layout(const slos_group* G, const int64_t* B, GroupWs* w) {
 int group_args(const slos_group* G, const void* const* U, const int64_t* B, float* const* out, const double* scale,
                unsigned flags, void* ws, size_t ws_bytes, GroupArgs* a, GroupWs* w) {
   if (!G || !U || !B || !out || !scale) return fail(SLOS_EINVAL, "NULL argument");
-  int rc = group_layout(G, B, w);
+  int rc = group_layout(G, B, w, (flags & SLOS_FLAG_COMPENSATED) != 0);
   if (rc) return rc;
   if (!ws || ws_bytes < w->total)
     return fail(SLOS_ENOMEM, "workspace too small: need " + std::to_string(w->total) + " bytes");
@@ -955,6 +999,10 @@ int group_layer(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w
     it.kind = last ? 1 : 0;
     it.scale = last ? a.scale[g] : 1.f;
     it.flags = a.flags;
+    // compensated sums carry hi/lo layers (the vacuum row of layer 0 has no low part)
+    const bool hilo = (a.flags & SLOS_FLAG_COMPENSATED) != 0;
+    it.par_lo = (hilo && K >= 2) ? a.B[g] * (int64_t)it.S_par : 0;
+    it.out_lo = (hilo && !last) ? a.B[g] * (int64_t)it.S : 0;
     it.me = p->me;
     it.multiset = p->multiset;
     it.m = p->m;
@@ -1180,37 +1228,37 @@ int slos_prob_norms(const slos_plan* p, int k, uint64_t first, int64_t count, do
   return SLOS_OK;
 }
 
-int slos_forward_workspace(const slos_plan* p, int64_t batch, int layer, size_t* bytes) {
+int slos_forward_workspace(const slos_plan* p, int64_t batch, int layer, unsigned flags, size_t* bytes) {
   if (!p || !bytes) return fail(SLOS_EINVAL, "NULL argument");
   if (batch < 1) return fail(SLOS_EINVAL, "batch must be >= 1");
   if (layer < 1 || layer > p->n) return fail(SLOS_EINVAL, "layer out of range");
   WsLayout w;
-  int rc = forward_layout(p, batch, layer, &w);
+  int rc = forward_layout(p, batch, layer, &w, (flags & SLOS_FLAG_COMPENSATED) != 0);
   if (rc) return rc;
   *bytes = w.total;
   return SLOS_OK;
 }
 
-int slos_forward(const slos_plan* p, const void* U, int64_t batch, const int32_t* modes, int layer, int out_kind,
-                 void* out, double scale, unsigned flags, void* ws, size_t ws_bytes, slos_stream_t stream) {
+namespace {
+
+// argument checks of slos_forward: the input columns and the workspace layout
+int forward_check(const slos_plan* p, const void* U, int64_t batch, const int32_t* modes, int layer, int out_kind,
+                  void* out, unsigned flags, int* cols, WsLayout* w) {
   if (!p || !U || !out) return fail(SLOS_EINVAL, "NULL argument");
   if (batch < 1) return fail(SLOS_EINVAL, "batch must be >= 1");
   if (layer < 1 || layer > p->n) return fail(SLOS_EINVAL, "layer out of range");
   if (out_kind != SLOS_OUT_PROBS && out_kind != SLOS_OUT_AMPS) return fail(SLOS_EINVAL, "unknown output kind");
-  std::vector<int> cols(layer);
   for (int k = 0; k < layer; ++k) {
     cols[k] = modes ? modes[k] : p->modes[k];
     if (cols[k] < 0 || cols[k] >= p->m) return fail(SLOS_EINVAL, "input mode out of range");
   }
-  WsLayout w;
-  int rc = forward_layout(p, batch, layer, &w);
-  if (rc) return rc;
-  if (!ws || ws_bytes < w.total)
-    return fail(SLOS_ENOMEM, "workspace too small: need " + std::to_string(w.total) + " bytes");
-  DevTables dt;
-  rc = device_tables(p, &dt);
-  if (rc) return rc;
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  return forward_layout(p, batch, layer, w, (flags & SLOS_FLAG_COMPENSATED) != 0);
+}
+
+// the layer launches of a forward whose workspace `ws` follows layout `w` (no allocation)
+int forward_run(const slos_plan* p, const DevTables& dt, const void* U, int64_t batch, const int* cols, int layer,
+                int out_kind, void* out, double scale, unsigned flags, const WsLayout& w, void* ws, hipStream_t st) {
+  int rc = SLOS_OK;
   auto* base = static_cast<unsigned char*>(ws);
   float2* ones = reinterpret_cast<float2*>(base + w.ones);
   float2* bufs[2] = {reinterpret_cast<float2*>(base + w.even), reinterpret_cast<float2*>(base + w.odd)};
@@ -1220,17 +1268,37 @@ int slos_forward(const slos_plan* p, const void* U, int64_t batch, const int32_t
   const float2* Uf = static_cast<const float2*>(U);
   const float2* par = ones;
   uint64_t S_par = 1;
+  // compensated sums: every intermediate layer is a hi/lo pair [2][B][S] (the buffers are sized for it)
+  const bool hilo = (flags & SLOS_FLAG_COMPENSATED) != 0;
   for (int k = 1; k <= layer; ++k) {
     const uint64_t S = p->size(k);
     const bool last = (k == layer);
     void* dst = last ? out : bufs[k & 1];
     const int kind = last ? (out_kind == SLOS_OUT_PROBS ? 1 : 2) : 0;
-    rc = launch_layer(p, dt, k, cols[k - 1], Uf, batch, par, S_par, dst, S, kind, (float)scale, flags, st);
+    rc = launch_layer(p, dt, k, cols[k - 1], Uf, batch, par, S_par, dst, S, kind, (float)scale, flags & ~SLOS_FLAG_HILO,
+                      st, hilo && k >= 2, hilo && !last);
     if (rc) return rc;
     par = static_cast<const float2*>(dst);
     S_par = S;
   }
   return SLOS_OK;
+}
+
+}  // namespace
+
+int slos_forward(const slos_plan* p, const void* U, int64_t batch, const int32_t* modes, int layer, int out_kind,
+                 void* out, double scale, unsigned flags, void* ws, size_t ws_bytes, slos_stream_t stream) {
+  int cols[kMaxPhotons] = {};  // layer <= n <= 32 (checked before cols is written)
+  WsLayout w;
+  int rc = forward_check(p, U, batch, modes, layer, out_kind, out, flags, cols, &w);
+  if (rc) return rc;
+  if (!ws || ws_bytes < w.total)
+    return fail(SLOS_ENOMEM, "workspace too small: need " + std::to_string(w.total) + " bytes");
+  DevTables dt;
+  rc = device_tables(p, &dt);
+  if (rc) return rc;
+  return forward_run(p, dt, U, batch, cols, layer, out_kind, out, scale, flags, w, ws,
+                     reinterpret_cast<hipStream_t>(stream));
 }
 
 int slos_layer_step(const slos_plan* p, const void* U, int64_t batch, int k, int col, const void* psi_prev,
@@ -1244,8 +1312,12 @@ int slos_layer_step(const slos_plan* p, const void* U, int64_t batch, int k, int
   int rc = device_tables(p, &dt);
   if (rc) return rc;
   const int kind = out_kind == 2 ? 0 : (out_kind == SLOS_OUT_PROBS ? 1 : 2);
+  const bool hilo = (flags & SLOS_FLAG_HILO) && (flags & SLOS_FLAG_COMPENSATED);
+  if ((flags & SLOS_FLAG_HILO) && !(flags & SLOS_FLAG_COMPENSATED))
+    return fail(SLOS_EINVAL, "SLOS_FLAG_HILO needs SLOS_FLAG_COMPENSATED");
   return launch_layer(p, dt, k, col, static_cast<const float2*>(U), batch, static_cast<const float2*>(psi_prev),
-                      p->size(k - 1), out, p->size(k), kind, (float)scale, flags, reinterpret_cast<hipStream_t>(stream));
+                      p->size(k - 1), out, p->size(k), kind, (float)scale, flags & ~SLOS_FLAG_HILO,
+                      reinterpret_cast<hipStream_t>(stream), hilo, hilo && kind == 0);
 }
 
 int slos_readout_adjoint(const slos_plan* p, int64_t batch, const void* amps, const float* grad_p, double scale,
@@ -1267,12 +1339,66 @@ int slos_readout_adjoint(const slos_plan* p, int64_t batch, const void* amps, co
   return SLOS_OK;
 }
 
-static int adjoint_blocks(uint64_t Sp) { return (int)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 63) / 64, 1024)); }
+// work groups of the thread-per-state adjoint (grid-stride over the parents; one FP64 partial per
+// work group and mode)
+static int adjoint_blocks(uint64_t Sp) { return (int)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 63) / 64, 1u << 16)); }
+
+namespace {
+
+// parents per XCD chunk of the block descent and adjoint (see slos::xcd_chunk_index)
+constexpr uint64_t kDescChunkParents = 1ull << 18;
+
+// How one adjoint layer runs: the block kernel (multiset spaces whose parent layer has a 1..8
+// position prefix before its 4-position suffix, up to 32 modes) or the thread-per-state kernel;
+// `nblk` work groups, each writing one FP64 partial per mode.
+struct AdjLaunch {
+  bool block = false;
+  int maxd = 0;
+  uint64_t per_wave = 0;
+  uint32_t xcd_chunk = 1;
+  size_t lds = 0;
+  uint64_t nblk = 0;
+};
+
+int adjoint_launch(const slos_plan* p, int k, AdjLaunch* L) {
+  const uint64_t Sp = p->size(k - 1);
+  const int dp = k - 1 - slos::kSuf;
+  if (p->multiset && !p->tuples.empty() && dp >= 1 && dp <= 8 && p->me <= 32) {
+    const int maxd = dp <= 4 ? 4 : dp <= 6 ? 6 : 8;
+    const size_t shb = slos::desc_dyn_lds(k, p->me, maxd);
+    if (shb + sizeof(double) * 2 * slos::kWaves * 32 <= 64 * 1024) {
+      // parents per wave: at least 2048, and at most about 2^16 work groups (FP64 partials per mode)
+      uint64_t pw = (Sp / ((uint64_t)slos::kWaves << 16) + 63) & ~63ull;
+      pw = std::max<uint64_t>(pw, 2048);
+      const uint64_t wgs = ((Sp + pw - 1) / pw + slos::kWaves - 1) / slos::kWaves;
+      if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
+      L->block = true;
+      L->maxd = maxd;
+      L->per_wave = pw;
+      L->xcd_chunk = (uint32_t)std::max<uint64_t>(1, kDescChunkParents / ((uint64_t)slos::kWaves * pw));
+      L->lds = shb;
+      L->nblk = wgs;
+      return SLOS_OK;
+    }
+  }
+  const size_t sh = (((size_t)(k + 1) * (p->me + 1) * 8 + p->me * 4 + 15) & ~(size_t)15) + (size_t)p->m * 8 +
+                    (size_t)kAdjThreads * p->me * 8;
+  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "adjoint LDS exceeds 64 KB (too many modes)");
+  L->block = false;
+  L->lds = sh;
+  L->nblk = adjoint_blocks(Sp);
+  return SLOS_OK;
+}
+
+}  // namespace
 
 int slos_adjoint_workspace(const slos_plan* p, int64_t batch, int k, size_t* bytes) {
   if (!p || !bytes) return fail(SLOS_EINVAL, "NULL argument");
   if (k < 1 || k > p->n || batch < 1) return fail(SLOS_EINVAL, "bad argument");
-  *bytes = (size_t)batch * adjoint_blocks(p->size(k - 1)) * p->me * sizeof(double2);
+  AdjLaunch L;
+  int rc = adjoint_launch(p, k, &L);
+  if (rc) return rc;
+  *bytes = (size_t)batch * L.nblk * p->me * sizeof(double2);
   return SLOS_OK;
 }
 
@@ -1284,28 +1410,43 @@ int slos_adjoint_layer(const slos_plan* p, const void* U, int64_t batch, int k, 
   if (col < 0 || col >= p->m) return fail(SLOS_EINVAL, "input mode out of range");
   if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
   const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
-  const int nblk = adjoint_blocks(Sp);
-  if (!ws || ws_bytes < (size_t)batch * nblk * p->me * sizeof(double2)) return fail(SLOS_ENOMEM, "workspace too small");
-  const size_t sh = (((size_t)(k + 1) * (p->me + 1) * 8 + p->me * 4 + 15) & ~(size_t)15) + (size_t)p->m * 8 +
-                    (size_t)kAdjThreads * p->me * 8;
-  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "adjoint LDS exceeds 64 KB (too many modes)");
+  AdjLaunch L;
+  int rc = adjoint_launch(p, k, &L);
+  if (rc) return rc;
+  if (!ws || ws_bytes < (size_t)batch * L.nblk * p->me * sizeof(double2)) return fail(SLOS_ENOMEM, "workspace too small");
   DevTables dt;
-  int rc = device_tables(p, &dt);
+  rc = device_tables(p, &dt);
   if (rc) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_adjoint_layer, dim3((unsigned)nblk, (unsigned)batch), dim3(kAdjThreads), sh, st, dt.cg, dt.lo,
-                     p->me, p->multiset, k, p->m, col, static_cast<const float2*>(U), static_cast<const float2*>(psi_prev),
-                     static_cast<const float2*>(grad_k), Sk, static_cast<float2*>(grad_prev), Sp,
-                     static_cast<double2*>(ws), grad_prev != nullptr);
+  const auto* Uf = static_cast<const float2*>(U);
+  const auto* psi = static_cast<const float2*>(psi_prev);
+  const auto* gk = static_cast<const float2*>(grad_k);
+  auto* gout = static_cast<float2*>(grad_prev);
+  auto* part = static_cast<double2*>(ws);
+  if (L.block && dt.tuples) {
+    const dim3 grid((unsigned)L.nblk, (unsigned)batch);
+    const uint64_t S_pre = p->size(k - 1 - slos::kSuf);
+#define SLOS_ADJ(D, E)                                                                                                 \
+  hipLaunchKernelGGL((slos::k_adjoint_block<D, E>), grid, dim3(slos::kBlockThreads), L.lds, st, dt.cg, dt.lo, dt.tuples, \
+                     p->me, p->me + 1, k, p->m, col, Uf, psi, gk, Sk, gout, Sp, S_pre, L.per_wave, L.xcd_chunk, part)
+#define SLOS_ADJ_E(D)                   \
+  if (p->me <= 16) SLOS_ADJ(D, 16);     \
+  else SLOS_ADJ(D, 32)
+    if (L.maxd == 4) { SLOS_ADJ_E(4); }
+    else if (L.maxd == 6) { SLOS_ADJ_E(6); }
+    else { SLOS_ADJ_E(8); }
+#undef SLOS_ADJ_E
+#undef SLOS_ADJ
+  } else {
+    hipLaunchKernelGGL(k_adjoint_layer, dim3((unsigned)L.nblk, (unsigned)batch), dim3(kAdjThreads), L.lds, st, dt.cg,
+                       dt.lo, p->me, p->multiset, k, p->m, col, Uf, psi, gk, Sk, gout, Sp, part, grad_prev != nullptr);
+  }
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_gu, dim3((unsigned)p->me, (unsigned)batch), dim3(64), 0, st,
-                     static_cast<const double2*>(ws), nblk, p->me, p->m, col, static_cast<float2*>(grad_U), accumulate);
+                     static_cast<const double2*>(ws), (int)L.nblk, p->me, p->m, col, static_cast<float2*>(grad_U), accumulate);
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
-
-// parents per XCD chunk of the block descent (see slos::xcd_chunk_index)
-constexpr uint64_t kDescChunkParents = 1ull << 18;
 
 int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1, int64_t batch, unsigned flags,
                       double scale, slos_stream_t stream) {
@@ -1510,10 +1651,10 @@ int slos_group_destroy(slos_group* G) {
   return SLOS_OK;
 }
 
-int slos_group_workspace(const slos_group* G, const int64_t* batch, size_t* bytes) {
+int slos_group_workspace(const slos_group* G, const int64_t* batch, unsigned flags, size_t* bytes) {
   if (!G || !batch || !bytes) return fail(SLOS_EINVAL, "NULL argument");
   GroupWs w;
-  int rc = group_layout(G, batch, &w);
+  int rc = group_layout(G, batch, &w, (flags & SLOS_FLAG_COMPENSATED) != 0);
   if (rc) return rc;
   *bytes = w.total;
   return SLOS_OK;

@@ -131,15 +131,15 @@ def thin_counts(counts: torch.Tensor, eta, generator: torch.Generator | None = N
 
 
 def thin_to_clicks(occupations, probs: torch.Tensor) -> tuple[np.ndarray, torch.Tensor]:
-    """Group a pattern distribution by click pattern (imperfections.py:429-438)."""
-    occ = np.asarray(occupations)
-    cl = (occ > 0).astype(np.int64)
-    keys = (cl * (2 ** np.arange(occ.shape[1], dtype=np.int64))).sum(axis=1)
-    uk, inv = np.unique(keys, return_inverse=True)
-    first = np.zeros(uk.shape[0], dtype=np.int64)
-    first[inv[::-1]] = np.arange(occ.shape[0])[::-1]
-    out = torch.zeros((*probs.shape[:-1], uk.shape[0]), dtype=probs.dtype, device=probs.device)
-    return cl[first], out.index_add(-1, torch.as_tensor(inv, device=probs.device), probs)
+    """The distribution seen by threshold detectors (imperfections.py:429-438): patterns [C, m] of
+    0/1 clicks and their probabilities [..., C] (the sum over the occupations with that click
+    pattern). Patterns come in ascending order of sum_j click_j 2^j, the reference's order: the
+    lexicographic order of the rows read from the last mode to the first."""
+    cl = torch.as_tensor(np.asarray(occupations) > 0, dtype=torch.uint8)
+    rev, inv = torch.unique(cl.flip(1), dim=0, return_inverse=True)
+    out = torch.zeros((*probs.shape[:-1], rev.shape[0]), dtype=probs.dtype, device=probs.device)
+    out.index_add_(-1, inv.to(probs.device), probs)
+    return rev.flip(1).to(torch.int64).numpy(), out
 
 
 @dataclass
@@ -183,9 +183,15 @@ class LossyDistribution:
         return thin_to_clicks(occ, p)
 
     def sample(self, shots: int, generator: torch.Generator | None = None) -> torch.Tensor:
-        """Samples [shots, m] (one distribution) (imperfections.py:213-218): the flat host draw."""
+        """Samples [shots, m] (one distribution) (imperfections.py:213-218). With no generator or a
+        CPU one: the reference's flat host draw, so the same generator gives the same samples.
+        With a CUDA generator (which the reference's host draw cannot use): drawn on the GPU, the
+        photon number first and then the pattern inside its layer (slos.sampling.sample_lossy_device)."""
         if any(q.dim() != 1 for q in self.probs):
             raise ValueError("one distribution at a time")
+        if generator is not None and generator.device.type == "cuda":
+            from .slos.sampling import sample_lossy_device
+            return sample_lossy_device(self, shots, generator)
         occ, p = self.flat()
         return torch.as_tensor(occ)[draw_indices(p.detach().cpu().to(torch.float64), shots, generator)]
 

@@ -65,9 +65,9 @@ class Plan:
         return out
 
     # ---- device work --------------------------------------------------------------------------
-    def workspace_bytes(self, batch: int, layer: int) -> int:
+    def workspace_bytes(self, batch: int, layer: int, flags: int = 0) -> int:
         out = ctypes.c_size_t()
-        nat.check(self._lib.slos_forward_workspace(self._h, batch, layer, ctypes.byref(out)),
+        nat.check(self._lib.slos_forward_workspace(self._h, batch, layer, int(flags), ctypes.byref(out)),
                   "slos_forward_workspace")
         return int(out.value)
 
@@ -83,7 +83,7 @@ class Plan:
         if out is None:
             out = torch.empty((B, S), dtype=torch.float32 if out_kind == nat.OUT_PROBS else torch.complex64,
                               device=dev)
-        ws = torch.empty(max(self.workspace_bytes(B, layer), 1), dtype=torch.uint8, device=dev)
+        ws = torch.empty(max(self.workspace_bytes(B, layer, flags), 1), dtype=torch.uint8, device=dev)
         mp = None
         if modes is not None:
             mp = (ctypes.c_int32 * layer)(*[int(i) for i in modes])
@@ -93,12 +93,21 @@ class Plan:
         return out
 
     def layer_step(self, U: torch.Tensor, k: int, col: int, psi_prev: torch.Tensor, out_kind: int = nat.OUT_RAW,
-                   scale: float = 1.0, flags: int = 0) -> torch.Tensor:
-        """Layer k from layer k-1 (input column col): raw complex64, probs or scaled amplitudes."""
+                   scale: float = 1.0, flags: int = 0, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Layer k from layer k-1 (input column col): raw complex64, probs or scaled amplitudes,
+        into `out` if given (contiguous [B, S_k] of the output's dtype on U's device). With
+        FLAG_HILO the raw layers in and out are hi/lo pairs [2, B, S] (compensated sums carry
+        their low-order parts)."""
         B = U.shape[0]
         S = self.sizes[k]
-        out = torch.empty((B, S), dtype=torch.float32 if out_kind == nat.OUT_PROBS else torch.complex64,
-                          device=U.device)
+        dtype = torch.float32 if out_kind == nat.OUT_PROBS else torch.complex64
+        shape = (2, B, S) if (flags & nat.FLAG_HILO and out_kind == nat.OUT_RAW) else (B, S)
+        if flags & nat.FLAG_HILO and (psi_prev.dim() != 3 or psi_prev.shape[0] != 2):
+            raise ValueError("FLAG_HILO: psi_prev must be a hi/lo pair [2, B, S_{k-1}]")
+        if out is None:
+            out = torch.empty(shape, dtype=dtype, device=U.device)
+        elif out.shape != shape or out.dtype != dtype or out.device != U.device or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous {dtype} tensor of shape {shape} on {U.device}")
         nat.check(self._lib.slos_layer_step(self._h, nat.ptr(U), B, k, int(col), nat.ptr(psi_prev), out_kind,
                                             nat.ptr(out), float(scale), int(flags), nat.stream_handle(U.device)),
                   "slos_layer_step")

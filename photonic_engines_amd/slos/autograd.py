@@ -1,9 +1,9 @@
-"""Hand-written adjoint of FockSLOSKernel.probs behind torch.autograd.Function.
+"""Hand-written adjoint of the SLOS recurrence behind torch.autograd.Function.
 
 Replaces autograd through gather / mul / scatter_add_ (photonic_engines/slos/kernel.py:216-257)
-and the opt-in recomputing layer (_CheckpointedLayer, kernel.py:40-80). Forward keeps only every
-other layer (the even ones); backward recomputes each odd layer once from the even layer below it,
-then runs the recurrence backwards with the adjoint kernels:
+and the opt-in recomputing layer (_CheckpointedLayer, kernel.py:40-80). A forward keeps only every
+other layer (the even ones); the backward recomputes each odd layer once from the even layer below
+it, then runs the recurrence backwards with the adjoint kernels:
 
   readout   g_n = 2 s^2 prod(t!) dL/dp * psi_n            (p = |s psi_n|^2 prod t!)
   layer k   g_{k-1}[t] = sum_j conj(U[j, i_k]) g_k[t + e_j]
@@ -11,6 +11,12 @@ then runs the recurrence backwards with the adjoint kernels:
 
 Gradients follow torch's convention for complex leaves (U.grad = dL/dRe U + i dL/dIm U), as the
 reference's autograd does; only the input columns i_k receive gradient.
+
+  SLOSProbs  probabilities of the last layer (the fused readout), the drop-in probs(U);
+  SLOSRaw    the raw (deferred-normalisation) amplitudes of layer `layer` for input columns
+             `modes`: amplitudes(U) and the interfering subsets of the x-model;
+  LayerStep  one layer step with a caller's coefficient matrix (the probability-level raise of the
+             x-model runs as a complex layer step with the column |U[j, col]|^2).
 """
 from __future__ import annotations
 
@@ -19,44 +25,121 @@ import torch
 from .. import _native as nat
 
 
+def _layer_flags(kernel) -> int:
+    """The layer-step flags of the kernel's forward: compensated sums carry every layer as a hi/lo
+    pair (as slos_forward does), so that the autograd forward equals the plain one bit for bit."""
+    return (nat.FLAG_COMPENSATED | nat.FLAG_HILO) if kernel._flags() & nat.FLAG_COMPENSATED else 0
+
+
+def _hi(psi: torch.Tensor) -> torch.Tensor:
+    """The high parts [B, S] of a hi/lo layer [2, B, S] (a plain layer as is)."""
+    return psi[0] if psi.dim() == 3 else psi
+
+
+def _checkpointed_forward(plan, Ug: torch.Tensor, modes, layer: int, cf: int):
+    """Layers 1..layer-1, keeping layer 0 and the even layers: (saved, psi_{layer-1})."""
+    psi = torch.ones((Ug.shape[0], 1), dtype=torch.complex64, device=Ug.device)
+    if cf & nat.FLAG_HILO:
+        psi = torch.stack([psi, torch.zeros_like(psi)])
+    saved = {0: psi}
+    for k in range(1, layer):
+        psi = plan.layer_step(Ug, k, modes[k - 1], psi, nat.OUT_RAW, flags=cf)
+        if k % 2 == 0:
+            saved[k] = psi
+    return saved, psi
+
+
+def _adjoint_layers(plan, Ug: torch.Tensor, modes, layer: int, saved: dict, cf: int, g: torch.Tensor,
+                    psi_top: torch.Tensor) -> torch.Tensor:
+    """dL/dU (complex64, like Ug) from g = dL/dpsi_layer, layers `layer` .. 1; psi_top = psi_{layer-1}."""
+
+    def psi_of(k: int) -> torch.Tensor:
+        if k in saved:
+            return saved[k]
+        return plan.layer_step(Ug, k, modes[k - 1], saved[k - 1], nat.OUT_RAW, flags=cf)
+
+    gU = torch.zeros_like(Ug)
+    psi_prev = psi_top
+    for k in range(layer, 0, -1):
+        if k < layer:
+            psi_prev = psi_of(k - 1)
+        g = plan.adjoint_layer(Ug, k, modes[k - 1], _hi(psi_prev), g, gU, want_prev=k > 1)
+    return gU
+
+
+def _user_grad(gU: torch.Tensor, ctx) -> torch.Tensor:
+    gU = gU.to(device=ctx.user_device, dtype=ctx.user_dtype)
+    return gU.squeeze(0) if len(ctx.user_shape) == 2 else gU
+
+
 class SLOSProbs(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U_user: torch.Tensor, Ug: torch.Tensor, kernel) -> torch.Tensor:
         plan, modes, n = kernel._plan, kernel.input_modes, kernel.n
-        psi = torch.ones((Ug.shape[0], 1), dtype=torch.complex64, device=Ug.device)
-        saved = {0: psi}
-        for k in range(1, n):
-            psi = plan.layer_step(Ug, k, modes[k - 1], psi, nat.OUT_RAW)
-            if k % 2 == 0:
-                saved[k] = psi
-        p = plan.layer_step(Ug, n, modes[n - 1], psi, nat.OUT_PROBS, scale=kernel._input_norm,
-                            flags=kernel._flags() & nat.FLAG_COMPENSATED)
-        ctx.kernel, ctx.saved, ctx.Ug = kernel, saved, Ug
+        cf = _layer_flags(kernel)                        # every layer as in the forward without grad
+        saved, psi = _checkpointed_forward(plan, Ug, modes, n, cf)
+        p = plan.layer_step(Ug, n, modes[n - 1], psi, nat.OUT_PROBS, scale=kernel._input_norm, flags=cf)
+        ctx.kernel, ctx.saved, ctx.Ug, ctx.cf = kernel, saved, Ug, cf
         ctx.user_shape, ctx.user_dtype, ctx.user_device = U_user.shape, U_user.dtype, U_user.device
         return p
 
     @staticmethod
     def backward(ctx, grad_p: torch.Tensor):
-        kernel, saved, Ug = ctx.kernel, ctx.saved, ctx.Ug
+        kernel, saved, Ug, cf = ctx.kernel, ctx.saved, ctx.Ug, ctx.cf
         plan, modes, n = kernel._plan, kernel.input_modes, kernel.n
         grad_p = grad_p.to(device=Ug.device, dtype=torch.float32).contiguous()
-
-        def layer(k: int) -> torch.Tensor:
-            if k in saved:
-                return saved[k]
-            return plan.layer_step(Ug, k, modes[k - 1], saved[k - 1], nat.OUT_RAW)
-
-        psi_prev = layer(n - 1)
-        amps = plan.layer_step(Ug, n, modes[n - 1], psi_prev, nat.OUT_RAW)
+        psi_prev = saved[n - 1] if (n - 1) in saved else plan.layer_step(Ug, n - 1, modes[n - 2], saved[n - 2],
+                                                                          nat.OUT_RAW, flags=cf)
+        amps = _hi(plan.layer_step(Ug, n, modes[n - 1], psi_prev, nat.OUT_RAW, flags=cf))
         g = plan.readout_adjoint(amps, grad_p, kernel._input_norm)
         del amps
-        gU = torch.zeros_like(Ug)
-        for k in range(n, 0, -1):
-            if k < n:
-                psi_prev = layer(k - 1)
-            g = plan.adjoint_layer(Ug, k, modes[k - 1], psi_prev, g, gU, want_prev=k > 1)
+        gU = _adjoint_layers(plan, Ug, modes, n, saved, cf, g, psi_prev)
         ctx.saved = None
-        gU = gU.to(device=ctx.user_device, dtype=ctx.user_dtype)
-        if len(ctx.user_shape) == 2:
-            gU = gU.squeeze(0)
-        return gU, None, None
+        return _user_grad(gU, ctx), None, None
+
+
+class SLOSRaw(torch.autograd.Function):
+    """Raw amplitudes [B, S_layer] (complex64) of layer `layer` for input columns `modes`."""
+
+    @staticmethod
+    def forward(ctx, U_user: torch.Tensor, Ug: torch.Tensor, kernel, modes, layer: int) -> torch.Tensor:
+        plan = kernel._plan
+        modes = [int(c) for c in modes]
+        cf = _layer_flags(kernel)
+        saved, psi = _checkpointed_forward(plan, Ug, modes, layer, cf)
+        out = _hi(plan.layer_step(Ug, layer, modes[layer - 1], psi, nat.OUT_RAW, flags=cf))
+        ctx.kernel, ctx.saved, ctx.Ug, ctx.cf, ctx.modes, ctx.layer = kernel, saved, Ug, cf, modes, layer
+        ctx.user_shape, ctx.user_dtype, ctx.user_device = U_user.shape, U_user.dtype, U_user.device
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        kernel, saved, Ug, cf, modes, layer = ctx.kernel, ctx.saved, ctx.Ug, ctx.cf, ctx.modes, ctx.layer
+        plan = kernel._plan
+        g = grad_out.to(device=Ug.device, dtype=torch.complex64).contiguous()
+        psi_prev = saved[layer - 1] if (layer - 1) in saved else plan.layer_step(
+            Ug, layer - 1, modes[layer - 2], saved[layer - 2], nat.OUT_RAW, flags=cf)
+        gU = _adjoint_layers(plan, Ug, modes, layer, saved, cf, g, psi_prev)
+        ctx.saved = None
+        return _user_grad(gU, ctx), None, None, None, None
+
+
+class LayerStep(torch.autograd.Function):
+    """psi_k = layer k of the recurrence from psi_{k-1} with coefficient column W[:, :, col]
+    (complex64 [B, m, m], differentiable) — one layer with its adjoint; psi_{k-1} is kept."""
+
+    @staticmethod
+    def forward(ctx, W: torch.Tensor, psi_prev: torch.Tensor, plan, k: int, col: int) -> torch.Tensor:
+        W = W.contiguous()
+        psi_prev = psi_prev.contiguous()
+        ctx.save_for_backward(W, psi_prev)
+        ctx.plan, ctx.k, ctx.col = plan, k, col
+        return plan.layer_step(W, k, col, psi_prev, nat.OUT_RAW)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        W, psi_prev = ctx.saved_tensors
+        gW = torch.zeros_like(W)
+        gprev = ctx.plan.adjoint_layer(W, ctx.k, ctx.col, psi_prev, g.to(torch.complex64).contiguous(), gW,
+                                       want_prev=True)
+        return gW, gprev, None, None, None

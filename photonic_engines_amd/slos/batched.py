@@ -95,7 +95,7 @@ class UnevenBatch:
     def workspace_bytes(self, batches: Sequence[int]) -> int:
         B = (ctypes.c_int64 * len(self.kernels))(*[int(b) for b in batches])
         out = ctypes.c_size_t()
-        nat.check(self._lib.slos_group_workspace(self._h, B, ctypes.byref(out)), "slos_group_workspace")
+        nat.check(self._lib.slos_group_workspace(self._h, B, self.flags, ctypes.byref(out)), "slos_group_workspace")
         return int(out.value)
 
     def _buffers(self, Us, dev):

@@ -149,7 +149,7 @@ class FockSLOSKernel:
         """With `input_modes` (g <= n columns) the recurrence stops on layer g of the same basis,
         raw, without the input norm (kernel.py:201-206, used by slos/species.py:94)."""
         if U.requires_grad and torch.is_grad_enabled():
-            raise NotImplementedError("gradients flow through probs(U); amplitudes() is forward-only")
+            return self._amplitudes_grad(U, input_modes)
         Ug, batched = self._prepare(U)
         if input_modes is None:
             out = self._plan.forward(Ug, out_kind=nat.OUT_AMPS, scale=self._input_norm, flags=self._flags())
@@ -161,6 +161,24 @@ class FockSLOSKernel:
                 out = self._plan.forward(Ug, modes=modes, layer=len(modes), out_kind=nat.OUT_AMPS,
                                          scale=1.0, flags=self._flags() & ~nat.FLAG_IMMEDIATE)
         return self._out(out, batched)
+
+    def _amplitudes_grad(self, U: torch.Tensor, input_modes: Sequence[int] | None) -> torch.Tensor:
+        """_amplitudes with U requiring a gradient: the raw layer through the hand-written adjoint
+        (SLOSRaw), then the input norm and, for immediate normalisation, sqrt(prod t_j!) as torch
+        products (kernel.py:191-260 under autograd)."""
+        from .autograd import SLOSRaw
+        Ug, batched = self._prepare(U)
+        modes = self.input_modes if input_modes is None else [int(i) for i in input_modes]
+        if not modes:
+            out = torch.ones((Ug.shape[0], 1), dtype=torch.complex64, device=Ug.device)
+            return self._out(out, batched)
+        raw = SLOSRaw.apply(U, Ug, self, modes, len(modes))
+        if input_modes is None:
+            if self._input_norm != 1.0:
+                raw = raw * self._input_norm
+            if self.normalization == "immediate" and not self._output_norm_is_one:
+                raw = raw * torch.sqrt(self.output_prob_norm.to(raw.device))
+        return self._out(raw, batched)
 
     # ---- probabilities ------------------------------------------------------------------------------
     def probs(self, U: torch.Tensor, loss=None, indist=None, order: int | None = None):
@@ -296,11 +314,14 @@ class SuperpositionSLOSKernel:
     def output_size(self) -> int:
         return self.layers.output_size()
 
+    def _flags(self) -> int:
+        return 0
+
     def amplitudes(self, U: torch.Tensor, weights=None, chunk_size: int | None = None) -> torch.Tensor:
         """Coherently summed output amplitudes [S] / [B, S] (complex64): sum_k w_k * norm_k * raw_k.
-        Default weights: the state's (from_state / weights=), else uniform 1/sqrt(K)."""
-        if U.requires_grad and torch.is_grad_enabled():
-            raise NotImplementedError("SuperpositionSLOSKernel is forward-only")
+        Default weights: the state's (from_state / weights=), else uniform 1/sqrt(K). Differentiable
+        in U (the raw amplitudes through the hand-written adjoint, SLOSRaw) and in tensor weights."""
+        grad_u = U.requires_grad and torch.is_grad_enabled()
         batched = U.dim() == 3
         Ub = (U if batched else U.unsqueeze(0))
         if Ub.shape[-2:] != (self.m, self.m):
@@ -311,6 +332,8 @@ class SuperpositionSLOSKernel:
             w = self.default_weights
         elif weights is None:
             w = torch.full((self.K,), 1.0 / math.sqrt(self.K), dtype=torch.complex64)
+        elif isinstance(weights, torch.Tensor):
+            w = weights.to(torch.complex64)
         else:
             w = torch.as_tensor(np.asarray(weights), dtype=torch.complex64)
         w = w.to(dev)
@@ -321,10 +344,15 @@ class SuperpositionSLOSKernel:
         # input norm and weight and are summed over k
         K, B, m = self.K, Ub.shape[0], self.m
         cols = torch.as_tensor(self._modes, dtype=torch.int64, device=dev)            # [K, n]
-        Up = torch.zeros((K, B, m, m), dtype=torch.complex64, device=dev)
-        Up[..., : self.n] = Ub[:, :, cols].permute(2, 0, 1, 3)
-        raw = self._plan.forward(Up.reshape(K * B, m, m).contiguous(), modes=list(range(self.n)), layer=self.n,
-                                 out_kind=nat.OUT_AMPS, scale=1.0)
+        Ug = Ub[:, :, cols].permute(2, 0, 1, 3)                                     # [K, B, m, n]
+        if self.n < m:
+            Ug = torch.cat([Ug, torch.zeros((K, B, m, m - self.n), dtype=torch.complex64, device=dev)], dim=3)
+        Up = Ug.reshape(K * B, m, m).contiguous()
+        if grad_u:
+            from .autograd import SLOSRaw
+            raw = SLOSRaw.apply(Up, Up.detach(), self, list(range(self.n)), self.n)
+        else:
+            raw = self._plan.forward(Up, modes=list(range(self.n)), layer=self.n, out_kind=nat.OUT_AMPS, scale=1.0)
         coef = (w * torch.as_tensor(self._norms, dtype=torch.float32, device=dev)).to(torch.complex64)
         amp = (raw.reshape(K, B, -1) * coef.view(K, 1, 1)).sum(0)
         amp = amp if batched else amp.squeeze(0)

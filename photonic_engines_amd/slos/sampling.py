@@ -70,6 +70,36 @@ def unrank_rows(layers, k: int, idx: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def sample_lossy_device(ld, shots: int, generator: torch.Generator) -> torch.Tensor:
+    """Samples [shots, m] (int64, on the generator's GPU) of a LossyDistribution whose blocks are on
+    the GPU, drawn there: the photon number k of every shot from the FP64 block masses
+    (torch.multinomial on the generator), then the pattern inside layer k by draw_indices on that
+    layer's float32 block (torch.multinomial up to 2^24 entries, else the device inverse CDF; no
+    concatenated or float64 copy of the layers) and the device unrank. The distribution is the
+    reference's flat draw over the concatenated layers (imperfections.py:213-218); the draws
+    differ (the reference draws on the host, from a CPU generator)."""
+    if any(q.dim() != 1 for q in ld.probs):
+        raise ValueError("one distribution at a time")
+    dev = generator.device
+    plan = ld.layers._plan
+    m = ld.layers.m
+    blocks = [q.detach().to(device=dev, dtype=torch.float32).contiguous() for q in ld.probs]
+    for k, q in enumerate(blocks):
+        if q.numel() != ld.layers.size(k):
+            raise ValueError(f"block {k} has {q.numel()} entries, layer {k} has {ld.layers.size(k)}")
+    masses = torch.cat([plan.total_mass(q.unsqueeze(0)) for q in blocks]).to(dev)
+    ks = torch.multinomial(masses, int(shots), replacement=True, generator=generator)
+    counts = torch.bincount(ks, minlength=len(blocks)).cpu().tolist()
+    out = torch.zeros((int(shots), m), dtype=torch.int64, device=dev)
+    for k, c in enumerate(counts):
+        if c == 0 or k == 0:
+            continue                                            # the vacuum row is all zeros
+        sel = torch.nonzero(ks == k).squeeze(1)
+        idx = draw_indices(blocks[k], int(c), generator)
+        out[sel] = unrank_rows(ld.layers, k, idx)
+    return out
+
+
 def sample_from_probs(layers, probs, shots: int, generator: torch.Generator | None = None, loss=None) -> torch.Tensor:
     """Output occupations [shots, m] drawn from a probability vector over `layers.final_states`
     (emitter.py:215-230); probs need not be normalised. A LossyDistribution is sampled over its

@@ -18,7 +18,11 @@ On the layered basis of the kernel:
 `x_mixture` is the x-model: the sum over the interfering subsets G of the input photons with
 weights x^|G| (1 - x)^(n - |G|), each a `species_probs`; `order=b` keeps at most b
 distinguishable photons (the dropped mass is `imperfections.truncation_bound`). A tensor x gets
-a gradient (the weights are torch operations); gradients in U are not provided on this path.
+a gradient (the weights are torch operations). U gets one too ("Differentiable in q and U",
+species.py:37-41): with U requiring a gradient, the interfering layers run as SLOSRaw (the raw
+amplitudes with the hand-written adjoint) squared by torch, and every raise runs as a LayerStep
+(the complex layer step with the column |U[j, col]|^2, whose adjoint gives the gradients of the
+layer below and of that column).
 """
 from __future__ import annotations
 
@@ -33,13 +37,43 @@ from .. import _native as nat
 from ..imperfections import spec
 
 
+def _needs_grad(U: torch.Tensor) -> bool:
+    return U.requires_grad and torch.is_grad_enabled()
+
+
 def _batched_u(kernel, U: torch.Tensor):
-    """U [m,m] | [B,m,m] -> (contiguous complex64 [B,m,m] on the compute GPU, was batched)."""
-    if U.requires_grad:
-        raise NotImplementedError("probs(indist=...): gradients in U are not provided on the "
-                                  "partial-distinguishability path (a tensor indist x gets one)")
-    Ub, batched = kernel._prepare(U.detach())
-    return Ub, batched
+    """U [m,m] | [B,m,m] -> (contiguous complex64 [B,m,m] on the compute GPU, was batched); the
+    result stays attached to U's graph when U requires a gradient."""
+    return kernel._prepare(U if _needs_grad(U) else U.detach())
+
+
+def _layer_norm(kernel, g: int, device) -> torch.Tensor:
+    """prod_j t_j! of the states of layer g as float32 [S_g] (1 when no state can bunch)."""
+    if kernel._output_norm_is_one:
+        return torch.ones(1, dtype=torch.float32, device=device)
+    return torch.as_tensor(kernel.layers.prob_norms(g), dtype=torch.float32, device=device)
+
+
+def _raw_probs(kernel, Ub: torch.Tensor, modes: Sequence[int], g: int, scale: float) -> torch.Tensor:
+    """|scale psi_g|^2 prod t! of layer g for the columns `modes` with U-gradients (SLOSRaw)."""
+    from .autograd import SLOSRaw
+    raw = SLOSRaw.apply(Ub, Ub.detach().contiguous(), kernel, list(modes), g)
+    return (raw.real ** 2 + raw.imag ** 2) * (scale * scale) * _layer_norm(kernel, g, Ub.device)
+
+
+def _raise_steps(kernel, q: torch.Tensor, k: int, Wcols: Sequence[torch.Tensor]) -> torch.Tensor:
+    """Raises of q [B', S_k] (float32) by the coefficient columns Wcols[t] ([B', m], real, >= 0)
+    as complex layer steps with their adjoint (LayerStep): [B', S_{k + len(Wcols)}]."""
+    from .autograd import LayerStep
+    Bp, m = q.shape[0], kernel.m
+    qc = torch.complex(q, torch.zeros_like(q))
+    for w in Wcols:
+        W = torch.zeros((Bp, m, m), dtype=torch.complex64, device=q.device)
+        W = W.index_copy(2, torch.zeros(1, dtype=torch.int64, device=q.device),
+                         w.to(torch.complex64).unsqueeze(2))
+        qc = LayerStep.apply(W, qc, kernel._plan, k + 1, 0)
+        k += 1
+    return qc.real
 
 
 def layer_probs(kernel, Ub: torch.Tensor, cols: Sequence[int]) -> torch.Tensor:
@@ -54,6 +88,8 @@ def layer_probs(kernel, Ub: torch.Tensor, cols: Sequence[int]) -> torch.Tensor:
     for c in cols:
         counts[c] = counts.get(c, 0) + 1
     s_norm = math.prod(math.factorial(v) for v in counts.values())
+    if _needs_grad(Ub):
+        return _raw_probs(kernel, Ub, [int(c) for c in cols], g, 1.0 / math.sqrt(s_norm))
     flags = nat.FLAG_COMPENSATED if kernel.compensated else 0
     return kernel._plan.forward(Ub, modes=[int(c) for c in cols], layer=g, out_kind=nat.OUT_PROBS,
                                 scale=1.0 / math.sqrt(s_norm), flags=flags)
@@ -63,6 +99,8 @@ def classical_raise(kernel, q: torch.Tensor, k: int, col: int, Ub: torch.Tensor)
     """One photon that interferes with nothing enters column `col`: the layer-k distribution
     q [B, S_k] becomes the layer-(k+1) one, q'(t + e_j) = sum q(t) |U[j, col]|^2."""
     w = (Ub[:, :, int(col)].abs() ** 2).to(torch.float32).contiguous()
+    if _needs_grad(Ub) or q.requires_grad:
+        return _raise_steps(kernel, q.contiguous(), k, [w])
     return kernel._plan.raise_layer(k + 1, w, q.contiguous())
 
 
@@ -115,13 +153,18 @@ def _subset_terms(kernel, Ub: torch.Tensor, cols: Sequence[int], bads: Sequence[
     goods = [[cols[j] for j in range(n) if j not in bad] for bad in bads]
     singles = [[cols[j] for j in bad] for bad in bads]
     g = n - nb
+    grad = _needs_grad(Ub)
     if g >= 2:
         gi = torch.as_tensor(goods, dtype=torch.int64, device=dev)                  # [C, g]
-        Up = torch.zeros((C, B, m, m), dtype=torch.complex64, device=dev)
-        Up[..., :g] = Ub[:, :, gi].permute(2, 0, 1, 3)                               # [C, B, m, g]
-        q = kernel._plan.forward(Up.reshape(C * B, m, m).contiguous(), modes=list(range(g)), layer=g,
-                                 out_kind=nat.OUT_PROBS, scale=1.0,
-                                 flags=nat.FLAG_COMPENSATED if kernel.compensated else 0)
+        Ug = Ub[:, :, gi].permute(2, 0, 1, 3)                                        # [C, B, m, g]
+        if g < m:
+            Ug = torch.cat([Ug, torch.zeros((C, B, m, m - g), dtype=torch.complex64, device=dev)], dim=3)
+        Up = Ug.reshape(C * B, m, m).contiguous()
+        if grad:
+            q = _raw_probs(kernel, Up, list(range(g)), g, 1.0)
+        else:
+            q = kernel._plan.forward(Up, modes=list(range(g)), layer=g, out_kind=nat.OUT_PROBS, scale=1.0,
+                                     flags=nat.FLAG_COMPENSATED if kernel.compensated else 0)
         norms = []
         for good in goods:
             counts: dict = {}
@@ -135,7 +178,14 @@ def _subset_terms(kernel, Ub: torch.Tensor, cols: Sequence[int], bads: Sequence[
         q = torch.ones((C * B, 1), dtype=torch.float32, device=dev)
         singles = [sg + gd for sg, gd in zip(singles, goods)]
         k = 0
-    if singles[0]:
+    if singles[0] and grad:
+        Wcols = []
+        for t in range(len(singles[0])):
+            ct = torch.as_tensor([sg[t] for sg in singles], dtype=torch.int64, device=dev)  # [C]
+            Wcols.append((Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(C * B, m).to(torch.float32))
+        q = _raise_steps(kernel, q.contiguous(), k, Wcols)
+        k += len(singles[0])
+    elif singles[0]:
         # The raises are the layer recurrence with a real coefficient column |U[j, col]|^2 on real
         # data: run them as complex layer steps (imaginary parts stay exactly zero, the real parts are
         # the same fused multiply-adds) so that the block kernel carries them.

@@ -13,6 +13,11 @@ Gaussian pair, the generator of the reference's tests/helpers.py:10-17):
       finite differences (h = 1e-6) on Re and Im of every entry
   C5  n=11, m=32, input [1]*11 + [0]*21                        -> 10,000 seeded outputs, Ryser complex128
   order  sha256 of the [S, n] int16 position arrays of the reference-order enumeration
+  G8  n=8, m=24, first n modes, U of seed 8024                 -> 1,024 indices drawn from the exact
+      distribution (numpy PCG64 seed 8), and dL/dU of L = -mean log p[idx] by complex128 torch
+      autograd of the stored-edge restatement (oracle/stored_edge.py)
+
+    python tests/golden/make_golden.py g8      (only G8)
 """
 from __future__ import annotations
 
@@ -56,6 +61,25 @@ def c3_batch() -> np.ndarray:
 def loss_c4(U, idx):
     p = pull.forward_probs(6, 16, first_modes(6, 16), U, prec=128)
     return -np.mean(np.log(p[idx]))
+
+
+def grad_n8m24() -> float:
+    """G8: the indices, the gradient and the loss value (written to g8_*.npy; returns the loss)."""
+    import torch
+    from oracle.stored_edge import StoredEdgeSLOS
+    n, m = 8, 24
+    U = haar_unitary(m, 8024)
+    ref = StoredEdgeSLOS(n, m, first_modes(n, m), dtype=torch.complex128)
+    Ut = torch.as_tensor(U).requires_grad_(True)
+    p = ref.probs(Ut)
+    pn = p.detach().numpy()
+    idx = np.random.default_rng(8).choice(pn.shape[0], size=1024, replace=True, p=pn / pn.sum()).astype(np.int64)
+    loss = -torch.log(p[torch.as_tensor(idx)]).mean()
+    loss.backward()
+    np.save(os.path.join(HERE, "g8_u.npy"), U)
+    np.save(os.path.join(HERE, "g8_idx.npy"), idx)
+    np.save(os.path.join(HERE, "g8_grad.npy"), Ut.grad.numpy())
+    return float(loss)
 
 
 def main():
@@ -102,10 +126,18 @@ def main():
         L = enumerate_layers(n, m, mask)[n].astype(np.int16)
         order[f"{n},{m},{mask}"] = {"size": int(L.shape[0]), "sha256": hashlib.sha256(L.tobytes()).hexdigest()}
     out["order"] = order
+    out["g8_loss"] = grad_n8m24()
     with open(os.path.join(HERE, "golden.json"), "w") as fh:
         json.dump(out, fh, indent=1, sort_keys=True)
     print("wrote", sorted(os.listdir(HERE)))
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["g8"]:
+        with open(os.path.join(HERE, "golden.json")) as fh:
+            rec = json.load(fh)
+        rec["g8_loss"] = grad_n8m24()
+        with open(os.path.join(HERE, "golden.json"), "w") as fh:
+            json.dump(rec, fh, indent=1, sort_keys=True)
+    else:
+        main()

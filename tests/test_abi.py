@@ -51,7 +51,10 @@ def test_plan_errors():
     # compute entry points validate before touching a device
     assert L.slos_forward(h, None, 1, None, 3, 0, None, 1.0, 0, None, 0, None) == nat.SLOS_EINVAL
     wsb = ctypes.c_size_t()
-    assert L.slos_forward_workspace(h, 2, 3, ctypes.byref(wsb)) == 0 and wsb.value >= 2 * 8 * (6 + 21)
+    assert L.slos_forward_workspace(h, 2, 3, 0, ctypes.byref(wsb)) == 0 and wsb.value >= 2 * 8 * (6 + 21)
+    wsc = ctypes.c_size_t()
+    assert L.slos_forward_workspace(h, 2, 3, nat.FLAG_COMPENSATED, ctypes.byref(wsc)) == 0
+    assert wsc.value >= 2 * 2 * 8 * (6 + 21) and wsc.value > wsb.value          # hi/lo layers
     idx = (ctypes.c_uint64 * 1)(56)
     pos = (ctypes.c_int32 * 3)()
     assert L.slos_unrank(h, 3, idx, 1, pos) == nat.SLOS_ERANGE
@@ -90,7 +93,7 @@ def test_group_host_checks():
     assert L.slos_group_create(3, three, ctypes.byref(g)) == nat.SLOS_OK
     B = (ctypes.c_int64 * 3)(2, 3, 1)
     size = ctypes.c_size_t()
-    assert L.slos_group_workspace(g, B, ctypes.byref(size)) == nat.SLOS_OK
+    assert L.slos_group_workspace(g, B, 0, ctypes.byref(size)) == nat.SLOS_OK
     # per plan: B ones + two ping-pong layers of B * max_{k<n} S_k complex64, each 256-byte aligned
     from math import comb
 
@@ -101,8 +104,11 @@ def test_group_host_checks():
         mx = max(comb(m + k - 1, k) for k in range(1, n))
         exp += a256(8 * b) + 2 * a256(8 * b * mx)
     assert size.value == exp
+    size_c = ctypes.c_size_t()                                                  # compensated: hi/lo layers
+    assert L.slos_group_workspace(g, B, nat.FLAG_COMPENSATED, ctypes.byref(size_c)) == nat.SLOS_OK
+    assert size_c.value > size.value
     B0 = (ctypes.c_int64 * 3)(2, 0, 1)
-    assert L.slos_group_workspace(g, B0, ctypes.byref(size)) == nat.SLOS_EINVAL
+    assert L.slos_group_workspace(g, B0, 0, ctypes.byref(size)) == nat.SLOS_EINVAL
     assert L.slos_group_replay(g, None) == nat.SLOS_EINVAL
     assert b"capture" in L.slos_last_error()
     blk, pw, gr = ctypes.c_int(), ctypes.c_uint64(), ctypes.c_uint32()

@@ -141,3 +141,102 @@ def test_gradient_matches_restatement(n, m, mask, occ):
     tot.backward()
     g, gr = U.grad.cpu().numpy(), Ur.grad.numpy()
     assert np.abs(g - gr).max() < 1e-5 * max(1.0, np.abs(gr).max())
+
+
+def test_compensated_probs_equal_with_and_without_grad():
+    """FockSLOSKernel(compensated=True): every layer is compensated whether or not U requires a
+    gradient (the autograd forward runs the same launches as the plain forward)."""
+    n, m = 7, 12
+    k = FockSLOSKernel(n, m, [1] * n + [0] * (m - n), device=DEV, compensated=True)
+    U = haar_unitary(m, 712).to(torch.complex64).to(DEV)
+    a = k.probs(U)
+    b = k.probs(U.clone().requires_grad_(True))
+    assert torch.equal(a, b.detach())
+
+
+def test_gradient_n8_m24_block_adjoint():
+    """n=8, m=24 (7.9M outputs; layers 6..8 run the block-structured adjoint): dL/dU of
+    L = -mean log p[idx] over 1,024 golden indices against complex128 torch autograd of the
+    stored-edge restatement (tests/golden/g8_*.npy)."""
+    U = torch.as_tensor(golden("g8_u.npy")).to(torch.complex64).to(DEV).requires_grad_(True)
+    idx = torch.as_tensor(golden("g8_idx.npy"), device=DEV)
+    k = FockSLOSKernel(8, 24, first_modes(8, 24), device=DEV)
+    loss = -torch.log(k.probs(U)[idx]).mean()
+    assert abs(float(loss.detach()) - golden("golden.json")["g8_loss"]) < 1e-3
+    loss.backward()
+    g, gr = U.grad.cpu().numpy(), golden("g8_grad.npy")
+    assert np.abs(g - gr).max() / np.abs(gr).max() < 1e-3
+    assert np.abs(g[:, 8:]).max() == 0.0
+
+
+def _restated_raw(Ur, n, m, occ, mask="fock"):
+    """Raw deferred amplitudes of the oracle's stored-edge recurrence in complex128 torch (autograd)."""
+    L = so.enumerate_layers(n, m, mask)
+    modes = [j for j, c in enumerate(occ) for _ in range(c)]
+    psi = torch.ones(1, dtype=torch.complex128)
+    for kk, i in enumerate(modes):
+        par, chi, mode, _ = so.edges(L[kk], L[kk + 1])
+        contrib = Ur[torch.as_tensor(mode), i] * psi[torch.as_tensor(par)]
+        psi = torch.zeros(L[kk + 1].shape[0], dtype=torch.complex128).index_add(0, torch.as_tensor(chi), contrib)
+    return psi, L
+
+
+@pytest.mark.parametrize("n,m,occ,norm", [(4, 7, [1, 1, 0, 1, 1, 0, 0], "deferred"), (3, 6, [2, 0, 1, 0, 0, 0], "deferred"),
+                                          (3, 5, [1, 0, 2, 0, 0], "immediate"), (7, 9, None, "deferred")])
+def test_amplitudes_gradient_matches_restatement(n, m, occ, norm):
+    """amplitudes(U) under autograd (kernel.py:191-260): the raw layer through the adjoint kernels,
+    the input norm and (immediate) sqrt(prod t!) as torch products, against complex128 autograd of
+    the oracle's recurrence, for a loss that uses the complex amplitudes themselves."""
+    occ = occ or [1] * n + [0] * (m - n)
+    U0 = haar_unitary(m, 30 + n + m)
+    k = FockSLOSKernel(n, m, occ, device=DEV, normalization=norm)
+    U = U0.to(torch.complex64).to(DEV).requires_grad_(True)
+    a = k.amplitudes(U)
+    S = a.shape[0]
+    c = torch.exp(1j * torch.linspace(0.0, 3.0, S, dtype=torch.float64)).to(torch.complex64)
+    w = torch.linspace(0.5, 1.5, S, dtype=torch.float32)
+    loss = (a * c.to(DEV)).real.sum() + (a.abs() ** 2 * w.to(DEV)).sum()
+    loss.backward()
+    Ur = U0.clone().requires_grad_(True)
+    psi, L = _restated_raw(Ur, n, m, occ)
+    ar = psi / math.sqrt(math.prod(math.factorial(v) for v in occ))
+    if norm == "immediate":
+        ar = ar * torch.sqrt(torch.as_tensor(so.prob_norms(L[n]), dtype=torch.float64))
+    assert np.abs(a.detach().cpu().numpy() - ar.detach().numpy()).max() < 1e-6
+    lr = (ar * c.to(torch.complex128)).real.sum() + (ar.abs() ** 2 * w.double()).sum()
+    lr.backward()
+    g, gr = U.grad.cpu().numpy(), Ur.grad.numpy()
+    assert np.abs(g - gr).max() < 2e-5 * max(1.0, np.abs(gr).max())
+
+
+@pytest.mark.parametrize("n,m,x,order", [(3, 6, 0.8, None), (4, 6, 0.6, None), (4, 7, 0.9, 1)])
+def test_xmodel_gradient_in_u_matches_finite_differences(n, m, x, order):
+    """probs(U, indist=x) is differentiable in U (species.py:37-41): the interfering subsets through
+    SLOSRaw, the raises as complex layer steps with their adjoint; dL/dU of L = sum w p against
+    complex128 central finite differences (h = 1e-6) of the oracle's species mixture."""
+    occ = [1] * n + [0] * (m - n)
+    U0 = haar_unitary(m, 70 + n + m)
+    k = FockSLOSKernel(n, m, occ, device=DEV)
+    U = U0.to(torch.complex64).to(DEV).requires_grad_(True)
+    p = k.probs(U, indist=x, order=order)
+    w = torch.linspace(0.5, 1.5, p.shape[-1], device=DEV)
+    (p * w).sum().backward()
+    wn = w.cpu().double().numpy()
+    Un = U0.numpy()
+
+    def L(Uv):
+        return float(so.species_mixture(Uv, n, m, occ, x, order=order) @ wn)
+
+    assert np.abs(p.detach().cpu().numpy() - so.species_mixture(Un, n, m, occ, x, order=order)).max() < 1e-6
+    h = 1e-6
+    g_fd = np.zeros((m, m), dtype=np.complex128)
+    for j in range(m):
+        for i in range(n):                               # only the input columns carry gradient
+            for unit in (1.0, 1j):
+                Up, Um = Un.copy(), Un.copy()
+                Up[j, i] += unit * h
+                Um[j, i] -= unit * h
+                g_fd[j, i] += (L(Up) - L(Um)) / (2 * h) * (1.0 if unit == 1.0 else 1j)
+    g = U.grad.cpu().numpy()
+    assert np.abs(g - g_fd).max() < 2e-5 * max(1.0, np.abs(g_fd).max())
+    assert np.abs(g[:, n:]).max() == 0.0

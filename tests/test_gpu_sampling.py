@@ -78,18 +78,58 @@ def test_sample_frontier_distribution():
     del p
 
 
-@pytest.mark.skip(reason="a GPU fault ended the box of the first run that executed this test (cause not identified by reading the kernels it calls; the device branch of LossyDistribution.sample is switched off); kept off the GPU")
 def test_small_space_is_torch_multinomial_and_loss():
+    """At most 2^24 outputs the draw is torch.multinomial on the probabilities' device, as in the
+    reference; the loss thinning of samples draws binomials on the host, so it takes a CPU
+    generator (a CUDA generator raises there, in the reference too)."""
     k = FockSLOSKernel(3, 6, [1, 0, 1, 0, 1, 0], device=DEV)
     p = k.probs(haar_unitary(6, 2).to(torch.complex64))
     a = sample_from_probs(k.layers, p, 500, generator=torch.Generator(device=DEV).manual_seed(3))
     ref_idx = torch.multinomial(p.double(), 500, replacement=True, generator=torch.Generator(device=DEV).manual_seed(3))
     assert torch.equal(a.cpu(), torch.as_tensor(k.layers.occupations(3))[ref_idx.cpu()])
-    t = sample_from_probs(k.layers, p, 20_000, generator=torch.Generator(device=DEV).manual_seed(4), loss=0.5)
+    t = sample_from_probs(k.layers, p.cpu(), 20_000, generator=torch.Generator().manual_seed(4), loss=0.5)
     assert abs(float(t.sum(1).double().mean()) - 1.5) < 0.05            # E[photons] = n eta
+    with pytest.raises(RuntimeError):
+        sample_from_probs(k.layers, p.cpu(), 10, generator=torch.Generator(device=DEV).manual_seed(4), loss=0.5)
 
 
-@pytest.mark.skip(reason="a GPU fault ended the box of the first run that executed this test (cause not identified by reading the kernels it calls; the device branch of LossyDistribution.sample is switched off); kept off the GPU")
+def test_lossy_distribution_sampled_on_device():
+    """n=9, m=24, eta=0.7 with a CUDA generator: the photon number of each shot from the FP64 block
+    masses, then the pattern inside its layer (layer 9 has 28M entries: the device inverse CDF;
+    the others torch.multinomial). The photon-number histogram follows Binomial(9, eta), every row
+    is a valid pattern of its layer, and the mean occupation of mode 0 within layer 9 matches the
+    conditional distribution's."""
+    n, m, eta = 9, 24, 0.7
+    k = FockSLOSKernel(n, m, [1] * n + [0] * (m - n), device=DEV)
+    ld = k.probs(haar_unitary(m, 9).to(torch.complex64), loss=eta)
+    assert ld.probs[n].numel() > MULTINOMIAL_MAX
+    shots = 200_000
+    s = ld.sample(shots, generator=torch.Generator(device=DEV).manual_seed(8))
+    assert s.shape == (shots, m) and s.device.type == "cuda" and torch.all(s >= 0)
+    tot = s.sum(1)
+    counts = np.bincount(tot.cpu().numpy(), minlength=n + 1)
+    pmf = np.array([math.comb(n, j) * eta ** j * (1 - eta) ** (n - j) for j in range(n + 1)])
+    e = pmf * shots
+    keep = e > 5
+    chi2 = (((counts - e) ** 2)[keep] / e[keep]).sum()
+    assert chi2 < 35.0                                                  # ~p = 1e-4 at <= 9 dof
+    rows = s[tot == n]
+    occ0 = rows[:, 0].double()
+    # E[t_0 | n photons] over layer n: the lex blocks of t_0 = c are contiguous ranges of the layer
+    q = ld.probs[n].double()
+    L = k.layers
+    expect, start = 0.0, 0
+    for c in range(n, -1, -1):                       # states with c photons in mode 0 come first, c descending
+        size = math.comb(n - c + m - 2, n - c)      # the rest: n - c photons in modes 1..m-1
+        expect += c * float(q[start:start + size].sum())
+        start += size
+    assert start == L.size(n)
+    expect /= float(q.sum())
+    sd = float(occ0.std()) / math.sqrt(occ0.numel())
+    assert abs(float(occ0.mean()) - expect) < 6 * sd
+
+
+@pytest.mark.skip(reason="the GPU service reported a fault after the one run of this test with a device branch of LossyDistribution.sample that no longer exists; its kernels are bounds-checked on the CPU at these sizes (test_sanitize.py) but no cause was shown, so the case stays off the GPU")
 def test_lossy_distribution_device_sampling():
     """n=10, m=20 with loss: 20M final patterns (> 2^24 over the layers) sampled on the device; the
     photon-number histogram follows Binomial(10, eta)."""

@@ -76,11 +76,18 @@ def test_order_truncation_batch_gradient_and_loss():
     assert abs(float(sum(blk.double().sum() for blk in ld.probs)) - 1) < 1e-5
 
 
-def test_u_gradient_is_refused_on_this_path():
+def test_u_gradient_flows_on_this_path():
+    """U requiring a gradient: the same probabilities as without (to float32 rounding; the
+    differentiable path squares the raw amplitudes in torch), and a finite gradient confined to the
+    input columns (its values are checked against finite differences in test_gpu_loss_grad.py)."""
     k = FockSLOSKernel(3, 5, [1, 1, 1, 0, 0], device=DEV)
-    U = haar_unitary(5, 1).to(torch.complex64).requires_grad_(True)
-    with pytest.raises(NotImplementedError):
-        k.probs(U, indist=0.5)
+    U0 = haar_unitary(5, 1).to(torch.complex64)
+    U = U0.clone().requires_grad_(True)
+    p = k.probs(U, indist=0.5)
+    assert torch.allclose(p.detach().cpu(), k.probs(U0, indist=0.5).cpu(), atol=1e-7)
+    (p * torch.linspace(0.5, 1.5, p.shape[0], device=p.device)).sum().backward()
+    assert torch.isfinite(U.grad).all() and U.grad.abs().max() > 1e-3
+    assert U.grad[:, 3:].abs().max() == 0
 
 
 @pytest.mark.parametrize("n,m,mask,occ", [(6, 9, "fock", None), (7, 6, "fock", [2, 0, 1, 3, 1, 0]),

@@ -65,3 +65,44 @@ def test_grouped_probabilities():
     assert torch.equal(ident, k.probs(U.to(torch.complex64)))
     with pytest.raises(ValueError):
         k.group_matrix(torch.zeros(S + 1, dtype=torch.int64))
+
+
+def test_superposition_gradient_matches_finite_differences():
+    """SuperpositionSLOSKernel under autograd (the reference's kernel is a torch graph,
+    kernel.py:369-420): dL/dU and dL/dw of L = sum v p against complex128 central finite
+    differences (h = 1e-6) of the oracle's superposition_probs."""
+    n, m = 3, 6
+    occs = [[1, 1, 1, 0, 0, 0], [2, 0, 0, 1, 0, 0], [0, 0, 1, 1, 1, 0]]
+    U0 = haar_unitary(m, 41)
+    rng = np.random.default_rng(3)
+    w0 = rng.normal(size=3) + 1j * rng.normal(size=3)
+    k = SuperpositionSLOSKernel(n, m, occs, device=DEV)
+    U = U0.to(torch.complex64).to(DEV).requires_grad_(True)
+    w = torch.as_tensor(w0, dtype=torch.complex64, device=DEV).requires_grad_(True)
+    p = k.probs(U, weights=w)
+    v = torch.linspace(0.5, 1.5, p.shape[0], device=DEV)
+    (p * v).sum().backward()
+    vn = v.cpu().double().numpy()
+
+    def L(Uv, wv):
+        return float(so.superposition_probs(Uv, n, m, occs, wv) @ vn)
+
+    h = 1e-6
+    Un = U0.numpy()
+    gU = np.zeros((m, m), dtype=np.complex128)
+    for j in range(m):
+        for i in range(m):
+            for unit in (1.0, 1j):
+                Up, Um = Un.copy(), Un.copy()
+                Up[j, i] += unit * h
+                Um[j, i] -= unit * h
+                gU[j, i] += (L(Up, w0) - L(Um, w0)) / (2 * h) * (1.0 if unit == 1.0 else 1j)
+    gw = np.zeros(3, dtype=np.complex128)
+    for c in range(3):
+        for unit in (1.0, 1j):
+            wp, wm = w0.copy(), w0.copy()
+            wp[c] += unit * h
+            wm[c] -= unit * h
+            gw[c] += (L(Un, wp) - L(Un, wm)) / (2 * h) * (1.0 if unit == 1.0 else 1j)
+    assert np.abs(U.grad.cpu().numpy() - gU).max() < 2e-5 * max(1.0, np.abs(gU).max())
+    assert np.abs(w.grad.cpu().numpy() - gw).max() < 2e-5 * max(1.0, np.abs(gw).max())

@@ -365,3 +365,19 @@ def test_superposition_oracle_pins():
     rec = np.abs(sum(wk * a for wk, a in zip(w, amps))) ** 2 * norms
     assert np.abs(sup - rec).max() < 1e-13
     assert abs(sup.sum() - 1) < 1e-12
+
+
+def test_stored_edge_restatement_autograd_is_the_fd_gradient():
+    """The torch stored-edge restatement (oracle/stored_edge.py), whose complex128 autograd made the
+    n=8, m=24 gradient fixture: its probabilities equal the oracle's at C1, and its autograd
+    gradient at C4 equals the complex128 central finite differences of the golden C4 fixture."""
+    from oracle.stored_edge import StoredEdgeSLOS
+    U1 = golden("c1_u.npy")
+    p1 = StoredEdgeSLOS(3, 12, [1, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0], dtype=torch.complex128).probs(torch.as_tensor(U1))
+    assert np.abs(p1.numpy() - golden("c1_probs.npy")).max() < 1e-15
+    U4 = torch.as_tensor(golden("c4_u.npy")).requires_grad_(True)
+    idx = torch.as_tensor(golden("c4_idx.npy"))
+    p4 = StoredEdgeSLOS(6, 16, [1] * 6 + [0] * 10, dtype=torch.complex128).probs(U4)
+    (-torch.log(p4[idx]).mean()).backward()
+    g_fd = golden("c4_grad_fd.npy")
+    assert np.abs(U4.grad.numpy() - g_fd).max() < 1e-6 * np.abs(g_fd).max()
