@@ -120,6 +120,19 @@ int slos_forward(const slos_plan* plan, const void* U, int64_t batch, const int3
                  int layer, int out_kind, void* out, double scale, unsigned flags,
                  void* workspace, size_t workspace_bytes, slos_stream_t stream);
 
+/* slos_forward without a workspace argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. It calls no hipMalloc / hipFree: the intermediate layers live in a
+ * fixed 64 MiB device array of the library, one per GPU, held until the process ends; a forward
+ * whose workspace (slos_forward_workspace) exceeds 64 MiB fails with SLOS_ERANGE (n = 8, m = 24
+ * needs 21 MB). The plan's tables must already be resident on the current GPU (any earlier
+ * slos_forward or slos_layer_step of the plan there), else SLOS_EINVAL. Every call on one GPU
+ * shares that array: calls that overlap on one GPU (different streams, or threads) race. Results
+ * equal slos_forward's bit for bit (same kernels). */
+int slos_forward_noalloc(const slos_plan* plan, const void* U, int64_t batch, const int32_t* modes,
+                         int layer, int out_kind, void* out, double scale, unsigned flags,
+                         slos_stream_t stream);
+
 /* One layer of the recurrence (layer k from layer k-1 with input column `col`): the building
  * block of a checkpointed forward under autograd (kernel.py:40-80 recomputes layers the same
  * way). psi_prev: device complex64 [batch][S_{k-1}]; out: SLOS_OUT_PROBS / SLOS_OUT_AMPS as in

@@ -35,6 +35,8 @@ SIGNATURES = {
     "slos_forward_workspace": (_c.c_int, [_vp, _c.c_int64, _c.c_int, _c.c_uint, _c.POINTER(_c.c_size_t)]),
     "slos_forward": (_c.c_int, [_vp, _vp, _c.c_int64, _i32p, _c.c_int, _c.c_int, _vp, _c.c_double, _c.c_uint,
                                 _vp, _c.c_size_t, _vp]),
+    "slos_forward_noalloc": (_c.c_int, [_vp, _vp, _c.c_int64, _i32p, _c.c_int, _c.c_int, _vp, _c.c_double,
+                                        _c.c_uint, _vp]),
     "slos_layer_step": (_c.c_int, [_vp, _vp, _c.c_int64, _c.c_int, _c.c_int, _vp, _c.c_int, _vp, _c.c_double,
                                    _c.c_uint, _vp]),
     "slos_readout_adjoint": (_c.c_int, [_vp, _c.c_int64, _vp, _vp, _c.c_double, _vp, _vp]),

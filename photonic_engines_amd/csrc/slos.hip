@@ -1241,7 +1241,8 @@ int slos_forward_workspace(const slos_plan* p, int64_t batch, int layer, unsigne
 
 namespace {
 
-// argument checks of slos_forward: the input columns and the workspace layout
+// argument checks shared by slos_forward and slos_forward_noalloc: the input columns and the
+// workspace layout
 int forward_check(const slos_plan* p, const void* U, int64_t batch, const int32_t* modes, int layer, int out_kind,
                   void* out, unsigned flags, int* cols, WsLayout* w) {
   if (!p || !U || !out) return fail(SLOS_EINVAL, "NULL argument");
@@ -1286,6 +1287,56 @@ int forward_run(const slos_plan* p, const DevTables& dt, const void* U, int64_t 
 
 }  // namespace
 
+// Scratch of slos_forward_noalloc: the vacuum row and the two ping-pong layers, one copy per GPU
+// (held until the process ends). Host code never passes the symbol itself to a kernel: its device
+// address comes from hipGetSymbolAddress, once per device.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+struct ScratchSlot {
+  int device;
+  void* ptr;
+};
+std::mutex g_scratch_mu;
+std::vector<ScratchSlot> g_scratch;
+
+// device address of the current GPU's scratch array (looked up on the first call on each GPU)
+int scratch_address(void** out) {
+  int d = 0;
+  HIP_TRY(hipGetDevice(&d));
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  for (const auto& s : g_scratch)
+    if (s.device == d) {
+      *out = s.ptr;
+      return SLOS_OK;
+    }
+  void* ptr = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fail(SLOS_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+  g_scratch.push_back(ScratchSlot{d, ptr});
+  *out = ptr;
+  return SLOS_OK;
+}
+
+// the plan's tables on the current GPU if they are already resident (no allocation)
+int resident_tables(const slos_plan* cp, DevTables* out) {
+  slos_plan* p = const_cast<slos_plan*>(cp);
+  int d = 0;
+  HIP_TRY(hipGetDevice(&d));
+  std::lock_guard<std::mutex> g(p->mu);
+  for (auto& t : p->dev)
+    if (t.device == d) {
+      *out = t;
+      return SLOS_OK;
+    }
+  return fail(SLOS_EINVAL, "the plan has no tables on this GPU yet: run slos_forward (or slos_layer_step) with "
+                           "this plan once on it first");
+}
+
+}  // namespace
+
 int slos_forward(const slos_plan* p, const void* U, int64_t batch, const int32_t* modes, int layer, int out_kind,
                  void* out, double scale, unsigned flags, void* ws, size_t ws_bytes, slos_stream_t stream) {
   int cols[kMaxPhotons] = {};  // layer <= n <= 32 (checked before cols is written)
@@ -1296,6 +1347,26 @@ int slos_forward(const slos_plan* p, const void* U, int64_t batch, const int32_t
     return fail(SLOS_ENOMEM, "workspace too small: need " + std::to_string(w.total) + " bytes");
   DevTables dt;
   rc = device_tables(p, &dt);
+  if (rc) return rc;
+  return forward_run(p, dt, U, batch, cols, layer, out_kind, out, scale, flags, w, ws,
+                     reinterpret_cast<hipStream_t>(stream));
+}
+
+int slos_forward_noalloc(const slos_plan* p, const void* U, int64_t batch, const int32_t* modes, int layer,
+                         int out_kind, void* out, double scale, unsigned flags, slos_stream_t stream) {
+  int cols[kMaxPhotons] = {};  // layer <= n <= 32 (checked before cols is written)
+  WsLayout w;
+  int rc = forward_check(p, U, batch, modes, layer, out_kind, out, flags, cols, &w);
+  if (rc) return rc;
+  if (w.total > kNoAllocScratchBytes)
+    return fail(SLOS_ERANGE, "slos_forward_noalloc: the intermediate layers need " + std::to_string(w.total) +
+                                 " bytes, above its fixed scratch of " + std::to_string(kNoAllocScratchBytes) +
+                                 " bytes; use slos_forward with a workspace");
+  DevTables dt;
+  rc = resident_tables(p, &dt);
+  if (rc) return rc;
+  void* ws = nullptr;
+  rc = scratch_address(&ws);
   if (rc) return rc;
   return forward_run(p, dt, U, batch, cols, layer, out_kind, out, scale, flags, w, ws,
                      reinterpret_cast<hipStream_t>(stream));
