@@ -70,6 +70,24 @@ struct slos_plan {
 
 namespace {
 
+// LDS one work group may use on the current GPU (hipDeviceAttributeMaxSharedMemoryPerBlock; 160 KB
+// on gfx950), read once per device. Should the query fail, 64 KB (the LDS of every earlier CDNA
+// part) is assumed, so launches only get more conservative.
+size_t lds_limit() {
+  static std::mutex mu;
+  static std::vector<std::pair<int, size_t>> seen;
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess) return 64 * 1024;
+  std::lock_guard<std::mutex> g(mu);
+  for (const auto& e : seen)
+    if (e.first == d) return e.second;
+  int v = 0;
+  size_t lim = 64 * 1024;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, d) == hipSuccess && v > 0) lim = (size_t)v;
+  seen.emplace_back(d, lim);
+  return lim;
+}
+
 int device_tables(const slos_plan* cp, DevTables* out) {
   slos_plan* p = const_cast<slos_plan*>(cp);
   int d = 0;
@@ -491,10 +509,10 @@ __global__ __launch_bounds__(64) void k_reduce_gu(const double2* __restrict__ pa
 int block_maxd(int d) { return d <= 4 ? 4 : d <= 8 ? d : 16; }
 
 // dynamic LDS of the single-plan block kernel for layer K with suffix length suf, or 0 if it
-// does not fit 64 KB
+// does not fit the GPU's LDS
 size_t block_lds_fit(int K, int me, int suf) {
   const size_t sh = slos::block_dyn_lds(K, me, block_maxd(K - suf));
-  return sh + slos::kBlockStaticLds <= 64 * 1024 ? sh : 0;
+  return sh + slos::kBlockStaticLds <= lds_limit() ? sh : 0;
 }
 
 // suffix length of the single-plan block kernel for layer K: 5-position suffixes where their table
@@ -553,7 +571,7 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
     const uint32_t gn = dt.n_items[a.K];
     const size_t shg = slos::group_dyn_lds(a.K, p->me);
     const uint64_t gwgs = ((uint64_t)gn + slos::kWaves - 1) / slos::kWaves;
-    if (shg + slos::kBlockStaticLds <= 64 * 1024 && gwgs <= 0x7fffffffull) {
+    if (shg + slos::kBlockStaticLds <= lds_limit() && gwgs <= 0x7fffffffull) {
       L.xcd_chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 20) / ((uint64_t)slos::kWaves * slos::kGroupItemChildren));
       const dim3 ggrid((unsigned)gwgs, (unsigned)a.B);
       const int maxd = a.K - suf;
@@ -575,7 +593,7 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
     }
   }
   const size_t sh = block_lds_fit(a.K, p->me, suf);
-  if (!sh) return fail(SLOS_ERANGE, "block kernel LDS exceeds 64 KB");
+  if (!sh) return fail(SLOS_ERANGE, "block kernel LDS exceeds the GPU's LDS");
   dim3 grid((unsigned)wgs, (unsigned)a.B);
   const int maxd = a.K - suf;
   if (kind == 0) comp ? launch_block_kind<0, true>(maxd, suf, grid, sh, st, dt, p, a.par, a.out, L)
@@ -604,7 +622,7 @@ int launch_layer(const slos_plan* p, const DevTables& dt, int K, int col, const 
       block_lds_fit(K, p->me, slos::kSuf))
     return launch_block(p, dt, a, kind, comp, st);
   const size_t sh = (((size_t)(K + 1) * (p->me + 1) * 8 + p->me * 4 + 15) & ~(size_t)15) + (size_t)p->m * 8;
-  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   const uint64_t blocks = std::min<uint64_t>((S + 255) / 256, 1u << 20);
   dim3 grid((unsigned)std::max<uint64_t>(blocks, 1), (unsigned)std::min<int64_t>(B, 65535));
   if (kind == 0 && !comp) hipLaunchKernelGGL((k_layer_simple<0, false>), grid, dim3(256), sh, st, a);
@@ -951,7 +969,7 @@ bool group_can_block(const slos_group* G, const std::vector<DevTables>& dts, int
       off = std::max(off, slos::block_table_offset(K, G->plans[g]->me));
     }
   const size_t sh = off + (size_t)slos::kWaves * slos::kBlockBatch * slos::block_words(block_maxd(K - slos::kSuf)) * 4;
-  return sh + slos::kBlockStaticLds <= 64 * 1024;
+  return sh + slos::kBlockStaticLds <= lds_limit();
 }
 
 template <bool COMP>
@@ -1029,7 +1047,7 @@ int group_layer(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w
     gl.table_off = (uint32_t)sh;
     sh += (size_t)slos::kWaves * slos::kBlockBatch * slos::block_words(block_maxd(K - slos::kSuf)) * 4;
   }
-  if (sh + (block ? slos::kBlockStaticLds : 0) > 64 * 1024) return fail(SLOS_ERANGE, "group layer LDS exceeds 64 KB");
+  if (sh + (block ? slos::kBlockStaticLds : 0) > lds_limit()) return fail(SLOS_ERANGE, "group layer LDS exceeds the GPU's LDS");
   const uint32_t all = (uint32_t)(8 * ((total + 7) / 8));
   const uint32_t grid = c.grid ? std::min<uint32_t>(std::max<uint32_t>(c.grid & ~7u, 8), all) : all;
   const bool comp = a.flags & SLOS_FLAG_COMPENSATED;
@@ -1400,7 +1418,7 @@ int slos_readout_adjoint(const slos_plan* p, int64_t batch, const void* amps, co
   if (rc) return rc;
   const uint64_t S = p->size(p->n);
   const size_t sh = (size_t)(p->n + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
-  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((S + 255) / 256, 1u << 20)),
             (unsigned)std::min<int64_t>(batch, 16));
   hipLaunchKernelGGL(k_readout_adjoint, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo,
@@ -1437,7 +1455,7 @@ int adjoint_launch(const slos_plan* p, int k, AdjLaunch* L) {
   if (p->multiset && !p->tuples.empty() && dp >= 1 && dp <= 8 && p->me <= 32) {
     const int maxd = dp <= 4 ? 4 : dp <= 6 ? 6 : 8;
     const size_t shb = slos::desc_dyn_lds(k, p->me, maxd);
-    if (shb + sizeof(double) * 2 * slos::kWaves * 32 <= 64 * 1024) {
+    if (shb + sizeof(double) * 2 * slos::kWaves * 32 <= lds_limit()) {
       // parents per wave: at least 2048, and at most about 2^16 work groups (FP64 partials per mode)
       uint64_t pw = (Sp / ((uint64_t)slos::kWaves << 16) + 63) & ~63ull;
       pw = std::max<uint64_t>(pw, 2048);
@@ -1454,7 +1472,7 @@ int adjoint_launch(const slos_plan* p, int k, AdjLaunch* L) {
   }
   const size_t sh = (((size_t)(k + 1) * (p->me + 1) * 8 + p->me * 4 + 15) & ~(size_t)15) + (size_t)p->m * 8 +
                     (size_t)kAdjThreads * p->me * 8;
-  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "adjoint LDS exceeds 64 KB (too many modes)");
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "adjoint LDS exceeds the GPU's LDS (too many modes)");
   L->block = false;
   L->lds = sh;
   L->nblk = adjoint_blocks(Sp);
@@ -1532,7 +1550,7 @@ int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1,
   if (!(flags & SLOS_FLAG_SIMPLE) && p->multiset && dt.tuples && dp >= 1 && dp <= 8) {
     const int maxd = dp <= 4 ? 4 : dp <= 6 ? 6 : 8;
     const size_t shb = slos::desc_dyn_lds(k, p->me, maxd);
-    if (shb <= 64 * 1024) {
+    if (shb <= lds_limit()) {
       uint64_t pw = (Sp / 16384 + 63) & ~63ull;
       pw = std::min<uint64_t>(std::max<uint64_t>(pw, 256), 4096);
       const uint64_t wgs = ((Sp + pw - 1) / pw + slos::kWaves - 1) / slos::kWaves;
@@ -1558,7 +1576,7 @@ int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1,
     }
   }
   const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
-  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 255) / 256, 1u << 20)), (unsigned)batch);
   hipLaunchKernelGGL(k_descend, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
                      p->multiset, k, q_k, Sk, q_km1, Sp, batch, (float)scale);
@@ -1576,7 +1594,7 @@ int slos_raise(const slos_plan* p, int k, const float* w, const float* q_km1, fl
   if (rc) return rc;
   const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
   const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4 + (size_t)p->m * 4;
-  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sk + 255) / 256, 1u << 20)), (unsigned)batch);
   hipLaunchKernelGGL(k_raise, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
                      p->multiset, k, p->m, w, q_km1, Sp, q_k, Sk, batch);
@@ -1593,7 +1611,7 @@ int slos_loss_ascend(const slos_plan* p, int k, const float* g_km1, float* g_k, 
   if (rc) return rc;
   const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
   const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
-  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sk + 255) / 256, 1u << 20)), (unsigned)batch);
   hipLaunchKernelGGL(k_ascend, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
                      p->multiset, k, g_km1, Sp, g_k, Sk, batch);
@@ -1682,7 +1700,7 @@ int slos_unrank_device(const slos_plan* p, int k, const uint64_t* idx, int64_t c
   int rc = device_tables(p, &dt);
   if (rc) return rc;
   const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
-  if (sh > 64 * 1024) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   const uint64_t blocks = std::min<uint64_t>(((uint64_t)count + 255) / 256, 1u << 16);
   hipLaunchKernelGGL(k_unrank_occ, dim3((unsigned)blocks), dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg,
                      dt.lo, p->me, k, p->m, idx, count, occ);
@@ -1702,7 +1720,7 @@ int slos_group_create(int count, const slos_plan* const* plans, slos_group** out
       delete G;
       return fail(SLOS_EINVAL, "NULL plan");
     }
-    if (simple_lds_bytes(p->n, p->me, p->m) > 64 * 1024) {
+    if (simple_lds_bytes(p->n, p->me, p->m) > lds_limit()) {
       delete G;
       return fail(SLOS_ERANGE, "plan too large for a group launch (LDS)");
     }

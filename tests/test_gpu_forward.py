@@ -222,3 +222,29 @@ def test_frontier_shape_mass_and_spot_checks():
     assert rel_c.max() < rel.max(), (rel_c.max(), rel.max())
     assert np.median(rel_c) < np.median(rel), (np.median(rel_c), np.median(rel))
     assert abs(float(kc.total_mass(pc)) - 1) < 1e-4
+
+
+def _phase_dft(m: int, seed: int) -> torch.Tensor:
+    """A cheap m x m unitary for wide-mode tests: diag(phases) . DFT / sqrt(m) . diag(phases)."""
+    g = np.random.default_rng(seed)
+    d1 = np.exp(2j * np.pi * g.random(m))
+    d2 = np.exp(2j * np.pi * g.random(m))
+    k = np.arange(m)
+    F = np.exp(-2j * np.pi * np.outer(k, k) / m) / math.sqrt(m)
+    return torch.as_tensor((d1[:, None] * F) * d2[None, :])
+
+
+def test_two_photons_wide_modes_counting_table_above_64k():
+    """n=2, m=3000 (4.5M outputs): the counting table, U column and mode list need 108 KB of LDS,
+    more than the 64 KB of earlier CDNA parts; every probability against the closed form
+    |U[i,0] U[j,1] + U[j,0] U[i,1]|^2 (i < j), 2 |U[i,0] U[i,1]|^2 (i = j), in lex order of (i, j)."""
+    m = 3000
+    U = _phase_dft(m, 3000)
+    p = FockSLOSKernel(2, m, [1, 1] + [0] * (m - 2), device=DEV).probs(U.to(torch.complex64)).double().cpu().numpy()
+    i, j = np.triu_indices(m)
+    u0, u1 = U[:, 0].numpy(), U[:, 1].numpy()
+    ref = np.abs(u0[i] * u1[j] + u0[j] * u1[i]) ** 2
+    ref[i == j] /= 2
+    assert p.shape == ref.shape
+    _check(torch.as_tensor(p), ref, tol=1e-9, what="n=2 m=3000")
+    assert abs(p.sum() - 1) < 1e-4

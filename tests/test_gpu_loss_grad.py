@@ -240,3 +240,26 @@ def test_xmodel_gradient_in_u_matches_finite_differences(n, m, x, order):
     g = U.grad.cpu().numpy()
     assert np.abs(g - g_fd).max() < 2e-5 * max(1.0, np.abs(g_fd).max())
     assert np.abs(g[:, n:]).max() == 0.0
+
+
+def test_gradient_wide_modes_adjoint_lds_above_64k():
+    """n=2, m=200: the thread adjoint's per-lane mode sums take 102 KB of LDS (more than the 64 KB of
+    earlier CDNA parts). dL/dU of L = sum_s w_s p_s against complex128 torch autograd of the closed
+    form p(i, j) = |U[i,0] U[j,1] + U[j,0] U[i,1]|^2 (i < j), 2 |U[i,0] U[i,1]|^2 (i = j)."""
+    m = 200
+    U0 = haar_unitary(m, 200)
+    k = FockSLOSKernel(2, m, [1, 1] + [0] * (m - 2), device=DEV)
+    U = U0.to(torch.complex64).to(DEV).requires_grad_(True)
+    p = k.probs(U)
+    w = torch.linspace(0.5, 1.5, p.shape[-1], device=DEV)
+    (p * w).sum().backward()
+    i, j = np.triu_indices(m)
+    i, j = torch.as_tensor(i), torch.as_tensor(j)
+    Ur = U0.clone().requires_grad_(True)
+    amp = Ur[i, 0] * Ur[j, 1] + Ur[j, 0] * Ur[i, 1]
+    pr = amp.abs() ** 2 / torch.where(i == j, 2.0, 1.0).double()
+    assert np.abs(p.detach().double().cpu().numpy() - pr.detach().numpy()).max() < 1e-6
+    (pr * w.cpu().double()).sum().backward()
+    g, gr = U.grad.cpu().numpy(), Ur.grad.numpy()
+    assert np.abs(g - gr).max() < 1e-5 * max(1.0, np.abs(gr).max())
+    assert np.abs(g[:, 2:]).max() == 0.0
