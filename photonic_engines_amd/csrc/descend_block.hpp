@@ -43,6 +43,12 @@ struct DescCtx {
   float invk, scale;
 };
 
+// Modes are taken in chunks of kDescChunk: a chunk's child loads are all issued before its sums
+// (ascending modes throughout, so the sums and their order are those of the thread kernel). A chunk
+// keeps only its own per-mode wave-uniform values (stream bases, rank terms, copies) in scalar
+// registers; loading every mode's before the first sum needs ~7 scalars a mode and spills them.
+constexpr int kDescChunk = 8;
+
 // q_K of one child (mode a) of each parent held by a lane, one parent per lane
 template <int ME>
 __device__ __forceinline__ void descend_step1(const DescCtx& c, uint32_t uc) {
@@ -53,30 +59,36 @@ __device__ __forceinline__ void descend_step1(const DescCtx& c, uint32_t uc) {
   const int sx[kSuf] = {(int)(w & 63u), (int)((w >> 8) & 63u), (int)((w >> 16) & 63u), (int)((w >> 24) & 63u)};
   DescLane dl;
   desc_lane_init(c.cgl, c.W, c.lastP, sx, dl);
-  // every child load first (ascending modes), then the sums in the same order
-  float qv[ME];
+  float acc = 0.f;
 #pragma unroll
-  for (int a = 0; a < ME; ++a) {
-    qv[a] = 0.f;
-    if (a < c.me) {
-      const uint64_t sba = rl64(c.sb, a);
-      if (a < c.lastP) {
-        qv[a] = at_bytes(c.qk + sba, uu * 4u);
-      } else {
-        const uint32_t loc = desc_local_offset(dl, sx, a, rl(c.tA0, a), rl(c.tA1, a), rl(c.tA2, a), rl(c.tA3, a));
-        const uint64_t idx = a <= sx[0] ? sba + uu : c.CB0 + loc;
-        qv[a] = c.qk[idx];
+  for (int a0 = 0; a0 < ME; a0 += kDescChunk) {
+    if (a0 >= c.me) break;
+    float qv[kDescChunk];
+#pragma unroll
+    for (int k = 0; k < kDescChunk; ++k) {
+      const int a = a0 + k;
+      qv[k] = 0.f;
+      if (a < c.me) {
+        const uint64_t sba = rl64(c.sb, a);
+        if (a < c.lastP) {
+          qv[k] = at_bytes(c.qk + sba, uu * 4u);
+        } else {
+          const uint32_t loc = desc_local_offset(dl, sx, a, rl(c.tA0, a), rl(c.tA1, a), rl(c.tA2, a), rl(c.tA3, a));
+          const uint64_t idx = a <= sx[0] ? sba + uu : c.CB0 + loc;
+          qv[k] = c.qk[idx];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kDescChunk; ++k) {
+      const int a = a0 + k;
+      if (a < c.me) {
+        const float cnt = a < c.lastP ? __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, c.scnt), a))
+                                      : (float)desc_tail_copies(sx, a, c.lastP, c.lastrun);
+        acc = fmaf(qv[k], cnt, acc);
       }
     }
   }
-  float acc = 0.f;
-#pragma unroll
-  for (int a = 0; a < ME; ++a)
-    if (a < c.me) {
-      const float cnt = a < c.lastP ? __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, c.scnt), a))
-                                    : (float)desc_tail_copies(sx, a, c.lastP, c.lastrun);
-      acc = fmaf(qv[a], cnt, acc);
-    }
   if (in) c.qout[c.CBp + u] = (acc * c.invk) * c.scale;
 }
 
@@ -94,39 +106,46 @@ __device__ __forceinline__ void descend_step2(const DescCtx& c, uint32_t uc) {
   DescLane d0, d1;
   desc_lane_init(c.cgl, c.W, c.lastP, s0x, d0);
   desc_lane_init(c.cgl, c.W, c.lastP, s1x, d1);
-  float q0[ME], q1[ME];
+  float acc0 = 0.f, acc1 = 0.f;
 #pragma unroll
-  for (int a = 0; a < ME; ++a) {
-    q0[a] = q1[a] = 0.f;
-    if (a < c.me) {
-      const uint64_t sba = rl64(c.sb, a);
-      if (a < c.lastP || a <= s0min) {
-        float2 v;
-        __builtin_memcpy(&v, reinterpret_cast<const char*>(c.qk + sba) + u * 4u, 8);
-        q0[a] = v.x;
-        q1[a] = v.y;
-      } else {
-        const uint32_t A0 = rl(c.tA0, a), A1 = rl(c.tA1, a), A2 = rl(c.tA2, a), A3 = rl(c.tA3, a);
-        const uint64_t i0 = a <= s0x[0] ? sba + u : c.CB0 + desc_local_offset(d0, s0x, a, A0, A1, A2, A3);
-        const uint64_t i1 = a <= s1x[0] ? sba + u + 1u : c.CB0 + desc_local_offset(d1, s1x, a, A0, A1, A2, A3);
-        q0[a] = c.qk[i0];
-        q1[a] = c.qk[i1];
+  for (int a0 = 0; a0 < ME; a0 += kDescChunk) {
+    if (a0 >= c.me) break;
+    float q0[kDescChunk], q1[kDescChunk];
+#pragma unroll
+    for (int k = 0; k < kDescChunk; ++k) {
+      const int a = a0 + k;
+      q0[k] = q1[k] = 0.f;
+      if (a < c.me) {
+        const uint64_t sba = rl64(c.sb, a);
+        if (a < c.lastP || a <= s0min) {
+          float2 v;
+          __builtin_memcpy(&v, reinterpret_cast<const char*>(c.qk + sba) + u * 4u, 8);
+          q0[k] = v.x;
+          q1[k] = v.y;
+        } else {
+          const uint32_t A0 = rl(c.tA0, a), A1 = rl(c.tA1, a), A2 = rl(c.tA2, a), A3 = rl(c.tA3, a);
+          const uint64_t i0 = a <= s0x[0] ? sba + u : c.CB0 + desc_local_offset(d0, s0x, a, A0, A1, A2, A3);
+          const uint64_t i1 = a <= s1x[0] ? sba + u + 1u : c.CB0 + desc_local_offset(d1, s1x, a, A0, A1, A2, A3);
+          q0[k] = c.qk[i0];
+          q1[k] = c.qk[i1];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kDescChunk; ++k) {
+      const int a = a0 + k;
+      if (a < c.me) {
+        if (a < c.lastP) {
+          const float cnt = __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, c.scnt), a));
+          acc0 = fmaf(q0[k], cnt, acc0);
+          acc1 = fmaf(q1[k], cnt, acc1);
+        } else {
+          acc0 = fmaf(q0[k], (float)desc_tail_copies(s0x, a, c.lastP, c.lastrun), acc0);
+          acc1 = fmaf(q1[k], (float)desc_tail_copies(s1x, a, c.lastP, c.lastrun), acc1);
+        }
       }
     }
   }
-  float acc0 = 0.f, acc1 = 0.f;
-#pragma unroll
-  for (int a = 0; a < ME; ++a)
-    if (a < c.me) {
-      if (a < c.lastP) {
-        const float cnt = __builtin_bit_cast(float, rl(__builtin_bit_cast(uint32_t, c.scnt), a));
-        acc0 = fmaf(q0[a], cnt, acc0);
-        acc1 = fmaf(q1[a], cnt, acc1);
-      } else {
-        acc0 = fmaf(q0[a], (float)desc_tail_copies(s0x, a, c.lastP, c.lastrun), acc0);
-        acc1 = fmaf(q1[a], (float)desc_tail_copies(s1x, a, c.lastP, c.lastrun), acc1);
-      }
-    }
   const float2 o = make_float2((acc0 * c.invk) * c.scale, (acc1 * c.invk) * c.scale);
   __builtin_memcpy(c.qout + c.CBp + u, &o, 8);
 }

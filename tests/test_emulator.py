@@ -37,7 +37,7 @@ FWD = [(4, 6, "fock", None), (5, 7, "fock", None), (6, 6, "fock", [3, 0, 2, 0, 1
        (7, 9, "fock", [1, 0, 2, 1, 0, 3, 0, 0, 0]), (8, 10, "fock", None), (6, 8, "fock", [6, 0, 0, 0, 0, 0, 0, 0]),
        (7, 12, "fock", None), (5, 20, "sparse_1in4", None), (9, 5, "fock", [2, 2, 2, 2, 1]), (10, 5, "fock", [3, 0, 4, 1, 2]),
        (11, 4, "fock", [5, 0, 3, 3]), (12, 4, "fock", [3, 3, 3, 3]), (13, 3, "fock", [5, 4, 4]), (9, 12, "unbunched", None),
-       (10, 20, "dual_rail", None), (5, 40, "unbunched", None)]
+       (10, 20, "dual_rail", None), (5, 40, "unbunched", None), (7, 20, "fock", None)]
 
 
 @pytest.mark.parametrize("suf", [4, 5, 6])
