@@ -437,6 +437,33 @@ def main() -> None:
         del psi, q
         return float(np.mean(ms))
 
+    def per_layer_ms(reps: int = 3) -> list:
+        """Each layer launch of the forward alone (the kernels probs(U) runs, on the same inputs),
+        from the layer below: the mean of `reps` launches bracketed by HIP events on their stream,
+        with the layer's algorithmic bytes (8 B per parent read, 8 B per complex64 child or 4 B per
+        probability written) over that time."""
+        rows = []
+        psi = torch.ones((1, 1), dtype=torch.complex64, device=dev)
+        for k in range(1, n + 1):
+            kind = nat.OUT_PROBS if k == n else nat.OUT_RAW
+            q = plan.layer_step(U, k, modes[k - 1], psi, kind, scale=1.0)
+            ms = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                q = plan.layer_step(U, k, modes[k - 1], psi, kind, scale=1.0, out=q)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms.append(e0.elapsed_time(e1))
+            t = float(np.mean(ms))
+            b = 8 * S[k - 1] + (4 if k == n else 8) * S[k]
+            rows.append({"layer": k, "ms": t, "alg_bytes": int(b), "alg_GBps": b / (t * 1e-3) / 1e9,
+                         "frac": b / (t * 1e-3) / 1e9 / PEAK_HBM_GBS})
+            psi = q if k < n else None
+            del q
+        torch.cuda.empty_cache()
+        return rows
+
     if args.copy_calibration_gb > 0:
         nbytes = int(args.copy_calibration_gb * 1e9) // 4 * 4
         src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
@@ -460,6 +487,7 @@ def main() -> None:
     del p
     torch.cuda.empty_cache()
     last_ms = last_layer_ms(0)
+    layers = per_layer_ms()
 
     last_bytes = 8 * S[n - 1] + 4 * S[n]
     achieved = last_bytes / (last_ms * 1e-3) / 1e9
@@ -488,7 +516,7 @@ def main() -> None:
                                 else f"k_layer_block (layer {n})"), "kernel_ms": last_ms,
                      "alg_bytes_per_launch": last_bytes},
         "whole_forward": {"alg_bytes": bytes_alg(n, m), "alg_GBps": bytes_alg(n, m) / dt / 1e9,
-                          "alg_frac": bytes_alg(n, m) / dt / 1e9 / PEAK_HBM_GBS},
+                          "alg_frac": bytes_alg(n, m) / dt / 1e9 / PEAK_HBM_GBS, "layers": layers},
         "accuracy": {"max_abs_dp": max_abs, "max_rel": max_rel, "spot_outputs": int(idx.size),
                      "oracle": "FP64 Ryser on the GPU", "mass_fp64": mass},
     }
