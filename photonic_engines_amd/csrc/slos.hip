@@ -16,6 +16,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/slos_mi355x.h"
 #include "space.hpp"
 #include "layer_block.hpp"
@@ -69,6 +71,16 @@ struct slos_plan {
 };
 
 namespace {
+
+// A roctx range over a scope (rocprofv3 --marker-trace shows it; without a profiler it is a no-op
+// call): the forward, each of its layers and each descent step are named ranges, so a trace ties
+// every kernel to its layer.
+struct RoctxRange {
+  explicit RoctxRange(const char* name) { roctxRangePushA(name); }
+  ~RoctxRange() { roctxRangePop(); }
+  RoctxRange(const RoctxRange&) = delete;
+  RoctxRange& operator=(const RoctxRange&) = delete;
+};
 
 // LDS one work group may use on the current GPU (hipDeviceAttributeMaxSharedMemoryPerBlock; 160 KB
 // on gfx950), read once per device. Should the query fail, 64 KB (the LDS of every earlier CDNA
@@ -1289,11 +1301,15 @@ int forward_run(const slos_plan* p, const DevTables& dt, const void* U, int64_t 
   uint64_t S_par = 1;
   // compensated sums: every intermediate layer is a hi/lo pair [2][B][S] (the buffers are sized for it)
   const bool hilo = (flags & SLOS_FLAG_COMPENSATED) != 0;
+  const RoctxRange whole("slos_forward");
   for (int k = 1; k <= layer; ++k) {
     const uint64_t S = p->size(k);
     const bool last = (k == layer);
     void* dst = last ? out : bufs[k & 1];
     const int kind = last ? (out_kind == SLOS_OUT_PROBS ? 1 : 2) : 0;
+    char name[32];
+    std::snprintf(name, sizeof name, "slos layer %d", k);
+    const RoctxRange range(name);
     rc = launch_layer(p, dt, k, cols[k - 1], Uf, batch, par, S_par, dst, S, kind, (float)scale, flags & ~SLOS_FLAG_HILO,
                       st, hilo && k >= 2, hilo && !last);
     if (rc) return rc;
@@ -1547,6 +1563,9 @@ int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1,
   if (rc) return rc;
   const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
   const int dp = k - 1 - slos::kSuf;
+  char name[32];
+  std::snprintf(name, sizeof name, "slos descent %d", k);
+  const RoctxRange range(name);
   if (!(flags & SLOS_FLAG_SIMPLE) && p->multiset && dt.tuples && dp >= 1 && dp <= 8) {
     const int maxd = dp <= 4 ? 4 : dp <= 6 ? 6 : 8;
     const size_t shb = slos::desc_dyn_lds(k, p->me, maxd);

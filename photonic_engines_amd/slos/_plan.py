@@ -71,6 +71,31 @@ class Plan:
                   "slos_forward_workspace")
         return int(out.value)
 
+    def memory_plan(self, batch: int, layer: int | None = None, out_kind: int = nat.OUT_PROBS, flags: int = 0) -> dict:
+        """Device bytes of one forward by formula, before anything is allocated: the output
+        [B, S_layer] (4 B a probability, 8 B a complex64 amplitude), the forward's workspace (the
+        vacuum row and the two ping-pong layers, slos_forward_workspace) and the unitaries (8 m^2 B
+        each)."""
+        layer = self.n if layer is None else int(layer)
+        out_b = int(batch) * self.sizes[layer] * (4 if out_kind == nat.OUT_PROBS else 8)
+        ws = self.workspace_bytes(batch, layer, flags)
+        u_b = int(batch) * self.m * self.m * 8
+        return {"output": out_b, "workspace": ws, "unitaries": u_b, "total": out_b + ws + u_b}
+
+    @staticmethod
+    def preflight(need: int, dev: torch.device, what: str, free_bytes=None) -> None:
+        """Raise torch.cuda.OutOfMemoryError (the reference's exception for this condition), with
+        the sizes, before anything is allocated or launched, if `need` bytes exceed the device's
+        free memory plus what torch's cache holds unused. `free_bytes()` replaces that query."""
+        if free_bytes is not None:
+            free = int(free_bytes())
+        else:
+            free = int(torch.cuda.mem_get_info(dev)[0]) + int(torch.cuda.memory_reserved(dev) -
+                                                               torch.cuda.memory_allocated(dev))
+        if need > free:
+            raise torch.cuda.OutOfMemoryError(
+                f"{what}: needs {need / 2**30:.2f} GiB more on {dev}, {free / 2**30:.2f} GiB are free")
+
     def forward(self, U: torch.Tensor, *, modes=None, layer: int | None = None, out_kind: int = nat.OUT_PROBS,
                 scale: float = 1.0, flags: int = 0, out: torch.Tensor | None = None) -> torch.Tensor:
         """U: complex64 [B, m, m] on the compute device -> [B, S_layer] float32 / complex64."""
@@ -80,10 +105,14 @@ class Plan:
         if U.dtype != torch.complex64 or not U.is_contiguous() or U.shape[1:] != (self.m, self.m):
             raise ValueError("U must be a contiguous complex64 [B, m, m] tensor")
         S = self.sizes[layer]
+        ws_bytes = self.workspace_bytes(B, layer, flags)
+        out_bytes = 0 if out is not None else B * S * (4 if out_kind == nat.OUT_PROBS else 8)
+        self.preflight(out_bytes + ws_bytes, dev, f"SLOS forward (n={self.n}, m={self.m}, batch {B}: "
+                       f"output {out_bytes:,} B, workspace {ws_bytes:,} B)")
         if out is None:
             out = torch.empty((B, S), dtype=torch.float32 if out_kind == nat.OUT_PROBS else torch.complex64,
                               device=dev)
-        ws = torch.empty(max(self.workspace_bytes(B, layer, flags), 1), dtype=torch.uint8, device=dev)
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
         mp = None
         if modes is not None:
             mp = (ctypes.c_int32 * layer)(*[int(i) for i in modes])

@@ -103,6 +103,12 @@ class FockSLOSKernel:
         """Size S of the output distribution (kernel.py:169-171)."""
         return self.layers.output_size()
 
+    def memory_plan(self, batch: int = 1) -> dict:
+        """Device bytes a probs(U) of `batch` unitaries needs (output, workspace, unitaries, total),
+        by formula; probs raises torch.cuda.OutOfMemoryError before launching when they exceed
+        the free device memory."""
+        return self._plan.memory_plan(batch, flags=self._flags())
+
     def _flags(self) -> int:
         f = nat.FLAG_COMPENSATED if self.compensated else 0
         if self.normalization == "immediate":

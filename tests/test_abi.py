@@ -128,3 +128,26 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_memory_plan_and_preflight():
+    """Memory pre-flight by formula, host only: the plan's byte counts, and a need above the free
+    memory raises torch.cuda.OutOfMemoryError (the reference's exception) before any allocation."""
+    import math
+
+    import torch
+
+    from photonic_engines_amd.slos._plan import Plan
+    p = Plan(6, 10, "fock", [0, 1, 2, 3, 4, 5])
+    mp = p.memory_plan(2)
+    assert mp["output"] == 2 * 4 * math.comb(15, 6)
+    assert mp["workspace"] == p.workspace_bytes(2, 6)
+    assert mp["unitaries"] == 2 * 10 * 10 * 8
+    assert mp["total"] == mp["output"] + mp["workspace"] + mp["unitaries"]
+    assert p.memory_plan(1, layer=3, out_kind=nat.OUT_RAW)["output"] == 8 * math.comb(12, 3)
+    dev = torch.device("cuda", 0)
+    Plan.preflight(mp["total"], dev, "fits", free_bytes=lambda: mp["total"])
+    with pytest.raises(torch.cuda.OutOfMemoryError, match="needs"):
+        Plan.preflight(mp["total"], dev, "too big", free_bytes=lambda: mp["total"] - 1)
+    big = Plan(12, 64, "fock", list(range(12)))
+    assert big.memory_plan(1)["output"] == 4 * math.comb(75, 12)              # ~105 TB of probabilities

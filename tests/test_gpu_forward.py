@@ -248,3 +248,16 @@ def test_two_photons_wide_modes_counting_table_above_64k():
     assert p.shape == ref.shape
     _check(torch.as_tensor(p), ref, tol=1e-9, what="n=2 m=3000")
     assert abs(p.sum() - 1) < 1e-4
+
+
+def test_impossible_size_raises_before_launch():
+    """n=12, m=64 needs ~105 TB of output: probs(U) raises torch.cuda.OutOfMemoryError from the
+    pre-flight, with the sizes, before it allocates or launches anything."""
+    k = FockSLOSKernel(12, 64, [1] * 12 + [0] * 52, device=DEV)
+    assert k.memory_plan(1)["total"] > 10 ** 14
+    U = torch.eye(64, dtype=torch.complex64, device=DEV)
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated()
+    with pytest.raises(torch.cuda.OutOfMemoryError, match="needs"):
+        k.probs(U)
+    assert torch.cuda.memory_allocated() == before
