@@ -634,11 +634,15 @@ __host__ __device__ inline size_t group_dyn_lds(int K, int me) {
 
 // one block of the group for the pair step: NS stream slots (slot 0 shared), both children;
 // KIND 1 writes probabilities (float32), KIND 0 the raw complex64 amplitudes of a middle layer
-template <int NS, int KIND>
+// COMP: compensated sums as the block kernel's sweep (two-sum accumulation; with plo the parents'
+// low-order parts at element offset plo, slot 0's in v0lo; with olo the stored value's rounding
+// error at element offset olo of a complex64 output).
+template <int NS, int KIND, bool COMP = false>
 __device__ __forceinline__ void group_block_pairs(const uint4 (&bw)[8], const float2* par, const float4& v0,
                                                   const float2 (&lc)[2][5], const uint32_t (&off)[2][5], uint32_t u,
                                                   const uint32_t (&code)[2], const bool (&lead)[2], const float* rn,
-                                                  float scale, void* out) {
+                                                  float scale, void* out, const float4& v0lo = float4{},
+                                                  int64_t plo = 0, int64_t olo = 0) {
   const uint64_t CB = (uint64_t)bw[0].x | ((uint64_t)bw[0].y << 32);
   const uint64_t LBp = (uint64_t)bw[0].z | ((uint64_t)bw[0].w << 32);
   const uint32_t w6 = bw[1].x;
@@ -662,20 +666,52 @@ __device__ __forceinline__ void group_block_pairs(const uint4 (&bw)[8], const fl
     for (int j = 0; j < 5; ++j) lv[c][j] = at_bytes(lp, off[c][j]);
   const uint32_t lastrow = ((w6 >> 16) & 0xffu) + 1u;
   float res[2][2];
+  BlockAcc accs[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     BlockAcc acc{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int s = 0; s < NS; ++s)
-      bk_cmac<false>(acc, coef[s], c == 0 ? make_float2(sv[s].x, sv[s].y) : make_float2(sv[s].z, sv[s].w));
+      bk_cmac<COMP>(acc, coef[s], c == 0 ? make_float2(sv[s].x, sv[s].y) : make_float2(sv[s].z, sv[s].w));
 #pragma unroll
-    for (int j = 0; j < 5; ++j) bk_cmac<false>(acc, lc[c][j], lv[c][j]);
-    res[c][0] = acc.re + acc.cre;
-    res[c][1] = acc.im + acc.cim;
+    for (int j = 0; j < 5; ++j) bk_cmac<COMP>(acc, lc[c][j], lv[c][j]);
+    accs[c] = acc;
+  }
+  if constexpr (COMP) {
+    if (plo) {  // the parents' low-order parts, into the correction terms (plain products)
+      float4 sl[NS];
+      sl[0] = v0lo;
+#pragma unroll
+      for (int s = 1; s < NS; ++s) {
+        const uint64_t base = (uint64_t)wd[10 + 4 * (s - 1)] | ((uint64_t)wd[11 + 4 * (s - 1)] << 32);
+        sl[s] = load_pair(par + plo + base, u * 8u);
+      }
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+          bk_lo_mac(accs[c], coef[s], c == 0 ? make_float2(sl[s].x, sl[s].y) : make_float2(sl[s].z, sl[s].w));
+#pragma unroll
+        for (int j = 0; j < 5; ++j) bk_lo_mac(accs[c], lc[c][j], at_bytes(lp + plo, off[c][j]));
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    res[c][0] = accs[c].re + accs[c].cre;
+    res[c][1] = accs[c].im + accs[c].cim;
   }
   if constexpr (KIND == 0) {
     const float4 v = make_float4(res[0][0], res[0][1], res[1][0], res[1][1]);
     __builtin_memcpy(static_cast<float2*>(out) + CB + u, &v, 16);
+    if constexpr (COMP) {
+      // the rounding error of the stored value (fast two-sum, as the block kernel)
+      if (olo) {
+        const float4 e = make_float4(accs[0].cre - (res[0][0] - accs[0].re), accs[0].cim - (res[0][1] - accs[0].im),
+                                     accs[1].cre - (res[1][0] - accs[1].re), accs[1].cim - (res[1][1] - accs[1].im));
+        __builtin_memcpy(static_cast<float2*>(out) + olo + CB + u, &e, 16);
+      }
+    }
   } else {
     float pr[2];
 #pragma unroll
@@ -689,10 +725,11 @@ __device__ __forceinline__ void group_block_pairs(const uint4 (&bw)[8], const fl
 }
 
 // the same for one child per lane (the item's last partial step)
-template <int NS, int KIND>
+template <int NS, int KIND, bool COMP = false>
 __device__ __forceinline__ void group_block_one(const uint4 (&bw)[8], const float2* par, const float2& v0,
                                                 const float2 (&lc)[5], const uint32_t (&off)[5], uint32_t u, bool in,
-                                                uint32_t code, bool lead, const float* rn, float scale, void* out) {
+                                                uint32_t code, bool lead, const float* rn, float scale, void* out,
+                                                const float2& v0lo = float2{}, int64_t plo = 0, int64_t olo = 0) {
   const uint64_t CB = (uint64_t)bw[0].x | ((uint64_t)bw[0].y << 32);
   const uint64_t LBp = (uint64_t)bw[0].z | ((uint64_t)bw[0].w << 32);
   const uint32_t w6 = bw[1].x;
@@ -713,12 +750,27 @@ __device__ __forceinline__ void group_block_one(const uint4 (&bw)[8], const floa
   for (int j = 0; j < 5; ++j) lv[j] = at_bytes(lp, off[j]);
   BlockAcc acc{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int s = 0; s < NS; ++s) bk_cmac<false>(acc, coef[s], sv[s]);
+  for (int s = 0; s < NS; ++s) bk_cmac<COMP>(acc, coef[s], sv[s]);
 #pragma unroll
-  for (int j = 0; j < 5; ++j) bk_cmac<false>(acc, lc[j], lv[j]);
+  for (int j = 0; j < 5; ++j) bk_cmac<COMP>(acc, lc[j], lv[j]);
+  if constexpr (COMP) {
+    if (plo) {
+      bk_lo_mac(acc, coef[0], v0lo);
+#pragma unroll
+      for (int s = 1; s < NS; ++s) {
+        const uint64_t base = (uint64_t)wd[10 + 4 * (s - 1)] | ((uint64_t)wd[11 + 4 * (s - 1)] << 32);
+        bk_lo_mac(acc, coef[s], at_bytes(par + plo + base, u * 8u));
+      }
+#pragma unroll
+      for (int j = 0; j < 5; ++j) bk_lo_mac(acc, lc[j], at_bytes(lp + plo, off[j]));
+    }
+  }
   const float re = acc.re + acc.cre, im = acc.im + acc.cim;
   if constexpr (KIND == 0) {
     if (in) static_cast<float2*>(out)[CB + u] = make_float2(re, im);
+    if constexpr (COMP) {
+      if (olo && in) static_cast<float2*>(out)[olo + CB + u] = make_float2(acc.cre - (re - acc.re), acc.cim - (im - acc.im));
+    }
   } else {
     const float f = fQ * rn[(lead ? ((w6 >> 16) & 0xffu) + 1u : 0u) * 16u + code];
     if (in) static_cast<float*>(out)[CB + u] = prob_readout(re * scale, im * scale, f);
@@ -741,7 +793,7 @@ __device__ __forceinline__ void group_dispatch(uint32_t nact, F&& f) {
   else f(std::integral_constant<int, MAXD>{});
 }
 
-template <int MAXD, int KIND>
+template <int MAXD, int KIND, bool COMP = false>
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_layer_group(
     const uint64_t* __restrict__ cg_t, const int* __restrict__ lo_t, const uint64_t* __restrict__ tuples5, int me,
     int stride, const float2* __restrict__ par_all, void* __restrict__ out_all, BlockLayer a,
@@ -851,11 +903,14 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4
     suffix_edges(cx, tp.b, lc[1], off[1]);
     const uint32_t code[2] = {tuple_code<5>(tp.a), tuple_code<5>(tp.b)};
     const bool lead[2] = {skip0 && (int)((uint32_t)tp.a & 0xffu) == lo, skip0 && (int)((uint32_t)tp.b & 0xffu) == lo};
+    float4 v0lo{};
+    if (COMP && a.par_lo) v0lo = load_pair(sp0 + a.par_lo, u * 8u);
     for (int g = 0; g < G; ++g) {
       uint4 bw[8];
       group_words(tbl, g, bw);
       group_dispatch<MAXD>(bw[1].x >> 24, [&](auto ns) {
-        group_block_pairs<decltype(ns)::value, KIND>(bw, par, v0, lc, off, u, code, lead, rn, a.scale, out);
+        group_block_pairs<decltype(ns)::value, KIND, COMP>(bw, par, v0, lc, off, u, code, lead, rn, a.scale, out, v0lo,
+                                                            COMP ? a.par_lo : 0, COMP ? a.out_lo : 0);
       });
     }
   }
@@ -870,11 +925,14 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4
     suffix_edges(cx, w, lc, off);
     const uint32_t code = tuple_code<5>(w);
     const bool lead = skip0 && (int)((uint32_t)w & 0xffu) == lo;
+    float2 v0lo{};
+    if (COMP && a.par_lo) v0lo = at_bytes(sp0 + a.par_lo, u * 8u);
     for (int g = 0; g < G; ++g) {
       uint4 bw[8];
       group_words(tbl, g, bw);
       group_dispatch<MAXD>(bw[1].x >> 24, [&](auto ns) {
-        group_block_one<decltype(ns)::value, KIND>(bw, par, v0, lc, off, u, in, code, lead, rn, a.scale, out);
+        group_block_one<decltype(ns)::value, KIND, COMP>(bw, par, v0, lc, off, u, in, code, lead, rn, a.scale, out, v0lo,
+                                                          COMP ? a.par_lo : 0, COMP ? a.out_lo : 0);
       });
     }
   }

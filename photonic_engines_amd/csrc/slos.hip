@@ -576,9 +576,10 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   const uint64_t wgs = (waves + slos::kWaves - 1) / slos::kWaves;
   if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
   if (a.B > 65535) return fail(SLOS_ERANGE, "batch > 65535");
-  // layers in group order where their work items exist (plain sums; probabilities or complex64)
+  // layers in group order where their work items exist (plain or compensated sums; probabilities or
+  // complex64)
   const bool grp_last = kind == 1;
-  if (!comp && suf == 5 && kind <= 1 && dt.items[a.K]) {
+  if (suf == 5 && kind <= 1 && dt.items[a.K]) {
     const uint4* gitems = dt.items[a.K];
     const uint32_t gn = dt.n_items[a.K];
     const size_t shg = slos::group_dyn_lds(a.K, p->me);
@@ -587,19 +588,26 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
       L.xcd_chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 20) / ((uint64_t)slos::kWaves * slos::kGroupItemChildren));
       const dim3 ggrid((unsigned)gwgs, (unsigned)a.B);
       const int maxd = a.K - suf;
-#define SLOS_GRP(D)                                                                                                    \
+#define SLOS_GRP2(D, C)                                                                                                \
   if (grp_last)                                                                                                        \
-    hipLaunchKernelGGL((slos::k_layer_group<D, 1>), ggrid, dim3(slos::kBlockThreads), shg, st, dt.cg, dt.lo, dt.tuples5, \
-                       p->me, p->me + 1, a.par, a.out, L, gitems, gn);                                                  \
+    hipLaunchKernelGGL((slos::k_layer_group<D, 1, C>), ggrid, dim3(slos::kBlockThreads), shg, st, dt.cg, dt.lo,          \
+                       dt.tuples5, p->me, p->me + 1, a.par, a.out, L, gitems, gn);                                      \
   else                                                                                                                 \
-    hipLaunchKernelGGL((slos::k_layer_group<D, 0>), ggrid, dim3(slos::kBlockThreads), shg, st, dt.cg, dt.lo, dt.tuples5, \
-                       p->me, p->me + 1, a.par, a.out, L, gitems, gn)
+    hipLaunchKernelGGL((slos::k_layer_group<D, 0, C>), ggrid, dim3(slos::kBlockThreads), shg, st, dt.cg, dt.lo,          \
+                       dt.tuples5, p->me, p->me + 1, a.par, a.out, L, gitems, gn)
+#define SLOS_GRP(D)      \
+  if (comp) {            \
+    SLOS_GRP2(D, true);  \
+  } else {               \
+    SLOS_GRP2(D, false); \
+  }
       if (maxd <= 2) { SLOS_GRP(2); }
       else if (maxd == 3) { SLOS_GRP(3); }
       else if (maxd == 4) { SLOS_GRP(4); }
       else if (maxd == 5) { SLOS_GRP(5); }
       else { SLOS_GRP(6); }
 #undef SLOS_GRP
+#undef SLOS_GRP2
       HIP_TRY(hipGetLastError());
       return SLOS_OK;
     }
