@@ -80,8 +80,9 @@ const char* slos_last_error(void);
  * (ascending occupation expansion, kernel.py:124). Replaces FockSLOSKernel.__init__
  * (kernel.py:90-167) together with get_layers / FockLayers (basis.py:92-117,275-292): no state
  * or edge table is stored; only the counting tables (a few KB). Errors (kernel.py:96-126):
- * SLOS_EINVAL for an unknown mask, n < 1, m < 1, a mode outside [0, m), or an empty output
- * space; SLOS_ERANGE if a layer size does not fit 64 bits or n > 32. */
+ * SLOS_EINVAL for an unknown mask, n < 0, m < 1, a mode outside [0, m), or an empty output
+ * space; SLOS_ERANGE if a layer size does not fit 64 bits or n > 32. n = 0 is the vacuum: one
+ * output state, which slos_forward writes as probability |scale|^2 (amplitude scale). */
 int slos_plan_create(int n, int m, int mask, const int32_t* input_modes, slos_plan** out);
 
 /* Free a plan and its device tables. Replaces clear_layers_cache (basis.py:295-301). */

@@ -253,6 +253,14 @@ __global__ void k_fill_ones(float2* p, int64_t n) {
     p[i] = make_float2(1.f, 0.f);
 }
 
+// the output of layer 0, the vacuum (one state a batch row): probability |scale|^2, amplitude scale
+__global__ void k_vacuum_out(void* out, int64_t n, int probs, float scale) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (probs) static_cast<float*>(out)[i] = scale * scale;
+    else static_cast<float2*>(out)[i] = make_float2(scale, 0.f);
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // FP64 mass: per-block partial sums, then one block per batch row
 // ------------------------------------------------------------------------------------------
@@ -1140,7 +1148,7 @@ const char* slos_last_error(void) { return g_err.c_str(); }
 int slos_plan_create(int n, int m, int mask, const int32_t* input_modes, slos_plan** out) {
   if (!out) return fail(SLOS_EINVAL, "out is NULL");
   *out = nullptr;
-  if (n < 1) return fail(SLOS_EINVAL, "n must be >= 1");
+  if (n < 0) return fail(SLOS_EINVAL, "n must be >= 0");
   if (n > kMaxPhotons) return fail(SLOS_ERANGE, "at most 32 photons");
   if (m < 1 || m > 4096) return fail(SLOS_EINVAL, "m must be in [1, 4096]");
   if (!input_modes) return fail(SLOS_EINVAL, "input_modes is NULL");
@@ -1269,7 +1277,7 @@ int slos_prob_norms(const slos_plan* p, int k, uint64_t first, int64_t count, do
 int slos_forward_workspace(const slos_plan* p, int64_t batch, int layer, unsigned flags, size_t* bytes) {
   if (!p || !bytes) return fail(SLOS_EINVAL, "NULL argument");
   if (batch < 1) return fail(SLOS_EINVAL, "batch must be >= 1");
-  if (layer < 1 || layer > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (layer < 0 || layer > p->n) return fail(SLOS_EINVAL, "layer out of range");
   WsLayout w;
   int rc = forward_layout(p, batch, layer, &w, (flags & SLOS_FLAG_COMPENSATED) != 0);
   if (rc) return rc;
@@ -1285,7 +1293,7 @@ int forward_check(const slos_plan* p, const void* U, int64_t batch, const int32_
                   void* out, unsigned flags, int* cols, WsLayout* w) {
   if (!p || !U || !out) return fail(SLOS_EINVAL, "NULL argument");
   if (batch < 1) return fail(SLOS_EINVAL, "batch must be >= 1");
-  if (layer < 1 || layer > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (layer < 0 || layer > p->n) return fail(SLOS_EINVAL, "layer out of range");
   if (out_kind != SLOS_OUT_PROBS && out_kind != SLOS_OUT_AMPS) return fail(SLOS_EINVAL, "unknown output kind");
   for (int k = 0; k < layer; ++k) {
     cols[k] = modes ? modes[k] : p->modes[k];
@@ -1304,6 +1312,12 @@ int forward_run(const slos_plan* p, const DevTables& dt, const void* U, int64_t 
   hipLaunchKernelGGL(k_fill_ones, dim3((unsigned)std::min<int64_t>((batch + 255) / 256, 4096)), dim3(256), 0, st,
                      ones, batch);
   HIP_TRY(hipGetLastError());
+  if (layer == 0) {  // no photon: the vacuum
+    hipLaunchKernelGGL(k_vacuum_out, dim3((unsigned)std::min<int64_t>((batch + 255) / 256, 4096)), dim3(256), 0, st, out,
+                       batch, out_kind == SLOS_OUT_PROBS ? 1 : 0, (float)scale);
+    HIP_TRY(hipGetLastError());
+    return SLOS_OK;
+  }
   const float2* Uf = static_cast<const float2*>(U);
   const float2* par = ones;
   uint64_t S_par = 1;

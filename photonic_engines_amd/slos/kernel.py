@@ -154,7 +154,7 @@ class FockSLOSKernel:
     def _amplitudes(self, U: torch.Tensor, input_modes: Sequence[int] | None = None) -> torch.Tensor:
         """With `input_modes` (g <= n columns) the recurrence stops on layer g of the same basis,
         raw, without the input norm (kernel.py:201-206, used by slos/species.py:94)."""
-        if U.requires_grad and torch.is_grad_enabled():
+        if U.requires_grad and torch.is_grad_enabled() and (self.n if input_modes is None else len(input_modes)) > 0:
             return self._amplitudes_grad(U, input_modes)
         Ug, batched = self._prepare(U)
         if input_modes is None:
@@ -204,7 +204,7 @@ class FockSLOSKernel:
                     raise
                 return _maybe_thin(self.layers, p, loss)
         try:
-            if U.requires_grad and torch.is_grad_enabled():
+            if U.requires_grad and torch.is_grad_enabled() and self.n > 0:  # (n = 0: no dependence on U)
                 from .autograd import SLOSProbs
                 Ug, batched = self._prepare(U)
                 p = SLOSProbs.apply(U, Ug, self)

@@ -39,7 +39,12 @@ def test_plan_errors():
     assert _create(3, 6, 9, [0, 1, 2])[0] == nat.SLOS_EINVAL
     assert b"unknown mask" in L.slos_last_error()
     assert _create(3, 6, 0, [0, 1, 7])[0] == nat.SLOS_EINVAL
-    assert _create(0, 6, 0, [])[0] == nat.SLOS_EINVAL
+    assert _create(-1, 6, 0, [])[0] == nat.SLOS_EINVAL
+    rc0, h0 = _create(0, 6, 0, [])                                             # the vacuum: one state
+    assert rc0 == nat.SLOS_OK
+    one = ctypes.c_uint64()
+    assert L.slos_layer_size(h0, 0, ctypes.byref(one)) == 0 and one.value == 1
+    assert L.slos_plan_destroy(h0) == 0
     assert _create(40, 60, 0, [0] * 40)[0] == nat.SLOS_ERANGE
     assert _create(4, 3, 1, [0, 1, 2, 2])[0] == nat.SLOS_EINVAL               # unbunched C(3,4) = 0
     assert _create(30, 200, 0, [0] * 30)[0] == nat.SLOS_ERANGE               # C(229, 30) > 2^64

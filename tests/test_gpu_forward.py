@@ -261,3 +261,19 @@ def test_impossible_size_raises_before_launch():
     with pytest.raises(torch.cuda.OutOfMemoryError, match="needs"):
         k.probs(U)
     assert torch.cuda.memory_allocated() == before
+
+
+def test_vacuum_input_and_single_mode():
+    """No photon (n = 0): one output state, probability 1, amplitude 1, batched too; and every
+    photon in one mode of a 1-mode space: probability |U|^(2n) = 1 (kernel.py:90-126 accept both)."""
+    k0 = FockSLOSKernel(0, 5, [0] * 5, device=DEV)
+    U5 = torch.stack([haar_unitary(5, s) for s in (1, 2)]).to(torch.complex64).to(DEV)
+    assert k0.output_size() == 1
+    assert torch.equal(k0.probs(U5[0]), torch.ones(1, device=DEV))
+    assert torch.equal(k0.probs(U5), torch.ones(2, 1, device=DEV))
+    assert torch.equal(k0.amplitudes(U5[0]), torch.ones(1, dtype=torch.complex64, device=DEV))
+    k1 = FockSLOSKernel(4, 1, [4], device=DEV)
+    U1 = torch.tensor([[complex(math.cos(0.3), math.sin(0.3))]], dtype=torch.complex64, device=DEV)
+    assert abs(float(k1.probs(U1)[0]) - 1.0) < 1e-6
+    Ug = U5[0].clone().requires_grad_(True)
+    assert torch.equal(k0.probs(Ug).detach(), torch.ones(1, device=DEV))
