@@ -182,6 +182,17 @@ int slos_raise(const slos_plan* plan, int k, const float* w, const float* q_km1,
 int slos_loss_ascend(const slos_plan* plan, int k, const float* g_km1, float* g_k, int64_t batch,
                      slos_stream_t stream);
 
+/* Group sums of float32 probabilities: out[b][g] = sum of probs[b][i] over the states i of group g,
+ * in FP64 and in a fixed order (no atomics), rounded to float32. Replaces the scatter_add_ of
+ * GroupedSLOSKernel.probs (kernel.py:604-620). order [count]: the state indices sorted by group
+ * (stable); chunks [n_chunks][2]: the [begin, end) positions in `order` of each chunk (one wave
+ * sums a chunk: 4096 entries balance the work), a group's chunks consecutive; first [n_groups + 1]: group g owns chunks
+ * [first[g], first[g + 1]) (an empty group sums to 0). Device pointers. workspace >= batch *
+ * n_chunks * 8 bytes. */
+int slos_group_sums(const float* probs, int64_t batch, uint64_t count, const int64_t* order, const int64_t* chunks,
+                    uint32_t n_chunks, const uint32_t* first, int n_groups, float* out, void* ws, size_t ws_bytes,
+                    slos_stream_t stream);
+
 /* FP64 total mass of float32 probabilities: out[b] = sum_s probs[b][s] (device double [batch]).
  * Two-stage deterministic reduction (no atomics). workspace >= slos_mass_workspace bytes. */
 int slos_mass_workspace(int64_t batch, uint64_t count, size_t* bytes);

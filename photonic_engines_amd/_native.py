@@ -48,6 +48,8 @@ SIGNATURES = {
     "slos_raise": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _c.c_int64, _vp]),
     "slos_mass_workspace": (_c.c_int, [_c.c_int64, _c.c_uint64, _c.POINTER(_c.c_size_t)]),
     "slos_total_mass": (_c.c_int, [_vp, _c.c_int64, _c.c_uint64, _vp, _vp, _c.c_size_t, _vp]),
+    "slos_group_sums": (_c.c_int, [_vp, _c.c_int64, _c.c_uint64, _vp, _vp, _c.c_uint32, _vp, _c.c_int, _vp, _vp,
+                                   _c.c_size_t, _vp]),
     "slos_spot_probs": (_c.c_int, [_vp, _vp, _vp, _c.c_int64, _vp, _vp]),
     "slos_sample_workspace": (_c.c_int, [_c.c_uint64, _c.POINTER(_c.c_size_t)]),
     "slos_sample": (_c.c_int, [_vp, _c.c_uint64, _vp, _c.c_int64, _vp, _vp, _c.c_size_t, _vp]),

@@ -298,6 +298,38 @@ __global__ __launch_bounds__(256) void k_mass_final(const double* part, int nb, 
 }
 
 // ------------------------------------------------------------------------------------------
+// Group sums (GroupedSLOSKernel): a keyed reduction without atomics. The states are listed in group
+// order (`order`), cut into chunks (4096 entries balance the waves); one wave sums a chunk's
+// probabilities in FP64 (lane-strided, then a fixed butterfly), and one thread per group sums its
+// chunks' partials in ascending order. Every addition has a fixed place, so the sums are
+// reproducible run to run.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_group_partial(const float* probs, uint64_t count, const int64_t* order,
+                                                       const int64_t* chunks, uint32_t n_chunks, double* part) {
+  const uint32_t c = blockIdx.x * 4u + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= n_chunks) return;
+  const int64_t b = blockIdx.y;
+  const float* pb = probs + (size_t)b * count;
+  const int64_t i0 = chunks[2 * (size_t)c], i1 = chunks[2 * (size_t)c + 1];
+  double s = 0.0;
+  for (int64_t i = i0 + lane; i < i1; i += 64) s += (double)pb[order[i]];
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) s += __shfl_xor(s, w, 64);
+  if (lane == 0) part[(size_t)b * n_chunks + c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_group_final(const double* part, uint32_t n_chunks, const uint32_t* first,
+                                                     int n_groups, float* out) {
+  const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (g >= n_groups) return;
+  const int64_t b = blockIdx.y;
+  double s = 0.0;
+  for (uint32_t c = first[g]; c < first[g + 1]; ++c) s += part[(size_t)b * n_chunks + c];
+  out[(size_t)b * n_groups + g] = (float)s;
+}
+
+// ------------------------------------------------------------------------------------------
 // FP64 Ryser permanents for spot checks (one thread per output state)
 // ------------------------------------------------------------------------------------------
 
@@ -1676,6 +1708,26 @@ int slos_total_mass(const float* probs, int64_t batch, uint64_t count, double* o
   hipLaunchKernelGGL(k_mass_partial, dim3(kMassBlocks, (unsigned)batch), dim3(256), 0, st, probs, count, part);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_mass_final, dim3((unsigned)batch), dim3(256), 0, st, part, kMassBlocks, out);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_group_sums(const float* probs, int64_t batch, uint64_t count, const int64_t* order, const int64_t* chunks,
+                    uint32_t n_chunks, const uint32_t* first, int n_groups, float* out, void* ws, size_t ws_bytes,
+                    slos_stream_t stream) {
+  if (!probs || !order || !first || !out || batch < 1 || batch > 65535 || n_groups < 1 || (n_chunks && !chunks))
+    return fail(SLOS_EINVAL, "bad argument");
+  if (n_chunks && (!ws || ws_bytes < (size_t)batch * n_chunks * sizeof(double)))
+    return fail(SLOS_ENOMEM, "workspace too small");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(ws);
+  if (n_chunks) {
+    hipLaunchKernelGGL(k_group_partial, dim3((n_chunks + 3) / 4, (unsigned)batch), dim3(256), 0, st, probs, count,
+                       order, chunks, n_chunks, part);
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_group_final, dim3((unsigned)((n_groups + 255) / 256), (unsigned)batch), dim3(256), 0, st, part,
+                     n_chunks, first, n_groups, out);
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }

@@ -252,7 +252,11 @@ class FockSLOSKernel:
 class GroupedSLOSKernel:
     """Grouped probabilities (kernel.py:584-620): the returned output is [B, n_groups] instead of
     [B, S]. The full recurrence and the fused |amp|^2 * prod t! readout run as for the wrapped
-    kernel; the [B, S] probabilities are then summed into their groups on the GPU."""
+    kernel; the [B, S] probabilities are then summed into their groups by slos_group_sums, a keyed
+    reduction without atomics (FP64 sums in a fixed order, reproducible run to run). The states
+    are sorted by group once, here, into chunks of at most 4096 entries."""
+
+    _CHUNK = 4096
 
     def __init__(self, kernel: FockSLOSKernel, groups: torch.Tensor):
         self.kernel = kernel
@@ -263,6 +267,26 @@ class GroupedSLOSKernel:
             raise ValueError("group ids must be >= 0")
         self.groups = g.to(kernel.device)
         self.n_groups = int(self.groups.max().item()) + 1 if self.groups.numel() else 0
+        self._sort = None
+
+    def _sorted(self, dev: torch.device):
+        """(order, chunks, first) on `dev`: the states in group order, the [begin, end) chunks of
+        that list, and each group's first chunk (slos_group_sums)."""
+        if self._sort is None or self._sort[0].device != dev:
+            g = self.groups.to(dev)
+            order = torch.argsort(g, stable=True)
+            counts = torch.bincount(g, minlength=self.n_groups)
+            nch = (counts + self._CHUNK - 1) // self._CHUNK
+            first = torch.zeros(self.n_groups + 1, dtype=torch.int64, device=dev)
+            first[1:] = torch.cumsum(nch, 0)
+            seg = torch.cumsum(counts, 0) - counts
+            cg = torch.repeat_interleave(torch.arange(self.n_groups, device=dev), nch)
+            j = torch.arange(cg.numel(), device=dev) - first[cg]
+            begin = seg[cg] + self._CHUNK * j
+            end = torch.minimum(begin + self._CHUNK, seg[cg] + counts[cg])
+            chunks = torch.stack([begin, end], 1).contiguous()
+            self._sort = (order.contiguous(), chunks, first.to(torch.int32).contiguous())
+        return self._sort
 
     def output_size(self) -> int:
         """Number of groups, the width of the returned distribution."""
@@ -272,10 +296,21 @@ class GroupedSLOSKernel:
         """U [m, m] or [B, m, m] -> [n_groups] or [B, n_groups] float32, normalised like probs(U)."""
         p = self.kernel.probs(U)
         batched = p.dim() == 2
-        q = p if batched else p.unsqueeze(0)
-        idx = self.groups.to(q.device)
-        out = torch.zeros((q.shape[0], self.n_groups), dtype=q.dtype, device=q.device)
-        out.scatter_add_(1, idx.unsqueeze(0).expand(q.shape[0], -1), q)
+        q = (p if batched else p.unsqueeze(0)).contiguous()
+        if q.requires_grad or q.device.type != "cuda":
+            # autograd, or a kernel on the CPU device: the differentiable torch reduction
+            idx = self.groups.to(q.device)
+            out = torch.zeros((q.shape[0], self.n_groups), dtype=q.dtype, device=q.device)
+            out = out.scatter_add(1, idx.unsqueeze(0).expand(q.shape[0], -1), q)
+            return out if batched else out.squeeze(0)
+        order, chunks, first = self._sorted(q.device)
+        B = q.shape[0]
+        n_chunks = chunks.shape[0]
+        out = torch.empty((B, self.n_groups), dtype=torch.float32, device=q.device)
+        ws = torch.empty(max(B * n_chunks, 1), dtype=torch.float64, device=q.device)
+        nat.check(nat.lib().slos_group_sums(nat.ptr(q), B, q.shape[1], nat.ptr(order), nat.ptr(chunks), n_chunks,
+                                            nat.ptr(first), self.n_groups, nat.ptr(out), nat.ptr(ws),
+                                            ws.numel() * 8, nat.stream_handle(q.device)), "slos_group_sums")
         return out if batched else out.squeeze(0)
 
 

@@ -106,3 +106,27 @@ def test_superposition_gradient_matches_finite_differences():
             gw[c] += (L(Un, wp) - L(Un, wm)) / (2 * h) * (1.0 if unit == 1.0 else 1j)
     assert np.abs(U.grad.cpu().numpy() - gU).max() < 2e-5 * max(1.0, np.abs(gU).max())
     assert np.abs(w.grad.cpu().numpy() - gw).max() < 2e-5 * max(1.0, np.abs(gw).max())
+
+
+def test_grouped_sums_large_groups_empty_groups_and_reproducible():
+    """n=8, m=24 (7.9M states) summed into 9 interleaved groups (state index mod 9), with empty
+    groups between them (ids 2 * (index mod 9)): every group of ~0.9M states over many
+    4096-entry chunks, against the FP64 bincount of the same probabilities; two calls give
+    identical sums (no atomics); under autograd the torch reduction carries the gradient."""
+    n, m = 8, 24
+    k = FockSLOSKernel(n, m, [1] * n + [0] * (m - n), device=DEV)
+    groups = 2 * (torch.arange(k.output_size(), dtype=torch.int64) % 9)
+    g = k.group_matrix(groups)
+    U = haar_unitary(m, 824).to(torch.complex64).to(DEV)
+    a = g.probs(U)
+    b = g.probs(U)
+    assert torch.equal(a, b)
+    p = k.probs(U).double().cpu().numpy()
+    ref = np.bincount(groups.numpy(), weights=p, minlength=g.output_size())
+    assert a.shape == (2 * n + 1,)
+    assert float(a[1::2].abs().max()) == 0.0
+    assert np.abs(a.double().cpu().numpy() - ref).max() < 1e-6
+    Ug = U.clone().requires_grad_(True)
+    pg = g.probs(Ug)
+    pg[0].backward()
+    assert Ug.grad is not None and float(Ug.grad.abs().sum()) > 0
