@@ -82,16 +82,26 @@ class Plan:
         u_b = int(batch) * self.m * self.m * 8
         return {"output": out_b, "workspace": ws, "unitaries": u_b, "total": out_b + ws + u_b}
 
+    # below this many bytes a forward is not checked (the query costs more than such a forward)
+    PREFLIGHT_MIN = 1 << 30
+
     @staticmethod
     def preflight(need: int, dev: torch.device, what: str, free_bytes=None) -> None:
         """Raise torch.cuda.OutOfMemoryError (the reference's exception for this condition), with
         the sizes, before anything is allocated or launched, if `need` bytes exceed the device's
-        free memory plus what torch's cache holds unused. `free_bytes()` replaces that query."""
+        free memory plus what torch's cache holds unused. `free_bytes()` replaces that query.
+        Needs below PREFLIGHT_MIN are not checked (torch's own allocation error still applies)."""
         if free_bytes is not None:
             free = int(free_bytes())
+        elif need < Plan.PREFLIGHT_MIN:
+            return
         else:
-            free = int(torch.cuda.mem_get_info(dev)[0]) + int(torch.cuda.memory_reserved(dev) -
-                                                               torch.cuda.memory_allocated(dev))
+            # torch's unused cache first (no driver call; repeated forwards find their buffers there),
+            # the device query (which can take milliseconds) only when that is not enough
+            free = int(torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev))
+            if need <= free:
+                return
+            free += int(torch.cuda.mem_get_info(dev)[0])
         if need > free:
             raise torch.cuda.OutOfMemoryError(
                 f"{what}: needs {need / 2**30:.2f} GiB more on {dev}, {free / 2**30:.2f} GiB are free")

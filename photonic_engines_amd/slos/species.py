@@ -141,73 +141,64 @@ def species_probs(kernel, Ub: torch.Tensor, groups: Sequence[Sequence[int]], sin
     return q
 
 
-def _subset_terms(kernel, Ub: torch.Tensor, cols: Sequence[int], bads: Sequence[Sequence[int]]) -> torch.Tensor:
-    """sum over the subsets of species_probs(interfering = cols minus bad, singles = bad), for
-    subsets that all leave out len(bad) photons: [B, S_n]. The subsets form the batch dimension:
-    each row's U has the interfering columns first, so one forward with modes 0..g-1 covers all of
-    them; the singles then enter by per-row raises (layer steps with the coefficient column
-    |U[j, col]|^2 of the row)."""
-    n, B, m = len(cols), Ub.shape[0], Ub.shape[-1]
-    C, nb = len(bads), len(bads[0])
+def _subset_forwards(kernel, Ub: torch.Tensor, cols: Sequence[int], goods: Sequence[Sequence[int]]) -> torch.Tensor:
+    """The interfering part of species_probs for several subsets at once: |<t| U |good>|^2 on layer
+    g = len(good) (g >= 2), normalised by 1 / prod s_i!, as [C, B, S_g]. The subsets form the batch
+    dimension: each row's U has the interfering columns first, so one forward with modes 0..g-1
+    covers all of them."""
+    B, m = Ub.shape[0], Ub.shape[-1]
+    C, g = len(goods), len(goods[0])
     dev = Ub.device
-    goods = [[cols[j] for j in range(n) if j not in bad] for bad in bads]
-    singles = [[cols[j] for j in bad] for bad in bads]
-    g = n - nb
-    grad = _needs_grad(Ub)
-    if g >= 2:
-        gi = torch.as_tensor(goods, dtype=torch.int64, device=dev)                  # [C, g]
-        Ug = Ub[:, :, gi].permute(2, 0, 1, 3)                                        # [C, B, m, g]
-        if g < m:
-            Ug = torch.cat([Ug, torch.zeros((C, B, m, m - g), dtype=torch.complex64, device=dev)], dim=3)
-        Up = Ug.reshape(C * B, m, m).contiguous()
-        if grad:
-            q = _raw_probs(kernel, Up, list(range(g)), g, 1.0)
-        else:
-            q = kernel._plan.forward(Up, modes=list(range(g)), layer=g, out_kind=nat.OUT_PROBS, scale=1.0,
-                                     flags=nat.FLAG_COMPENSATED if kernel.compensated else 0)
-        norms = []
-        for good in goods:
-            counts: dict = {}
-            for c in good:
-                counts[c] = counts.get(c, 0) + 1
-            norms.append(1.0 / math.prod(math.factorial(v) for v in counts.values()))
-        if any(v != 1.0 for v in norms):                                             # 1 / prod s_i!
-            q = q * torch.as_tensor(norms, dtype=torch.float32, device=dev).repeat_interleave(B).unsqueeze(1)
-        k = g
+    gi = torch.as_tensor([list(gd) for gd in goods], dtype=torch.int64, device=dev)  # [C, g]
+    Ug = Ub[:, :, gi].permute(2, 0, 1, 3)                                             # [C, B, m, g]
+    if g < m:
+        Ug = torch.cat([Ug, torch.zeros((C, B, m, m - g), dtype=torch.complex64, device=dev)], dim=3)
+    Up = Ug.reshape(C * B, m, m).contiguous()
+    if _needs_grad(Ub):
+        q = _raw_probs(kernel, Up, list(range(g)), g, 1.0)
     else:
-        q = torch.ones((C * B, 1), dtype=torch.float32, device=dev)
-        singles = [sg + gd for sg, gd in zip(singles, goods)]
-        k = 0
-    if singles[0] and grad:
-        Wcols = []
-        for t in range(len(singles[0])):
-            ct = torch.as_tensor([sg[t] for sg in singles], dtype=torch.int64, device=dev)  # [C]
-            Wcols.append((Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(C * B, m).to(torch.float32))
-        q = _raise_steps(kernel, q.contiguous(), k, Wcols)
-        k += len(singles[0])
-    elif singles[0]:
-        # The raises are the layer recurrence with a real coefficient column |U[j, col]|^2 on real
-        # data: run them as complex layer steps (imaginary parts stay exactly zero, the real parts are
-        # the same fused multiply-adds) so that the block kernel carries them.
-        qc = torch.complex(q, torch.zeros_like(q))
-        Uw = torch.zeros((C * B, m, m), dtype=torch.complex64, device=dev)
-        for t in range(len(singles[0])):
-            ct = torch.as_tensor([sg[t] for sg in singles], dtype=torch.int64, device=dev)  # [C]
-            Uw[:, :, 0] = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(C * B, m).to(torch.complex64)
-            qc = kernel._plan.layer_step(Uw, k + 1, 0, qc.contiguous(), nat.OUT_RAW)
-            k += 1
-        q = qc.real
-    if k != kernel.n:
-        raise ValueError(f"the partition covers {k} photons, the kernel has {kernel.n}")
-    return q.reshape(C, B, -1).sum(0)
+        q = kernel._plan.forward(Up, modes=list(range(g)), layer=g, out_kind=nat.OUT_PROBS, scale=1.0,
+                                 flags=nat.FLAG_COMPENSATED if kernel.compensated else 0)
+    norms = []
+    for good in goods:
+        counts: dict = {}
+        for c in good:
+            counts[c] = counts.get(c, 0) + 1
+        norms.append(1.0 / math.prod(math.factorial(v) for v in counts.values()))
+    if any(v != 1.0 for v in norms):                                                  # 1 / prod s_i!
+        q = q * torch.as_tensor(norms, dtype=torch.float32, device=dev).repeat_interleave(B).unsqueeze(1)
+    return q.reshape(C, B, -1)
+
+
+def _raise_rows(kernel, q: torch.Tensor, k: int, W: torch.Tensor) -> torch.Tensor:
+    """One classical raise of every row of q [R, S_k] (float32), row r by its own coefficient
+    column W[r] ([m], real, >= 0): [R, S_{k+1}]. Without a gradient it runs as a complex layer step
+    (imaginary parts stay exactly zero, the real parts are the same fused multiply-adds), so the
+    block kernel carries it; with one, through LayerStep and its adjoint."""
+    if _needs_grad(q) or _needs_grad(W):
+        return _raise_steps(kernel, q.contiguous(), k, [W])
+    R, m = q.shape[0], kernel.m
+    Uw = torch.zeros((R, m, m), dtype=torch.complex64, device=q.device)
+    Uw[:, :, 0] = W.to(torch.complex64)
+    qc = torch.complex(q, torch.zeros_like(q)).contiguous()
+    return kernel._plan.layer_step(Uw, k + 1, 0, qc, nat.OUT_RAW).real
 
 
 def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torch.Tensor:
     """The x-model distribution on the final layer ([S] or [B, S], float32): the sum over the
     interfering subsets G of the input photons of x^|G| (1 - x)^(n - |G|) species_probs(G, rest).
     `indist` a float, a Distinguishability(x=...) or a tensor x (differentiable); `order=b` keeps
-    the subsets with at most b distinguishable photons. Subsets with the same number of
-    distinguishable photons run together (chunks of at most 2 GB of float32 terms)."""
+    the subsets with at most b distinguishable photons.
+
+    The distinguishable photons D = rest enter species_probs by classical raises, which are linear
+    and commute, so every term raises D in ascending photon order and the terms share their raises:
+    A[T] collects, on layer n - |T|, everything still to be raised by the photons T. A subset's
+    interfering forward (batched over the subsets of one size) starts A[D]; then, for |T| from the
+    largest down, A[T] is raised by its smallest photon into A[T minus it], all sets of one size in
+    one batched layer step, and the contributions are added in ascending order of T. A[()] is the
+    result. The final layer then takes n raises instead of one per subset. An interfering set of
+    fewer than two photons is itself classical: its photons join T (a single photon's
+    distribution is its column |U[:, c]|^2)."""
     d = spec(indist)
     if d.x is None:
         raise NotImplementedError("strong simulation: the x-model only (random species need a partition sum)")
@@ -216,15 +207,53 @@ def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torc
     x = x.to(dtype=torch.float32, device=Ub.device)
     cols = kernel.input_modes
     n = len(cols)
+    B, m = Ub.shape[0], Ub.shape[-1]
+    dev = Ub.device
     b_max = n if order is None else max(0, min(int(order), n))
-    per = max(1, (2 << 30) // (4 * kernel.layers.size(n) * Ub.shape[0]))
-    out = None
-    for b in range(b_max + 1):
-        w = x ** (n - b) * (1 - x) ** b
-        subsets = list(combinations(range(n), b))
-        acc = None
-        for i0 in range(0, len(subsets), per):
-            term = _subset_terms(kernel, Ub, cols, subsets[i0:i0 + per])
-            acc = term if acc is None else acc + term
-        out = acc * w if out is None else out + acc * w
+    per = max(1, (2 << 30) // (4 * kernel.layers.size(n) * B))
+    A: dict = {}
+
+    def add(T, v):
+        A[T] = v if T not in A else A[T] + v
+
+    def start(L):
+        # the terms whose distinguishable set has L photons (at least two interfering): weight
+        # x^(n-L) (1-x)^L
+        w = x ** (n - L) * (1 - x) ** L
+        bads = list(combinations(range(n), L))
+        for i0 in range(0, len(bads), per):
+            chunk = bads[i0:i0 + per]
+            goods = [[cols[j] for j in range(n) if j not in bad] for bad in chunk]
+            F = _subset_forwards(kernel, Ub, cols, goods) * w
+            for r, bad in enumerate(chunk):
+                add(tuple(bad), F[r])
+
+    # fewer than two interfering photons: the whole input is classical, every photon raised from the
+    # vacuum (C(n, L) such terms for L = n and n - 1)
+    wc = None
+    for L in (n, n - 1):
+        if 0 <= L <= b_max and n - L <= 1:
+            t = math.comb(n, L) * (x ** (n - L) * (1 - x) ** L)
+            wc = t if wc is None else wc + t
+    if wc is not None and n >= 1:
+        add(tuple(range(n)), wc * torch.ones((B, 1), dtype=torch.float32, device=dev))
+    for L in range(n, 0, -1):
+        if L <= b_max and n - L >= 2:
+            start(L)
+        Ts = sorted(T for T in A if len(T) == L)
+        if not Ts:
+            continue
+        k = n - L
+        q = torch.cat([A[T] for T in Ts], 0)                                          # [len(Ts) * B, S_k]
+        ct = torch.as_tensor([cols[T[0]] for T in Ts], dtype=torch.int64, device=dev)
+        W = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(len(Ts) * B, m).to(torch.float32)
+        r = _raise_rows(kernel, q, k, W).reshape(len(Ts), B, -1)
+        for i, T in enumerate(Ts):
+            del A[T]
+            add(T[1:], r[i])
+    if n >= 2:
+        add((), _subset_forwards(kernel, Ub, cols, [list(cols)])[0] * (x ** n))
+    if () not in A:  # no photon
+        add((), torch.ones((B, 1), dtype=torch.float32, device=dev))
+    out = A[()]
     return kernel._out(out, batched)
