@@ -298,6 +298,18 @@ def bench_next_rows(kern, U: torch.Tensor) -> dict:
     kg = k2.group_matrix(groups)
     dt_g, _ = clock(lambda: kg.probs(U2))
     out["grouped_n8_m24"] = {"groups": kg.output_size(), "ms": dt_g * 1e3}
+    # dtype=torch.complex128: every layer in complex128, float64 probabilities (FP64 kernels)
+    k3 = FockSLOSKernel(n2, m2, [1] * n2 + [0] * (m2 - n2), device="cuda", dtype=torch.complex128)
+    U3 = haar_unitary(m2, 8024).to(torch.complex128).cuda()
+    dt_z, pz = clock(lambda: k3.probs(U3))
+    out["complex128_n8_m24"] = {"ms": dt_z * 1e3, "probs_per_s": S2 / dt_z, "mass_minus_1": float(pz.sum()) - 1.0}
+    del pz
+    kz = FockSLOSKernel(n, kern.m, [1] * n + [0] * (kern.m - n), device="cuda", dtype=torch.complex128)
+    Uz = haar_unitary(kern.m, 1132).to(torch.complex128).cuda()    # unitary to FP64 rounding
+    dt_zz, pz = clock(lambda: kz.probs(Uz), reps=1)
+    out["complex128_n11_m32"] = {"ms": dt_zz * 1e3, "probs_per_s": S / dt_zz, "mass_minus_1": float(pz.sum()) - 1.0}
+    del pz
+    torch.cuda.empty_cache()
     be = StrongBackend(device="cuda")
     be.set_circuit(U.cpu().numpy())
     be.set_input_state([1] * n + [0] * (kern.m - n))

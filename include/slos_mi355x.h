@@ -69,6 +69,11 @@ enum {
                                     high parts then the low-order parts; the low parts enter every
                                     child sum and the stored layer keeps its rounding error
                                     (slos_forward with SLOS_FLAG_COMPENSATED does this internally) */
+#define SLOS_FLAG_F64 16u        /* slos_forward / slos_forward_noalloc / slos_forward_workspace /
+                                    slos_layer_step in complex128: U, the layers and the amplitudes
+                                    complex128, the probabilities float64, every sum in FP64 (the
+                                    reference's dtype=torch.complex128, kernel.py:92,117-118).
+                                    Thread-per-state kernels; not with SLOS_FLAG_COMPENSATED */
 
 /* Version string of the library. */
 const char* slos_version(void);
@@ -258,6 +263,29 @@ int slos_group_config(const slos_group* group, int K, int* block, uint64_t* per_
 int slos_group_capture(slos_group* group, const void* const* U, const int64_t* batch, float* const* out,
                        const double* scale, unsigned flags, void* workspace, size_t workspace_bytes);
 int slos_group_replay(slos_group* group, slos_stream_t stream);
+
+/* complex128 / float64 twins (FockSLOSKernel(dtype=torch.complex128), kernel.py:92,117-118) of
+ * the entry points with typed float pointers or no flags argument; same arguments and results
+ * as their complex64 forms, in FP64, thread-per-state:
+ *   slos_loss_descend_f64   slos_loss_descend (imperfections.py:254-276) on float64 q
+ *   slos_loss_ascend_f64    slos_loss_ascend on float64 gradients
+ *   slos_readout_adjoint_f64  slos_readout_adjoint: complex128 amps / grad_amps, float64 grad_p
+ *   slos_adjoint_workspace_f64, slos_adjoint_layer_f64  slos_adjoint_workspace / slos_adjoint_layer
+ *     (kernel.py:63-80): complex128 U, psi_prev, grad_k, grad_prev and grad_U
+ *   slos_group_sums_f64     slos_group_sums (kernel.py:604-620) on float64 probabilities and sums */
+int slos_loss_descend_f64(const slos_plan* plan, int k, const double* q_k, double* q_km1, int64_t batch, double scale,
+                          slos_stream_t stream);
+int slos_loss_ascend_f64(const slos_plan* plan, int k, const double* g_km1, double* g_k, int64_t batch,
+                         slos_stream_t stream);
+int slos_readout_adjoint_f64(const slos_plan* plan, int64_t batch, const void* amps, const double* grad_p,
+                             double scale, void* grad_amps, slos_stream_t stream);
+int slos_adjoint_workspace_f64(const slos_plan* plan, int64_t batch, int k, size_t* bytes);
+int slos_group_sums_f64(const double* probs, int64_t batch, uint64_t count, const int64_t* order,
+                        const int64_t* chunks, uint32_t n_chunks, const uint32_t* first, int n_groups, double* out,
+                        void* workspace, size_t workspace_bytes, slos_stream_t stream);
+int slos_adjoint_layer_f64(const slos_plan* plan, const void* U, int64_t batch, int k, int col, const void* psi_prev,
+                           const void* grad_k, void* grad_prev, void* grad_U, int accumulate, void* workspace,
+                           size_t workspace_bytes, slos_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("SLOS_MI355X_LIB") or os.path.join(_HERE, "_build", "l
 SLOS_OK, SLOS_EINVAL, SLOS_ENOMEM, SLOS_EHIP, SLOS_ERANGE, SLOS_ENODEV = range(6)
 MASK_IDS = {"fock": 0, "unbunched": 1, "dual_rail": 2, "sparse_1in4": 3, "tail_empty": 4}
 OUT_PROBS, OUT_AMPS, OUT_RAW = 0, 1, 2
-FLAG_COMPENSATED, FLAG_IMMEDIATE, FLAG_SIMPLE, FLAG_HILO = 1, 2, 4, 8
+FLAG_COMPENSATED, FLAG_IMMEDIATE, FLAG_SIMPLE, FLAG_HILO, FLAG_F64 = 1, 2, 4, 8, 16
 
 # every symbol include/slos_mi355x.h declares: (name, restype, argtypes)
 _c = ctypes
@@ -46,6 +46,14 @@ SIGNATURES = {
     "slos_loss_descend": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int64, _c.c_uint, _c.c_double, _vp]),
     "slos_loss_ascend": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int64, _vp]),
     "slos_raise": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _c.c_int64, _vp]),
+    "slos_loss_descend_f64": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int64, _c.c_double, _vp]),
+    "slos_loss_ascend_f64": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int64, _vp]),
+    "slos_readout_adjoint_f64": (_c.c_int, [_vp, _c.c_int64, _vp, _vp, _c.c_double, _vp, _vp]),
+    "slos_adjoint_workspace_f64": (_c.c_int, [_vp, _c.c_int64, _c.c_int, _c.POINTER(_c.c_size_t)]),
+    "slos_group_sums_f64": (_c.c_int, [_vp, _c.c_int64, _c.c_uint64, _vp, _vp, _c.c_uint32, _vp, _c.c_int, _vp,
+                                       _vp, _c.c_size_t, _vp]),
+    "slos_adjoint_layer_f64": (_c.c_int, [_vp, _vp, _c.c_int64, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int,
+                                          _vp, _c.c_size_t, _vp]),
     "slos_mass_workspace": (_c.c_int, [_c.c_int64, _c.c_uint64, _c.POINTER(_c.c_size_t)]),
     "slos_total_mass": (_c.c_int, [_vp, _c.c_int64, _c.c_uint64, _vp, _vp, _c.c_size_t, _vp]),
     "slos_group_sums": (_c.c_int, [_vp, _c.c_int64, _c.c_uint64, _vp, _vp, _c.c_uint32, _vp, _c.c_int, _vp, _vp,

@@ -54,7 +54,7 @@ class StrongBackend:
     def set_circuit(self, circuit) -> None:
         u = _unitary(circuit)
         self._m = u.shape[0]
-        self._U = torch.as_tensor(u).to(torch.complex64)
+        self._U = torch.as_tensor(u).to(self.dtype)
         self._invalidate()
 
     def set_input_state(self, state, species=None) -> None:
@@ -75,13 +75,14 @@ class StrongBackend:
                 blocks[sum(o)][1].append(a)
             for n, (oc, am) in blocks.items():
                 self._kernels[n] = SuperpositionSLOSKernel(n, len(oc[0]), oc, mask=self.mask, device=self.device,
-                                                          weights=am)
+                                                          dtype=self.dtype, weights=am)
         else:
             occ = [int(c) for c in state]
             if species is not None and self.indist is not None:
                 raise NotImplementedError("annotated input and indist= together: annotate, or use the x-model, not both")
             self._species = species
-            self._kernels[sum(occ)] = FockSLOSKernel(sum(occ), len(occ), occ, mask=self.mask, device=self.device)
+            self._kernels[sum(occ)] = FockSLOSKernel(sum(occ), len(occ), occ, mask=self.mask, device=self.device,
+                                                     dtype=self.dtype)
         self._invalidate()
 
     def _invalidate(self) -> None:
@@ -109,11 +110,11 @@ class StrongBackend:
                     self._probs[n] = k.probs(self._U)
         if self.loss is not None:
             from ..imperfections import thin_distribution
-            self._lossy = {n: thin_distribution(k.layers, self._probs[n].to(torch.float32), self.loss)
+            self._lossy = {n: thin_distribution(k.layers, self._probs[n], self.loss)
                            for n, k in self._kernels.items()}
 
     def _amplitudes(self, n: int) -> torch.Tensor:
-        """Raw amplitudes times the input norm of block n ([S] complex64), computed on first use."""
+        """Raw amplitudes times the input norm of block n ([S] of the backend's dtype), computed on first use."""
         if n not in self._amps:
             self._amps[n] = self._kernels[n].amplitudes(self._U)
         return self._amps[n]
@@ -186,7 +187,7 @@ class StrongBackend:
         from ..slos.sampling import sample_from_probs
         if self._lossy is None and len(self._kernels) == 1:
             (n, k), = self._kernels.items()
-            rows = sample_from_probs(k.layers, self._probs[n].to(torch.float32), int(count), generator=generator)
+            rows = sample_from_probs(k.layers, self._probs[n], int(count), generator=generator)
             return [tuple(r) for r in rows.cpu().tolist()]
         dist = self.prob_distribution()
         states = list(dist)

@@ -22,6 +22,7 @@
 #include "space.hpp"
 #include "layer_block.hpp"
 #include "descend_block.hpp"
+#include "layer_f64.hpp"
 #include "pull_index.hpp"
 
 namespace {
@@ -302,15 +303,16 @@ __global__ __launch_bounds__(256) void k_mass_final(const double* part, int nb, 
 // order (`order`), cut into chunks (4096 entries balance the waves); one wave sums a chunk's
 // probabilities in FP64 (lane-strided, then a fixed butterfly), and one thread per group sums its
 // chunks' partials in ascending order. Every addition has a fixed place, so the sums are
-// reproducible run to run.
+// reproducible run to run. T: float32 or float64 probabilities (and sums).
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_group_partial(const float* probs, uint64_t count, const int64_t* order,
+template <class T>
+__global__ __launch_bounds__(256) void k_group_partial(const T* probs, uint64_t count, const int64_t* order,
                                                        const int64_t* chunks, uint32_t n_chunks, double* part) {
   const uint32_t c = blockIdx.x * 4u + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= n_chunks) return;
   const int64_t b = blockIdx.y;
-  const float* pb = probs + (size_t)b * count;
+  const T* pb = probs + (size_t)b * count;
   const int64_t i0 = chunks[2 * (size_t)c], i1 = chunks[2 * (size_t)c + 1];
   double s = 0.0;
   for (int64_t i = i0 + lane; i < i1; i += 64) s += (double)pb[order[i]];
@@ -319,14 +321,15 @@ __global__ __launch_bounds__(256) void k_group_partial(const float* probs, uint6
   if (lane == 0) part[(size_t)b * n_chunks + c] = s;
 }
 
+template <class T>
 __global__ __launch_bounds__(256) void k_group_final(const double* part, uint32_t n_chunks, const uint32_t* first,
-                                                     int n_groups, float* out) {
+                                                     int n_groups, T* out) {
   const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (g >= n_groups) return;
   const int64_t b = blockIdx.y;
   double s = 0.0;
   for (uint32_t c = first[g]; c < first[g + 1]; ++c) s += part[(size_t)b * n_chunks + c];
-  out[(size_t)b * n_groups + g] = (float)s;
+  out[(size_t)b * n_groups + g] = (T)s;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -695,6 +698,26 @@ int launch_layer(const slos_plan* p, const DevTables& dt, int K, int col, const 
   return SLOS_OK;
 }
 
+// one complex128 layer (thread-per-state, FP64): kind 0 raw complex128, 1 float64 probabilities,
+// 2 scaled complex128 amplitudes (SLOS_FLAG_IMMEDIATE: times sqrt(prod t!))
+int launch_layer_f64(const slos_plan* p, const DevTables& dt, int K, int col, const void* U, int64_t B, const void* par,
+                     uint64_t S_par, void* out, uint64_t S, int kind, double scale, unsigned flags, hipStream_t st) {
+  slos::LayerArgsF64 a;
+  a.cg = dt.cg; a.lo = dt.lo; a.me = p->me; a.multiset = p->multiset; a.K = K; a.m = p->m; a.col = col;
+  a.U = static_cast<const double2*>(U); a.par = static_cast<const double2*>(par); a.S_par = S_par;
+  a.out = out; a.S = S; a.B = B; a.scale = scale;
+  a.immediate = (flags & SLOS_FLAG_IMMEDIATE) ? 1u : 0u;
+  const size_t sh = slos::space_lds_f64(K, p->me) + (size_t)p->m * 16;
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  const uint64_t blocks = std::min<uint64_t>((S + 255) / 256, 1u << 20);
+  dim3 grid((unsigned)std::max<uint64_t>(blocks, 1), (unsigned)std::min<int64_t>(B, 65535));
+  if (kind == 0) hipLaunchKernelGGL(slos::k_layer_f64<0>, grid, dim3(256), sh, st, a);
+  else if (kind == 1) hipLaunchKernelGGL(slos::k_layer_f64<1>, grid, dim3(256), sh, st, a);
+  else hipLaunchKernelGGL(slos::k_layer_f64<2>, grid, dim3(256), sh, st, a);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
 bool mul_overflow(uint64_t a, uint64_t b, uint64_t* r) { return __builtin_mul_overflow(a, b, r); }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -704,16 +727,17 @@ struct WsLayout {
 };
 
 // hilo: the intermediate layers are hi/lo pairs [2][B][S] (compensated sums carry their rounding
-// errors from layer to layer), twice the bytes
-int forward_layout(const slos_plan* p, int64_t B, int layer, WsLayout* w, bool hilo = false) {
+// errors from layer to layer), twice the bytes; f64: complex128 layers (16 bytes a state)
+int forward_layout(const slos_plan* p, int64_t B, int layer, WsLayout* w, bool hilo = false, bool f64 = false) {
   uint64_t maxodd = 0, maxeven = 0;
   for (int k = 1; k < layer; ++k) (k & 1 ? maxodd : maxeven) = std::max(k & 1 ? maxodd : maxeven, p->size(k));
   uint64_t a = 0, b = 0;
-  const uint64_t per = (uint64_t)B * 8 * (hilo ? 2 : 1);
+  const uint64_t elem = f64 ? 16 : 8;
+  const uint64_t per = (uint64_t)B * elem * (hilo ? 2 : 1);
   if (mul_overflow(maxodd, per, &a) || mul_overflow(maxeven, per, &b))
     return fail(SLOS_ERANGE, "workspace size overflows");
   w->ones = 0;
-  w->odd = align256((size_t)B * 8);
+  w->odd = align256((size_t)B * elem);
   w->even = w->odd + align256(a);
   w->total = w->even + align256(b);
   return SLOS_OK;
@@ -1171,6 +1195,32 @@ void build_tuples(slos_plan* p) {
   }
 }
 
+// the keyed group sums of slos_group_sums / slos_group_sums_f64 (k_group_partial, k_group_final)
+namespace {
+
+template <class T>
+int group_sums(const T* probs, int64_t batch, uint64_t count, const int64_t* order, const int64_t* chunks,
+               uint32_t n_chunks, const uint32_t* first, int n_groups, T* out, void* ws, size_t ws_bytes,
+               slos_stream_t stream) {
+  if (!probs || !order || !first || !out || batch < 1 || batch > 65535 || n_groups < 1 || (n_chunks && !chunks))
+    return fail(SLOS_EINVAL, "bad argument");
+  if (n_chunks && (!ws || ws_bytes < (size_t)batch * n_chunks * sizeof(double)))
+    return fail(SLOS_ENOMEM, "workspace too small");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(ws);
+  if (n_chunks) {
+    hipLaunchKernelGGL(k_group_partial<T>, dim3((n_chunks + 3) / 4, (unsigned)batch), dim3(256), 0, st, probs, count,
+                       order, chunks, n_chunks, part);
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_group_final<T>, dim3((unsigned)((n_groups + 255) / 256), (unsigned)batch), dim3(256), 0, st,
+                     part, n_chunks, first, n_groups, out);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* slos_version(void) { return "slos_mi355x 0.1 (gfx950)"; }
@@ -1310,8 +1360,10 @@ int slos_forward_workspace(const slos_plan* p, int64_t batch, int layer, unsigne
   if (!p || !bytes) return fail(SLOS_EINVAL, "NULL argument");
   if (batch < 1) return fail(SLOS_EINVAL, "batch must be >= 1");
   if (layer < 0 || layer > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if ((flags & SLOS_FLAG_F64) && (flags & (SLOS_FLAG_COMPENSATED | SLOS_FLAG_HILO)))
+    return fail(SLOS_EINVAL, "SLOS_FLAG_F64 computes in FP64: no compensated sums");
   WsLayout w;
-  int rc = forward_layout(p, batch, layer, &w, (flags & SLOS_FLAG_COMPENSATED) != 0);
+  int rc = forward_layout(p, batch, layer, &w, (flags & SLOS_FLAG_COMPENSATED) != 0, (flags & SLOS_FLAG_F64) != 0);
   if (rc) return rc;
   *bytes = w.total;
   return SLOS_OK;
@@ -1331,12 +1383,50 @@ int forward_check(const slos_plan* p, const void* U, int64_t batch, const int32_
     cols[k] = modes ? modes[k] : p->modes[k];
     if (cols[k] < 0 || cols[k] >= p->m) return fail(SLOS_EINVAL, "input mode out of range");
   }
-  return forward_layout(p, batch, layer, w, (flags & SLOS_FLAG_COMPENSATED) != 0);
+  if ((flags & SLOS_FLAG_F64) && (flags & (SLOS_FLAG_COMPENSATED | SLOS_FLAG_HILO)))
+    return fail(SLOS_EINVAL, "SLOS_FLAG_F64 computes in FP64: no compensated sums");
+  return forward_layout(p, batch, layer, w, (flags & SLOS_FLAG_COMPENSATED) != 0, (flags & SLOS_FLAG_F64) != 0);
+}
+
+// the complex128 forward (SLOS_FLAG_F64): the same layer walk as forward_run, FP64 kernels
+int forward_run_f64(const slos_plan* p, const DevTables& dt, const void* U, int64_t batch, const int* cols, int layer,
+                    int out_kind, void* out, double scale, unsigned flags, const WsLayout& w, void* ws,
+                    hipStream_t st) {
+  auto* base = static_cast<unsigned char*>(ws);
+  double2* ones = reinterpret_cast<double2*>(base + w.ones);
+  void* bufs[2] = {base + w.even, base + w.odd};
+  const dim3 g1((unsigned)std::min<int64_t>((batch + 255) / 256, 4096));
+  if (layer == 0) {  // no photon: the vacuum
+    hipLaunchKernelGGL(slos::k_vacuum_out_f64, g1, dim3(256), 0, st, out, batch, out_kind == SLOS_OUT_PROBS ? 1 : 0,
+                       scale);
+    HIP_TRY(hipGetLastError());
+    return SLOS_OK;
+  }
+  hipLaunchKernelGGL(slos::k_fill_ones_f64, g1, dim3(256), 0, st, ones, batch);
+  HIP_TRY(hipGetLastError());
+  const void* par = ones;
+  uint64_t S_par = 1;
+  const RoctxRange whole("slos_forward");
+  for (int k = 1; k <= layer; ++k) {
+    const uint64_t S = p->size(k);
+    const bool last = (k == layer);
+    void* dst = last ? out : bufs[k & 1];
+    const int kind = last ? (out_kind == SLOS_OUT_PROBS ? 1 : 2) : 0;
+    char name[32];
+    std::snprintf(name, sizeof name, "slos layer %d", k);
+    const RoctxRange range(name);
+    const int rc = launch_layer_f64(p, dt, k, cols[k - 1], U, batch, par, S_par, dst, S, kind, scale, flags, st);
+    if (rc) return rc;
+    par = dst;
+    S_par = S;
+  }
+  return SLOS_OK;
 }
 
 // the layer launches of a forward whose workspace `ws` follows layout `w` (no allocation)
 int forward_run(const slos_plan* p, const DevTables& dt, const void* U, int64_t batch, const int* cols, int layer,
                 int out_kind, void* out, double scale, unsigned flags, const WsLayout& w, void* ws, hipStream_t st) {
+  if (flags & SLOS_FLAG_F64) return forward_run_f64(p, dt, U, batch, cols, layer, out_kind, out, scale, flags, w, ws, st);
   int rc = SLOS_OK;
   auto* base = static_cast<unsigned char*>(ws);
   float2* ones = reinterpret_cast<float2*>(base + w.ones);
@@ -1471,6 +1561,12 @@ int slos_layer_step(const slos_plan* p, const void* U, int64_t batch, int k, int
   int rc = device_tables(p, &dt);
   if (rc) return rc;
   const int kind = out_kind == 2 ? 0 : (out_kind == SLOS_OUT_PROBS ? 1 : 2);
+  if (flags & SLOS_FLAG_F64) {
+    if (flags & (SLOS_FLAG_COMPENSATED | SLOS_FLAG_HILO))
+      return fail(SLOS_EINVAL, "SLOS_FLAG_F64 computes in FP64: no compensated sums");
+    return launch_layer_f64(p, dt, k, col, U, batch, psi_prev, p->size(k - 1), out, p->size(k), kind, scale, flags,
+                            reinterpret_cast<hipStream_t>(stream));
+  }
   const bool hilo = (flags & SLOS_FLAG_HILO) && (flags & SLOS_FLAG_COMPENSATED);
   if ((flags & SLOS_FLAG_HILO) && !(flags & SLOS_FLAG_COMPENSATED))
     return fail(SLOS_EINVAL, "SLOS_FLAG_HILO needs SLOS_FLAG_COMPENSATED");
@@ -1712,24 +1808,17 @@ int slos_total_mass(const float* probs, int64_t batch, uint64_t count, double* o
   return SLOS_OK;
 }
 
+
 int slos_group_sums(const float* probs, int64_t batch, uint64_t count, const int64_t* order, const int64_t* chunks,
                     uint32_t n_chunks, const uint32_t* first, int n_groups, float* out, void* ws, size_t ws_bytes,
                     slos_stream_t stream) {
-  if (!probs || !order || !first || !out || batch < 1 || batch > 65535 || n_groups < 1 || (n_chunks && !chunks))
-    return fail(SLOS_EINVAL, "bad argument");
-  if (n_chunks && (!ws || ws_bytes < (size_t)batch * n_chunks * sizeof(double)))
-    return fail(SLOS_ENOMEM, "workspace too small");
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  double* part = static_cast<double*>(ws);
-  if (n_chunks) {
-    hipLaunchKernelGGL(k_group_partial, dim3((n_chunks + 3) / 4, (unsigned)batch), dim3(256), 0, st, probs, count,
-                       order, chunks, n_chunks, part);
-    HIP_TRY(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_group_final, dim3((unsigned)((n_groups + 255) / 256), (unsigned)batch), dim3(256), 0, st, part,
-                     n_chunks, first, n_groups, out);
-  HIP_TRY(hipGetLastError());
-  return SLOS_OK;
+  return group_sums(probs, batch, count, order, chunks, n_chunks, first, n_groups, out, ws, ws_bytes, stream);
+}
+
+int slos_group_sums_f64(const double* probs, int64_t batch, uint64_t count, const int64_t* order,
+                        const int64_t* chunks, uint32_t n_chunks, const uint32_t* first, int n_groups, double* out,
+                        void* ws, size_t ws_bytes, slos_stream_t stream) {
+  return group_sums(probs, batch, count, order, chunks, n_chunks, first, n_groups, out, ws, ws_bytes, stream);
 }
 
 int slos_spot_probs(const slos_plan* p, const void* U, const uint64_t* idx, int64_t count, double* out,
@@ -1962,3 +2051,119 @@ int slos_group_replay(slos_group* G, slos_stream_t stream) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// complex128 / float64 twins of the entry points that take typed pointers or no flags
+// (FockSLOSKernel(dtype=torch.complex128)); thread-per-state FP64 kernels of layer_f64.hpp
+// ------------------------------------------------------------------------------------------
+
+namespace {
+
+// plan tables on the device, the batch range and the layer range shared by the twins below
+int f64_common(const slos_plan* p, int k, int64_t batch, DevTables* dt) {
+  if (k < 1 || k > p->n) return fail(SLOS_EINVAL, "layer out of range");
+  if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
+  return device_tables(p, dt);
+}
+
+// work groups and dynamic LDS of k_adjoint_layer_f64 for layer k
+int adjoint_launch_f64(const slos_plan* p, int k, size_t* lds, uint64_t* nblk) {
+  *lds = slos::space_lds_f64(k, p->me) + (size_t)p->m * 16 + (size_t)slos::kAdjThreadsF64 * p->me * 16;
+  if (*lds > lds_limit()) return fail(SLOS_ERANGE, "adjoint LDS exceeds the GPU's LDS (too many modes)");
+  *nblk = (uint64_t)adjoint_blocks(p->size(k - 1));
+  return SLOS_OK;
+}
+
+}  // namespace
+
+int slos_loss_descend_f64(const slos_plan* p, int k, const double* q_k, double* q_km1, int64_t batch, double scale,
+                          slos_stream_t stream) {
+  if (!p || !q_k || !q_km1) return fail(SLOS_EINVAL, "NULL argument");
+  DevTables dt;
+  int rc = f64_common(p, k, batch, &dt);
+  if (rc) return rc;
+  const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
+  const size_t sh = slos::space_lds_f64(k, p->me);
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  char name[32];
+  std::snprintf(name, sizeof name, "slos descent %d", k);
+  const RoctxRange range(name);
+  dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 255) / 256, 1u << 20)), (unsigned)batch);
+  hipLaunchKernelGGL(slos::k_descend_f64, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo,
+                     p->me, p->multiset, k, q_k, Sk, q_km1, Sp, batch, scale);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_loss_ascend_f64(const slos_plan* p, int k, const double* g_km1, double* g_k, int64_t batch,
+                         slos_stream_t stream) {
+  if (!p || !g_km1 || !g_k) return fail(SLOS_EINVAL, "NULL argument");
+  DevTables dt;
+  int rc = f64_common(p, k, batch, &dt);
+  if (rc) return rc;
+  const uint64_t Sp = p->size(k - 1), Sk = p->size(k);
+  const size_t sh = slos::space_lds_f64(k, p->me);
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sk + 255) / 256, 1u << 20)), (unsigned)batch);
+  hipLaunchKernelGGL(slos::k_ascend_f64, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo,
+                     p->me, p->multiset, k, g_km1, Sp, g_k, Sk, batch);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_readout_adjoint_f64(const slos_plan* p, int64_t batch, const void* amps, const double* grad_p, double scale,
+                             void* grad_amps, slos_stream_t stream) {
+  if (!p || !amps || !grad_p || !grad_amps) return fail(SLOS_EINVAL, "NULL argument");
+  if (p->n < 1) return fail(SLOS_EINVAL, "no photon: nothing depends on U");
+  DevTables dt;
+  int rc = f64_common(p, p->n, batch, &dt);
+  if (rc) return rc;
+  const uint64_t S = p->size(p->n);
+  const size_t sh = slos::space_lds_f64(p->n, p->me);
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((S + 255) / 256, 1u << 20)),
+            (unsigned)std::min<int64_t>(batch, 16));
+  hipLaunchKernelGGL(slos::k_readout_adjoint_f64, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg,
+                     dt.lo, p->me, p->multiset, p->n, static_cast<const double2*>(amps), grad_p,
+                     static_cast<double2*>(grad_amps), S, batch, scale * scale);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_adjoint_workspace_f64(const slos_plan* p, int64_t batch, int k, size_t* bytes) {
+  if (!p || !bytes) return fail(SLOS_EINVAL, "NULL argument");
+  if (k < 1 || k > p->n || batch < 1) return fail(SLOS_EINVAL, "bad argument");
+  size_t lds = 0;
+  uint64_t nblk = 0;
+  int rc = adjoint_launch_f64(p, k, &lds, &nblk);
+  if (rc) return rc;
+  *bytes = (size_t)batch * nblk * p->me * sizeof(double2);
+  return SLOS_OK;
+}
+
+int slos_adjoint_layer_f64(const slos_plan* p, const void* U, int64_t batch, int k, int col, const void* psi_prev,
+                           const void* grad_k, void* grad_prev, void* grad_U, int accumulate, void* ws,
+                           size_t ws_bytes, slos_stream_t stream) {
+  if (!p || !U || !psi_prev || !grad_k || !grad_U) return fail(SLOS_EINVAL, "NULL argument");
+  if (col < 0 || col >= p->m) return fail(SLOS_EINVAL, "input mode out of range");
+  DevTables dt;
+  int rc = f64_common(p, k, batch, &dt);
+  if (rc) return rc;
+  size_t lds = 0;
+  uint64_t nblk = 0;
+  rc = adjoint_launch_f64(p, k, &lds, &nblk);
+  if (rc) return rc;
+  if (!ws || ws_bytes < (size_t)batch * nblk * p->me * sizeof(double2)) return fail(SLOS_ENOMEM, "workspace too small");
+  const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(slos::k_adjoint_layer_f64, dim3((unsigned)nblk, (unsigned)batch), dim3(slos::kAdjThreadsF64), lds,
+                     st, dt.cg, dt.lo, p->me, p->multiset, k, p->m, col, static_cast<const double2*>(U),
+                     static_cast<const double2*>(psi_prev), static_cast<const double2*>(grad_k), p->size(k),
+                     static_cast<double2*>(grad_prev), p->size(k - 1), static_cast<double2*>(ws),
+                     grad_prev != nullptr);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(slos::k_reduce_gu_f64, dim3((unsigned)p->me, (unsigned)batch), dim3(64), 0, st,
+                     static_cast<const double2*>(ws), (int)nblk, p->me, p->m, col, static_cast<double2*>(grad_U),
+                     accumulate);
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}

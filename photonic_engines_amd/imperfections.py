@@ -201,22 +201,23 @@ class _Descend(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, q: torch.Tensor, plan, k: int) -> torch.Tensor:
-        ctx.plan, ctx.k = plan, k
+        ctx.plan, ctx.k, ctx.dtype = plan, k, q.dtype
         return plan.loss_descend(k, q)
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
-        return ctx.plan.loss_ascend(ctx.k, g.contiguous().float()), None, None
+        return ctx.plan.loss_ascend(ctx.k, g.contiguous().to(ctx.dtype)), None, None
 
 
-def carried_weights(n: int, eta: Eta, device) -> torch.Tensor | None:
-    """The float32 binomial weights of layers 0..n when the weighted blocks can be carried down
-    the layers (a loss without gradient whose weights are all positive), else None."""
+def carried_weights(n: int, eta: Eta, device, dtype=torch.float32) -> torch.Tensor | None:
+    """The binomial weights of layers 0..n in the probabilities' dtype (float32, or float64) when
+    the weighted blocks can be carried down the layers (a loss without gradient whose weights are
+    all positive), else None."""
     e = _eta(eta)
     if e is None or (isinstance(e, torch.Tensor) and e.requires_grad):
         return None
-    w32 = binomial_pmf(n, e, dtype=torch.float64, device=device).to(torch.float32)
-    return w32 if bool((w32 > 0).all()) else None
+    w = binomial_pmf(n, e, dtype=torch.float64, device=device).to(dtype)
+    return w if bool((w > 0).all()) else None
 
 
 def thin_distribution(layers, probs: torch.Tensor, eta: Eta, chunk: int | None = None,
@@ -231,10 +232,11 @@ def thin_distribution(layers, probs: torch.Tensor, eta: Eta, chunk: int | None =
     user_dev = probs.device
     batched = probs.dim() == 2
     dev = nat.require_gpu(probs.device if probs.device.type == "cuda" else None)
-    q_top = (probs if batched else probs.unsqueeze(0)).to(device=dev, dtype=torch.float32).contiguous()
+    rdt = torch.float64 if probs.dtype == torch.float64 else torch.float32   # FP64 kernels for float64 input
+    q_top = (probs if batched else probs.unsqueeze(0)).to(device=dev, dtype=rdt).contiguous()
     plan = layers._plan
     grad = torch.is_grad_enabled() and (q_top.requires_grad or (isinstance(e, torch.Tensor) and e.requires_grad))
-    w32 = None if grad else carried_weights(n, e, dev)
+    w32 = None if grad else carried_weights(n, e, dev, dtype=rdt)
     if _top_weighted and w32 is None:
         raise ValueError("_top_weighted needs a loss whose weights can be carried down the layers")
     if w32 is not None:
@@ -256,11 +258,11 @@ def thin_distribution(layers, probs: torch.Tensor, eta: Eta, chunk: int | None =
     for k in range(n, 0, -1):
         q[k - 1] = _Descend.apply(q[k], plan, k) if grad else plan.loss_descend(k, q[k])
     if e is None:
-        weights = torch.zeros(n + 1, dtype=torch.float32, device=dev)
+        weights = torch.zeros(n + 1, dtype=rdt, device=dev)
         weights[n] = 1.0
     else:
         weights = binomial_pmf(n, e.to(dev) if isinstance(e, torch.Tensor) else e, dtype=torch.float64,
-                               device=dev).to(torch.float32)
+                               device=dev).to(rdt)
     blocks = []
     for k in range(n + 1):
         b = q[k] * weights[k]

@@ -14,6 +14,21 @@ import torch
 from .. import _native as nat
 
 
+def _dtype_flags(cdtype: torch.dtype, flags: int) -> int:
+    """The forward flags for data of complex dtype `cdtype`: complex128 adds FLAG_F64 (FP64
+    kernels; compensated sums are an FP32 option and are refused with it)."""
+    if cdtype == torch.complex128:
+        if flags & (nat.FLAG_COMPENSATED | nat.FLAG_HILO):
+            raise ValueError("compensated sums apply to complex64; complex128 sums are FP64")
+        return flags | nat.FLAG_F64
+    return flags & ~nat.FLAG_F64
+
+
+def _out_dtypes(cdtype: torch.dtype) -> tuple[torch.dtype, torch.dtype]:
+    """(probability dtype, amplitude dtype) of a forward over `cdtype` data."""
+    return (torch.float64, torch.complex128) if cdtype == torch.complex128 else (torch.float32, torch.complex64)
+
+
 class Plan:
     def __init__(self, n: int, m: int, mask: str, input_modes):
         if mask not in nat.MASK_IDS:
@@ -73,13 +88,14 @@ class Plan:
 
     def memory_plan(self, batch: int, layer: int | None = None, out_kind: int = nat.OUT_PROBS, flags: int = 0) -> dict:
         """Device bytes of one forward by formula, before anything is allocated: the output
-        [B, S_layer] (4 B a probability, 8 B a complex64 amplitude), the forward's workspace (the
-        vacuum row and the two ping-pong layers, slos_forward_workspace) and the unitaries (8 m^2 B
-        each)."""
+        [B, S_layer] (4 B a probability, 8 B a complex64 amplitude; twice that with FLAG_F64), the
+        forward's workspace (the vacuum row and the two ping-pong layers, slos_forward_workspace)
+        and the unitaries (8 m^2 B each, 16 with FLAG_F64)."""
         layer = self.n if layer is None else int(layer)
-        out_b = int(batch) * self.sizes[layer] * (4 if out_kind == nat.OUT_PROBS else 8)
+        w = 2 if flags & nat.FLAG_F64 else 1
+        out_b = int(batch) * self.sizes[layer] * (4 if out_kind == nat.OUT_PROBS else 8) * w
         ws = self.workspace_bytes(batch, layer, flags)
-        u_b = int(batch) * self.m * self.m * 8
+        u_b = int(batch) * self.m * self.m * 8 * w
         return {"output": out_b, "workspace": ws, "unitaries": u_b, "total": out_b + ws + u_b}
 
     # below this many bytes a forward is not checked (the query costs more than such a forward)
@@ -108,20 +124,22 @@ class Plan:
 
     def forward(self, U: torch.Tensor, *, modes=None, layer: int | None = None, out_kind: int = nat.OUT_PROBS,
                 scale: float = 1.0, flags: int = 0, out: torch.Tensor | None = None) -> torch.Tensor:
-        """U: complex64 [B, m, m] on the compute device -> [B, S_layer] float32 / complex64."""
+        """U: complex64 [B, m, m] on the compute device -> [B, S_layer] float32 / complex64;
+        complex128 U -> float64 / complex128 (FLAG_F64, FP64 throughout)."""
         layer = self.n if layer is None else int(layer)
         dev = U.device
         B = U.shape[0]
-        if U.dtype != torch.complex64 or not U.is_contiguous() or U.shape[1:] != (self.m, self.m):
-            raise ValueError("U must be a contiguous complex64 [B, m, m] tensor")
+        if U.dtype not in (torch.complex64, torch.complex128) or not U.is_contiguous() or U.shape[1:] != (self.m, self.m):
+            raise ValueError("U must be a contiguous complex64 or complex128 [B, m, m] tensor")
+        flags = _dtype_flags(U.dtype, flags)
         S = self.sizes[layer]
         ws_bytes = self.workspace_bytes(B, layer, flags)
-        out_bytes = 0 if out is not None else B * S * (4 if out_kind == nat.OUT_PROBS else 8)
+        rdt, cdt = _out_dtypes(U.dtype)
+        out_bytes = 0 if out is not None else B * S * (rdt if out_kind == nat.OUT_PROBS else cdt).itemsize
         self.preflight(out_bytes + ws_bytes, dev, f"SLOS forward (n={self.n}, m={self.m}, batch {B}: "
                        f"output {out_bytes:,} B, workspace {ws_bytes:,} B)")
         if out is None:
-            out = torch.empty((B, S), dtype=torch.float32 if out_kind == nat.OUT_PROBS else torch.complex64,
-                              device=dev)
+            out = torch.empty((B, S), dtype=rdt if out_kind == nat.OUT_PROBS else cdt, device=dev)
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
         mp = None
         if modes is not None:
@@ -139,7 +157,11 @@ class Plan:
         their low-order parts)."""
         B = U.shape[0]
         S = self.sizes[k]
-        dtype = torch.float32 if out_kind == nat.OUT_PROBS else torch.complex64
+        flags = _dtype_flags(U.dtype, flags)
+        if psi_prev.dtype != U.dtype:
+            raise ValueError(f"psi_prev must be {U.dtype} like U")
+        rdt, cdt = _out_dtypes(U.dtype)
+        dtype = rdt if out_kind == nat.OUT_PROBS else cdt
         shape = (2, B, S) if (flags & nat.FLAG_HILO and out_kind == nat.OUT_RAW) else (B, S)
         if flags & nat.FLAG_HILO and (psi_prev.dim() != 3 or psi_prev.shape[0] != 2):
             raise ValueError("FLAG_HILO: psi_prev must be a hi/lo pair [2, B, S_{k-1}]")
@@ -154,36 +176,53 @@ class Plan:
 
     def readout_adjoint(self, amps: torch.Tensor, grad_p: torch.Tensor, scale: float) -> torch.Tensor:
         g = torch.empty_like(amps)
-        nat.check(self._lib.slos_readout_adjoint(self._h, amps.shape[0], nat.ptr(amps), nat.ptr(grad_p), float(scale),
-                                                 nat.ptr(g), nat.stream_handle(amps.device)), "slos_readout_adjoint")
+        f64 = amps.dtype == torch.complex128
+        fn = self._lib.slos_readout_adjoint_f64 if f64 else self._lib.slos_readout_adjoint
+        grad_p = grad_p.to(torch.float64 if f64 else torch.float32).contiguous()
+        nat.check(fn(self._h, amps.shape[0], nat.ptr(amps), nat.ptr(grad_p), float(scale), nat.ptr(g),
+                     nat.stream_handle(amps.device)), "slos_readout_adjoint")
         return g
 
     def adjoint_layer(self, U: torch.Tensor, k: int, col: int, psi_prev: torch.Tensor, grad_k: torch.Tensor,
                       grad_U: torch.Tensor, want_prev: bool = True) -> torch.Tensor | None:
         """grad of layer k-1 (if want_prev) and grad_U[:, :, col] += the layer's contribution."""
         B = U.shape[0]
+        f64 = U.dtype == torch.complex128
+        if psi_prev.dtype != U.dtype or grad_k.dtype != U.dtype or grad_U.dtype != U.dtype:
+            raise ValueError(f"psi_prev, grad_k and grad_U must be {U.dtype} like U")
         wsb = ctypes.c_size_t()
-        nat.check(self._lib.slos_adjoint_workspace(self._h, B, k, ctypes.byref(wsb)), "slos_adjoint_workspace")
+        wsfn = self._lib.slos_adjoint_workspace_f64 if f64 else self._lib.slos_adjoint_workspace
+        nat.check(wsfn(self._h, B, k, ctypes.byref(wsb)), "slos_adjoint_workspace")
         ws = torch.empty(max(int(wsb.value), 1), dtype=torch.uint8, device=U.device)
-        gprev = torch.empty((B, self.sizes[k - 1]), dtype=torch.complex64, device=U.device) if want_prev else None
-        nat.check(self._lib.slos_adjoint_layer(self._h, nat.ptr(U), B, k, int(col), nat.ptr(psi_prev), nat.ptr(grad_k),
-                                               nat.ptr(gprev) if want_prev else None, nat.ptr(grad_U), 1,
-                                               nat.ptr(ws), ws.numel(), nat.stream_handle(U.device)),
-                  "slos_adjoint_layer")
+        gprev = torch.empty((B, self.sizes[k - 1]), dtype=U.dtype, device=U.device) if want_prev else None
+        fn = self._lib.slos_adjoint_layer_f64 if f64 else self._lib.slos_adjoint_layer
+        nat.check(fn(self._h, nat.ptr(U), B, k, int(col), nat.ptr(psi_prev), nat.ptr(grad_k),
+                     nat.ptr(gprev) if want_prev else None, nat.ptr(grad_U), 1, nat.ptr(ws), ws.numel(),
+                     nat.stream_handle(U.device)), "slos_adjoint_layer")
         return gprev
 
     def loss_descend(self, k: int, q: torch.Tensor, flags: int = 0, scale: float = 1.0) -> torch.Tensor:
-        """q_{k-1} = scale * descent of q_k ([B, S_k] float32 contiguous on the GPU)."""
-        out = torch.empty((q.shape[0], self.sizes[k - 1]), dtype=torch.float32, device=q.device)
-        nat.check(self._lib.slos_loss_descend(self._h, k, nat.ptr(q), nat.ptr(out), q.shape[0], flags, float(scale),
-                                              nat.stream_handle(q.device)), "slos_loss_descend")
+        """q_{k-1} = scale * descent of q_k ([B, S_k] float32, or float64 in FP64, contiguous on the GPU)."""
+        out = torch.empty((q.shape[0], self.sizes[k - 1]), dtype=q.dtype, device=q.device)
+        if q.dtype == torch.float64:
+            rc = self._lib.slos_loss_descend_f64(self._h, k, nat.ptr(q), nat.ptr(out), q.shape[0], float(scale),
+                                                 nat.stream_handle(q.device))
+        elif q.dtype == torch.float32:
+            rc = self._lib.slos_loss_descend(self._h, k, nat.ptr(q), nat.ptr(out), q.shape[0], flags, float(scale),
+                                             nat.stream_handle(q.device))
+        else:
+            raise ValueError(f"q must be float32 or float64, got {q.dtype}")
+        nat.check(rc, "slos_loss_descend")
         return out
 
     def loss_ascend(self, k: int, g: torch.Tensor) -> torch.Tensor:
-        """Adjoint of loss_descend: [B, S_{k-1}] -> [B, S_k]."""
-        out = torch.empty((g.shape[0], self.sizes[k]), dtype=torch.float32, device=g.device)
-        nat.check(self._lib.slos_loss_ascend(self._h, k, nat.ptr(g), nat.ptr(out), g.shape[0],
-                                             nat.stream_handle(g.device)), "slos_loss_ascend")
+        """Adjoint of loss_descend: [B, S_{k-1}] -> [B, S_k] (float32 or float64)."""
+        out = torch.empty((g.shape[0], self.sizes[k]), dtype=g.dtype, device=g.device)
+        fn = self._lib.slos_loss_ascend_f64 if g.dtype == torch.float64 else self._lib.slos_loss_ascend
+        if g.dtype not in (torch.float32, torch.float64):
+            raise ValueError(f"g must be float32 or float64, got {g.dtype}")
+        nat.check(fn(self._h, k, nat.ptr(g), nat.ptr(out), g.shape[0], nat.stream_handle(g.device)),
+                  "slos_loss_ascend")
         return out
 
     def raise_layer(self, k: int, w: torch.Tensor, q: torch.Tensor) -> torch.Tensor:
@@ -194,7 +233,10 @@ class Plan:
         return out
 
     def total_mass(self, probs: torch.Tensor) -> torch.Tensor:
-        """FP64 sum over the last axis of float32 probabilities [B, S] -> float64 [B]."""
+        """FP64 sum over the last axis of float32 probabilities [B, S] -> float64 [B] (float64
+        probabilities: torch's FP64 sum on the same GPU)."""
+        if probs.dtype == torch.float64:
+            return probs.sum(-1)
         B, S = probs.shape
         out = torch.empty(B, dtype=torch.float64, device=probs.device)
         wsb = ctypes.c_size_t()

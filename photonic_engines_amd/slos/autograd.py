@@ -38,7 +38,7 @@ def _hi(psi: torch.Tensor) -> torch.Tensor:
 
 def _checkpointed_forward(plan, Ug: torch.Tensor, modes, layer: int, cf: int):
     """Layers 1..layer-1, keeping layer 0 and the even layers: (saved, psi_{layer-1})."""
-    psi = torch.ones((Ug.shape[0], 1), dtype=torch.complex64, device=Ug.device)
+    psi = torch.ones((Ug.shape[0], 1), dtype=Ug.dtype, device=Ug.device)
     if cf & nat.FLAG_HILO:
         psi = torch.stack([psi, torch.zeros_like(psi)])
     saved = {0: psi}
@@ -51,7 +51,7 @@ def _checkpointed_forward(plan, Ug: torch.Tensor, modes, layer: int, cf: int):
 
 def _adjoint_layers(plan, Ug: torch.Tensor, modes, layer: int, saved: dict, cf: int, g: torch.Tensor,
                     psi_top: torch.Tensor) -> torch.Tensor:
-    """dL/dU (complex64, like Ug) from g = dL/dpsi_layer, layers `layer` .. 1; psi_top = psi_{layer-1}."""
+    """dL/dU (complex64 or complex128, like Ug) from g = dL/dpsi_layer, layers `layer` .. 1; psi_top = psi_{layer-1}."""
 
     def psi_of(k: int) -> torch.Tensor:
         if k in saved:
@@ -87,7 +87,8 @@ class SLOSProbs(torch.autograd.Function):
     def backward(ctx, grad_p: torch.Tensor):
         kernel, saved, Ug, cf = ctx.kernel, ctx.saved, ctx.Ug, ctx.cf
         plan, modes, n = kernel._plan, kernel.input_modes, kernel.n
-        grad_p = grad_p.to(device=Ug.device, dtype=torch.float32).contiguous()
+        rdt = torch.float64 if Ug.dtype == torch.complex128 else torch.float32
+        grad_p = grad_p.to(device=Ug.device, dtype=rdt).contiguous()
         psi_prev = saved[n - 1] if (n - 1) in saved else plan.layer_step(Ug, n - 1, modes[n - 2], saved[n - 2],
                                                                           nat.OUT_RAW, flags=cf)
         amps = _hi(plan.layer_step(Ug, n, modes[n - 1], psi_prev, nat.OUT_RAW, flags=cf))
@@ -99,7 +100,7 @@ class SLOSProbs(torch.autograd.Function):
 
 
 class SLOSRaw(torch.autograd.Function):
-    """Raw amplitudes [B, S_layer] (complex64) of layer `layer` for input columns `modes`."""
+    """Raw amplitudes [B, S_layer] (Ug's dtype) of layer `layer` for input columns `modes`."""
 
     @staticmethod
     def forward(ctx, U_user: torch.Tensor, Ug: torch.Tensor, kernel, modes, layer: int) -> torch.Tensor:
@@ -116,7 +117,7 @@ class SLOSRaw(torch.autograd.Function):
     def backward(ctx, grad_out: torch.Tensor):
         kernel, saved, Ug, cf, modes, layer = ctx.kernel, ctx.saved, ctx.Ug, ctx.cf, ctx.modes, ctx.layer
         plan = kernel._plan
-        g = grad_out.to(device=Ug.device, dtype=torch.complex64).contiguous()
+        g = grad_out.to(device=Ug.device, dtype=Ug.dtype).contiguous()
         psi_prev = saved[layer - 1] if (layer - 1) in saved else plan.layer_step(
             Ug, layer - 1, modes[layer - 2], saved[layer - 2], nat.OUT_RAW, flags=cf)
         gU = _adjoint_layers(plan, Ug, modes, layer, saved, cf, g, psi_prev)
@@ -126,7 +127,7 @@ class SLOSRaw(torch.autograd.Function):
 
 class LayerStep(torch.autograd.Function):
     """psi_k = layer k of the recurrence from psi_{k-1} with coefficient column W[:, :, col]
-    (complex64 [B, m, m], differentiable) — one layer with its adjoint; psi_{k-1} is kept."""
+    (complex64 or complex128 [B, m, m], differentiable) — one layer with its adjoint; psi_{k-1} is kept."""
 
     @staticmethod
     def forward(ctx, W: torch.Tensor, psi_prev: torch.Tensor, plan, k: int, col: int) -> torch.Tensor:
@@ -140,6 +141,6 @@ class LayerStep(torch.autograd.Function):
     def backward(ctx, g: torch.Tensor):
         W, psi_prev = ctx.saved_tensors
         gW = torch.zeros_like(W)
-        gprev = ctx.plan.adjoint_layer(W, ctx.k, ctx.col, psi_prev, g.to(torch.complex64).contiguous(), gW,
+        gprev = ctx.plan.adjoint_layer(W, ctx.k, ctx.col, psi_prev, g.to(W.dtype).contiguous(), gW,
                                        want_prev=True)
         return gW, gprev, None, None, None

@@ -13,9 +13,10 @@ Device model: the engine always computes on an AMD GPU. `device` is where U is t
 where results are returned; a CPU `device` (the reference's default) therefore means host-resident
 inputs and outputs around the same GPU computation. Without a visible GPU the engine raises.
 
-Precision: complex64 end to end; `compensated=True` (keyword) accumulates every child sum with
-fma-split products and two-sum (a compensated dot product), as an accuracy option. complex128 is
-not computed by this engine (ValueError), see the design notes.
+Precision: complex64 end to end (float32 probabilities); `compensated=True` (keyword) accumulates
+every child sum with fma-split products and two-sum (a compensated dot product), as an accuracy
+option. `dtype=torch.complex128` computes every layer in complex128 and the probabilities in
+float64 (kernel.py:117-118), on the thread-per-state FP64 kernels.
 """
 from __future__ import annotations
 
@@ -43,7 +44,7 @@ def _maybe_thin(layers, p: torch.Tensor, loss):
 class FockSLOSKernel:
     """Precompiled SLOS evaluator for a fixed (n, m, mask, input); `probs(U)` is the timed op.
 
-    U is [m, m] or [B, m, m]; probabilities come back as [S] or [B, S] float32 over the final
+    U is [m, m] or [B, m, m]; probabilities come back as [S] or [B, S] rdtype (float32; float64 for complex128) over the final
     (masked) basis in layers.final_states order.
     """
 
@@ -63,16 +64,17 @@ class FockSLOSKernel:
             raise ValueError(f"normalization must be 'deferred' or 'immediate', got {normalization!r}")
         if gradient_checkpointing and normalization != "deferred":
             raise ValueError("gradient_checkpointing requires deferred normalization")
-        if dtype not in (torch.complex64,):
-            raise ValueError(f"dtype {dtype}: the MI355X engine computes in complex64 "
-                             "(optionally compensated); complex128 is not supported")
+        if dtype not in (torch.complex64, torch.complex128):
+            raise ValueError(f"dtype must be torch.complex64 or torch.complex128, got {dtype}")
+        if compensated and dtype == torch.complex128:
+            raise ValueError("compensated=True applies to complex64; complex128 sums are FP64")
         self.gradient_checkpointing = gradient_checkpointing
         self.normalization = normalization
         self.compensated = bool(compensated)
         self.n, self.m, self.mask = int(n), int(m), Mask(mask).value
         self.device = torch.device(device)
         self.cdtype = dtype
-        self.rdtype = torch.float32
+        self.rdtype = torch.float32 if dtype == torch.complex64 else torch.float64
         self.layers: RankedLayers = layers if layers is not None else get_layers(n, m, self.mask, device=str(self.device))
         if self.layers.output_size() == 0:
             raise ValueError(f"empty output space for n={n} m={m} mask={mask}")
@@ -92,7 +94,7 @@ class FockSLOSKernel:
     # ---- reference attributes ------------------------------------------------------------------
     @property
     def output_prob_norm(self) -> torch.Tensor:
-        """prod_j t_j! per final state as an [S] float32 tensor on `device` (kernel.py:139),
+        """prod_j t_j! per final state as an [S] rdtype tensor on `device` (kernel.py:139),
         built on first access; the engine itself never reads it (the norm is fused)."""
         if self._output_prob_norm is None:
             self._output_prob_norm = torch.as_tensor(self.layers.output_prob_norms()).to(
@@ -107,7 +109,8 @@ class FockSLOSKernel:
         """Device bytes a probs(U) of `batch` unitaries needs (output, workspace, unitaries, total),
         by formula; probs raises torch.cuda.OutOfMemoryError before launching when they exceed
         the free device memory."""
-        return self._plan.memory_plan(batch, flags=self._flags())
+        from ._plan import _dtype_flags
+        return self._plan.memory_plan(batch, flags=_dtype_flags(self.cdtype, self._flags()))
 
     def _flags(self) -> int:
         f = nat.FLAG_COMPENSATED if self.compensated else 0
@@ -128,14 +131,14 @@ class FockSLOSKernel:
                 e.args = (*e.args, hint)
 
     def _prepare(self, U: torch.Tensor):
-        """U [m,m] | [B,m,m] (any device) -> (contiguous complex64 [B,m,m] on the GPU, batched)."""
+        """U [m,m] | [B,m,m] (any device) -> (contiguous [B,m,m] of the kernel's dtype on the GPU, batched)."""
         batched = U.dim() == 3
         if not batched:
             U = U.unsqueeze(0)
         if U.shape[-2:] != (self.m, self.m):
             raise ValueError(f"U must be [m, m] or [B, m, m] with m={self.m}, got {tuple(U.shape)}")
         dev = nat.require_gpu(U.device if U.device.type == "cuda" else self.device)
-        return U.to(device=dev, dtype=torch.complex64).contiguous(), batched
+        return U.to(device=dev, dtype=self.cdtype).contiguous(), batched
 
     def _out(self, t: torch.Tensor, batched: bool) -> torch.Tensor:
         t = t if batched else t.squeeze(0)
@@ -143,7 +146,7 @@ class FockSLOSKernel:
 
     # ---- amplitudes ------------------------------------------------------------------------------
     def amplitudes(self, U: torch.Tensor) -> torch.Tensor:
-        """Output amplitudes over the final basis, [S] / [B, S] complex64 (kernel.py:191-199):
+        """Output amplitudes over the final basis, [S] / [B, S] of the kernel's dtype (kernel.py:191-199):
         the raw (deferred-normalisation) amplitude times the input norm."""
         try:
             return self._amplitudes(U)
@@ -162,7 +165,7 @@ class FockSLOSKernel:
         else:
             modes = [int(i) for i in input_modes]
             if not modes:
-                out = torch.ones((Ug.shape[0], 1), dtype=torch.complex64, device=Ug.device)
+                out = torch.ones((Ug.shape[0], 1), dtype=self.cdtype, device=Ug.device)
             else:
                 out = self._plan.forward(Ug, modes=modes, layer=len(modes), out_kind=nat.OUT_AMPS,
                                          scale=1.0, flags=self._flags() & ~nat.FLAG_IMMEDIATE)
@@ -176,7 +179,7 @@ class FockSLOSKernel:
         Ug, batched = self._prepare(U)
         modes = self.input_modes if input_modes is None else [int(i) for i in input_modes]
         if not modes:
-            out = torch.ones((Ug.shape[0], 1), dtype=torch.complex64, device=Ug.device)
+            out = torch.ones((Ug.shape[0], 1), dtype=self.cdtype, device=Ug.device)
             return self._out(out, batched)
         raw = SLOSRaw.apply(U, Ug, self, modes, len(modes))
         if input_modes is None:
@@ -216,7 +219,7 @@ class FockSLOSKernel:
                 w32 = None
                 if loss is not None and not (isinstance(loss, torch.Tensor) and loss.requires_grad):
                     from ..imperfections import carried_weights
-                    w32 = carried_weights(self.n, loss, Ug.device)
+                    w32 = carried_weights(self.n, loss, Ug.device, dtype=self.rdtype)
                 scale = self._input_norm * (math.sqrt(float(w32[self.n])) if w32 is not None else 1.0)
                 p = self._out(self._plan.forward(Ug, out_kind=nat.OUT_PROBS, scale=scale, flags=self._flags()),
                               batched)
@@ -232,7 +235,8 @@ class FockSLOSKernel:
         """FP64 sum of a probability vector [S] / [B, S] on the GPU (deterministic, no atomics)."""
         q = p if p.dim() == 2 else p.unsqueeze(0)
         dev = nat.require_gpu(q.device if q.device.type == "cuda" else self.device)
-        out = self._plan.total_mass(q.to(device=dev, dtype=torch.float32).contiguous())
+        rdt = torch.float64 if q.dtype == torch.float64 else torch.float32
+        out = self._plan.total_mass(q.to(device=dev, dtype=rdt).contiguous())
         return out if p.dim() == 2 else out.squeeze(0)
 
     def spot_probs(self, U: torch.Tensor, indices) -> torch.Tensor:
@@ -293,7 +297,7 @@ class GroupedSLOSKernel:
         return self.n_groups
 
     def probs(self, U: torch.Tensor) -> torch.Tensor:
-        """U [m, m] or [B, m, m] -> [n_groups] or [B, n_groups] float32, normalised like probs(U)."""
+        """U [m, m] or [B, m, m] -> [n_groups] or [B, n_groups] (the kernel's rdtype), normalised like probs(U)."""
         p = self.kernel.probs(U)
         batched = p.dim() == 2
         q = (p if batched else p.unsqueeze(0)).contiguous()
@@ -306,11 +310,12 @@ class GroupedSLOSKernel:
         order, chunks, first = self._sorted(q.device)
         B = q.shape[0]
         n_chunks = chunks.shape[0]
-        out = torch.empty((B, self.n_groups), dtype=torch.float32, device=q.device)
+        out = torch.empty((B, self.n_groups), dtype=q.dtype, device=q.device)
         ws = torch.empty(max(B * n_chunks, 1), dtype=torch.float64, device=q.device)
-        nat.check(nat.lib().slos_group_sums(nat.ptr(q), B, q.shape[1], nat.ptr(order), nat.ptr(chunks), n_chunks,
-                                            nat.ptr(first), self.n_groups, nat.ptr(out), nat.ptr(ws),
-                                            ws.numel() * 8, nat.stream_handle(q.device)), "slos_group_sums")
+        fn = nat.lib().slos_group_sums_f64 if q.dtype == torch.float64 else nat.lib().slos_group_sums
+        nat.check(fn(nat.ptr(q), B, q.shape[1], nat.ptr(order), nat.ptr(chunks), n_chunks, nat.ptr(first),
+                     self.n_groups, nat.ptr(out), nat.ptr(ws), ws.numel() * 8, nat.stream_handle(q.device)),
+                  "slos_group_sums")
         return out if batched else out.squeeze(0)
 
 
@@ -323,11 +328,12 @@ class SuperpositionSLOSKernel:
 
     def __init__(self, n: int, m: int, input_occupations, mask: str = Mask.FOCK.value, device: str = "cpu",
                  dtype: torch.dtype = torch.complex64, weights=None):
-        if dtype not in (torch.complex64,):
-            raise ValueError(f"dtype {dtype}: the MI355X engine computes in complex64")
+        if dtype not in (torch.complex64, torch.complex128):
+            raise ValueError(f"dtype must be torch.complex64 or torch.complex128, got {dtype}")
         self.n, self.m, self.mask = int(n), int(m), Mask(mask).value
         self.device = torch.device(device)
-        self.cdtype, self.rdtype = dtype, torch.float32
+        self.cdtype = dtype
+        self.rdtype = torch.float32 if dtype == torch.complex64 else torch.float64
         self.layers: RankedLayers = get_layers(n, m, self.mask, device=str(self.device))
         if self.layers.output_size() == 0:
             raise ValueError(f"empty output space for n={n} m={m} mask={mask}")
@@ -359,7 +365,7 @@ class SuperpositionSLOSKernel:
         return 0
 
     def amplitudes(self, U: torch.Tensor, weights=None, chunk_size: int | None = None) -> torch.Tensor:
-        """Coherently summed output amplitudes [S] / [B, S] (complex64): sum_k w_k * norm_k * raw_k.
+        """Coherently summed output amplitudes [S] / [B, S] (the kernel's dtype): sum_k w_k * norm_k * raw_k.
         Default weights: the state's (from_state / weights=), else uniform 1/sqrt(K). Differentiable
         in U (the raw amplitudes through the hand-written adjoint, SLOSRaw) and in tensor weights."""
         grad_u = U.requires_grad and torch.is_grad_enabled()
@@ -368,15 +374,16 @@ class SuperpositionSLOSKernel:
         if Ub.shape[-2:] != (self.m, self.m):
             raise ValueError(f"U must be [m, m] or [B, m, m] with m={self.m}")
         dev = nat.require_gpu(U.device if U.device.type == "cuda" else self.device)
-        Ub = Ub.to(device=dev, dtype=torch.complex64).contiguous()
+        cdt = self.cdtype
+        Ub = Ub.to(device=dev, dtype=cdt).contiguous()
         if weights is None and self.default_weights is not None:
             w = self.default_weights
         elif weights is None:
-            w = torch.full((self.K,), 1.0 / math.sqrt(self.K), dtype=torch.complex64)
+            w = torch.full((self.K,), 1.0 / math.sqrt(self.K), dtype=cdt)
         elif isinstance(weights, torch.Tensor):
-            w = weights.to(torch.complex64)
+            w = weights.to(cdt)
         else:
-            w = torch.as_tensor(np.asarray(weights), dtype=torch.complex64)
+            w = torch.as_tensor(np.asarray(weights), dtype=cdt)
         w = w.to(dev)
         if w.numel() != self.K:
             raise ValueError(f"weights must have K = {self.K} entries")
@@ -387,14 +394,14 @@ class SuperpositionSLOSKernel:
         cols = torch.as_tensor(self._modes, dtype=torch.int64, device=dev)            # [K, n]
         Ug = Ub[:, :, cols].permute(2, 0, 1, 3)                                     # [K, B, m, n]
         if self.n < m:
-            Ug = torch.cat([Ug, torch.zeros((K, B, m, m - self.n), dtype=torch.complex64, device=dev)], dim=3)
+            Ug = torch.cat([Ug, torch.zeros((K, B, m, m - self.n), dtype=cdt, device=dev)], dim=3)
         Up = Ug.reshape(K * B, m, m).contiguous()
         if grad_u:
             from .autograd import SLOSRaw
             raw = SLOSRaw.apply(Up, Up.detach(), self, list(range(self.n)), self.n)
         else:
             raw = self._plan.forward(Up, modes=list(range(self.n)), layer=self.n, out_kind=nat.OUT_AMPS, scale=1.0)
-        coef = (w * torch.as_tensor(self._norms, dtype=torch.float32, device=dev)).to(torch.complex64)
+        coef = (w * torch.as_tensor(self._norms, dtype=self.rdtype, device=dev)).to(cdt)
         amp = (raw.reshape(K, B, -1) * coef.view(K, 1, 1)).sum(0)
         amp = amp if batched else amp.squeeze(0)
         return amp.to(self.device) if self.device.type != "cuda" else amp
@@ -406,6 +413,6 @@ class SuperpositionSLOSKernel:
         if not self._output_norm_is_one:
             if self._prob_norm is None or self._prob_norm.device != p.device:  # prod t_j!, kept where p lives
                 self._prob_norm = torch.as_tensor(self.layers.output_prob_norms()).to(device=p.device,
-                                                                                      dtype=torch.float32)
+                                                                                      dtype=self.rdtype)
             p = p * self._prob_norm
         return _maybe_thin(self.layers, p, loss)

@@ -75,7 +75,8 @@ def sample_lossy_device(ld, shots: int, generator: torch.Generator) -> torch.Ten
     the GPU, drawn there: the photon number k of every shot from the FP64 block masses
     (torch.multinomial on the generator), then the pattern inside layer k by draw_indices on that
     layer's float32 block (torch.multinomial up to 2^24 entries, else the device inverse CDF; no
-    concatenated or float64 copy of the layers) and the device unrank. The distribution is the
+    concatenated or float64 copy of the layers; float64 blocks, from complex128 kernels, draw by
+    their FP64 cumsum) and the device unrank. The distribution is the
     reference's flat draw over the concatenated layers (imperfections.py:213-218); the draws
     differ (the reference draws on the host, from a CPU generator)."""
     if any(q.dim() != 1 for q in ld.probs):
@@ -83,7 +84,8 @@ def sample_lossy_device(ld, shots: int, generator: torch.Generator) -> torch.Ten
     dev = generator.device
     plan = ld.layers._plan
     m = ld.layers.m
-    blocks = [q.detach().to(device=dev, dtype=torch.float32).contiguous() for q in ld.probs]
+    blocks = [q.detach().to(device=dev, dtype=torch.float64 if q.dtype == torch.float64 else torch.float32).contiguous()
+              for q in ld.probs]
     for k, q in enumerate(blocks):
         if q.numel() != ld.layers.size(k):
             raise ValueError(f"block {k} has {q.numel()} entries, layer {k} has {ld.layers.size(k)}")

@@ -41,17 +41,22 @@ def _needs_grad(U: torch.Tensor) -> bool:
     return U.requires_grad and torch.is_grad_enabled()
 
 
+def _complex_of(q: torch.Tensor) -> torch.dtype:
+    """The complex dtype of the layer steps over real probabilities q (float64: complex128)."""
+    return torch.complex128 if q.dtype == torch.float64 else torch.complex64
+
+
 def _batched_u(kernel, U: torch.Tensor):
-    """U [m,m] | [B,m,m] -> (contiguous complex64 [B,m,m] on the compute GPU, was batched); the
+    """U [m,m] | [B,m,m] -> (contiguous [B,m,m] of the kernel's dtype on the compute GPU, was batched); the
     result stays attached to U's graph when U requires a gradient."""
     return kernel._prepare(U if _needs_grad(U) else U.detach())
 
 
 def _layer_norm(kernel, g: int, device) -> torch.Tensor:
-    """prod_j t_j! of the states of layer g as float32 [S_g] (1 when no state can bunch)."""
+    """prod_j t_j! of the states of layer g as rdtype [S_g] (1 when no state can bunch)."""
     if kernel._output_norm_is_one:
-        return torch.ones(1, dtype=torch.float32, device=device)
-    return torch.as_tensor(kernel.layers.prob_norms(g), dtype=torch.float32, device=device)
+        return torch.ones(1, dtype=kernel.rdtype, device=device)
+    return torch.as_tensor(kernel.layers.prob_norms(g), dtype=kernel.rdtype, device=device)
 
 
 def _raw_probs(kernel, Ub: torch.Tensor, modes: Sequence[int], g: int, scale: float) -> torch.Tensor:
@@ -62,28 +67,28 @@ def _raw_probs(kernel, Ub: torch.Tensor, modes: Sequence[int], g: int, scale: fl
 
 
 def _raise_steps(kernel, q: torch.Tensor, k: int, Wcols: Sequence[torch.Tensor]) -> torch.Tensor:
-    """Raises of q [B', S_k] (float32) by the coefficient columns Wcols[t] ([B', m], real, >= 0)
+    """Raises of q [B', S_k] (float32 or float64) by the coefficient columns Wcols[t] ([B', m], real, >= 0)
     as complex layer steps with their adjoint (LayerStep): [B', S_{k + len(Wcols)}]."""
     from .autograd import LayerStep
     Bp, m = q.shape[0], kernel.m
+    cdt = _complex_of(q)
     qc = torch.complex(q, torch.zeros_like(q))
     for w in Wcols:
-        W = torch.zeros((Bp, m, m), dtype=torch.complex64, device=q.device)
-        W = W.index_copy(2, torch.zeros(1, dtype=torch.int64, device=q.device),
-                         w.to(torch.complex64).unsqueeze(2))
+        W = torch.zeros((Bp, m, m), dtype=cdt, device=q.device)
+        W = W.index_copy(2, torch.zeros(1, dtype=torch.int64, device=q.device), w.to(cdt).unsqueeze(2))
         qc = LayerStep.apply(W, qc, kernel._plan, k + 1, 0)
         k += 1
     return qc.real
 
 
 def layer_probs(kernel, Ub: torch.Tensor, cols: Sequence[int]) -> torch.Tensor:
-    """|<t| U |cols>|^2 on layer g = len(cols) of the kernel's basis, [B, S_g] float32: one forward
+    """|<t| U |cols>|^2 on layer g = len(cols) of the kernel's basis, [B, S_g] rdtype: one forward
     over the columns `cols` with the input norm 1/sqrt(prod s_i!) and the layer's prod t_j!.
     g = 0 is the vacuum (probability 1)."""
     g = len(cols)
     B = Ub.shape[0]
     if g == 0:
-        return torch.ones((B, 1), dtype=torch.float32, device=Ub.device)
+        return torch.ones((B, 1), dtype=kernel.rdtype, device=Ub.device)
     counts: dict = {}
     for c in cols:
         counts[c] = counts.get(c, 0) + 1
@@ -98,9 +103,11 @@ def layer_probs(kernel, Ub: torch.Tensor, cols: Sequence[int]) -> torch.Tensor:
 def classical_raise(kernel, q: torch.Tensor, k: int, col: int, Ub: torch.Tensor) -> torch.Tensor:
     """One photon that interferes with nothing enters column `col`: the layer-k distribution
     q [B, S_k] becomes the layer-(k+1) one, q'(t + e_j) = sum q(t) |U[j, col]|^2."""
-    w = (Ub[:, :, int(col)].abs() ** 2).to(torch.float32).contiguous()
+    w = (Ub[:, :, int(col)].abs() ** 2).to(q.dtype).contiguous()
     if _needs_grad(Ub) or q.requires_grad:
         return _raise_steps(kernel, q.contiguous(), k, [w])
+    if q.dtype == torch.float64:  # FP64: the raise as a complex128 layer step
+        return _raise_rows(kernel, q.contiguous(), k, w)
     return kernel._plan.raise_layer(k + 1, w, q.contiguous())
 
 
@@ -152,7 +159,7 @@ def _subset_forwards(kernel, Ub: torch.Tensor, cols: Sequence[int], goods: Seque
     gi = torch.as_tensor([list(gd) for gd in goods], dtype=torch.int64, device=dev)  # [C, g]
     Ug = Ub[:, :, gi].permute(2, 0, 1, 3)                                             # [C, B, m, g]
     if g < m:
-        Ug = torch.cat([Ug, torch.zeros((C, B, m, m - g), dtype=torch.complex64, device=dev)], dim=3)
+        Ug = torch.cat([Ug, torch.zeros((C, B, m, m - g), dtype=Ub.dtype, device=dev)], dim=3)
     Up = Ug.reshape(C * B, m, m).contiguous()
     if _needs_grad(Ub):
         q = _raw_probs(kernel, Up, list(range(g)), g, 1.0)
@@ -166,26 +173,26 @@ def _subset_forwards(kernel, Ub: torch.Tensor, cols: Sequence[int], goods: Seque
             counts[c] = counts.get(c, 0) + 1
         norms.append(1.0 / math.prod(math.factorial(v) for v in counts.values()))
     if any(v != 1.0 for v in norms):                                                  # 1 / prod s_i!
-        q = q * torch.as_tensor(norms, dtype=torch.float32, device=dev).repeat_interleave(B).unsqueeze(1)
+        q = q * torch.as_tensor(norms, dtype=q.dtype, device=dev).repeat_interleave(B).unsqueeze(1)
     return q.reshape(C, B, -1)
 
 
 def _raise_rows(kernel, q: torch.Tensor, k: int, W: torch.Tensor) -> torch.Tensor:
-    """One classical raise of every row of q [R, S_k] (float32), row r by its own coefficient
+    """One classical raise of every row of q [R, S_k] (float32 or float64), row r by its own coefficient
     column W[r] ([m], real, >= 0): [R, S_{k+1}]. Without a gradient it runs as a complex layer step
     (imaginary parts stay exactly zero, the real parts are the same fused multiply-adds), so the
     block kernel carries it; with one, through LayerStep and its adjoint."""
     if _needs_grad(q) or _needs_grad(W):
         return _raise_steps(kernel, q.contiguous(), k, [W])
     R, m = q.shape[0], kernel.m
-    Uw = torch.zeros((R, m, m), dtype=torch.complex64, device=q.device)
-    Uw[:, :, 0] = W.to(torch.complex64)
+    Uw = torch.zeros((R, m, m), dtype=_complex_of(q), device=q.device)
+    Uw[:, :, 0] = W.to(Uw.dtype)
     qc = torch.complex(q, torch.zeros_like(q)).contiguous()
     return kernel._plan.layer_step(Uw, k + 1, 0, qc, nat.OUT_RAW).real
 
 
 def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torch.Tensor:
-    """The x-model distribution on the final layer ([S] or [B, S], float32): the sum over the
+    """The x-model distribution on the final layer ([S] or [B, S], rdtype): the sum over the
     interfering subsets G of the input photons of x^|G| (1 - x)^(n - |G|) species_probs(G, rest).
     `indist` a float, a Distinguishability(x=...) or a tensor x (differentiable); `order=b` keeps
     the subsets with at most b distinguishable photons.
@@ -203,14 +210,15 @@ def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torc
     if d.x is None:
         raise NotImplementedError("strong simulation: the x-model only (random species need a partition sum)")
     Ub, batched = _batched_u(kernel, U)
-    x = indist if isinstance(indist, torch.Tensor) else torch.tensor(d.x, dtype=torch.float32)
-    x = x.to(dtype=torch.float32, device=Ub.device)
+    rdt = kernel.rdtype
+    x = indist if isinstance(indist, torch.Tensor) else torch.tensor(d.x, dtype=rdt)
+    x = x.to(dtype=rdt, device=Ub.device)
     cols = kernel.input_modes
     n = len(cols)
     B, m = Ub.shape[0], Ub.shape[-1]
     dev = Ub.device
     b_max = n if order is None else max(0, min(int(order), n))
-    per = max(1, (2 << 30) // (4 * kernel.layers.size(n) * B))
+    per = max(1, (2 << 30) // (rdt.itemsize * kernel.layers.size(n) * B))
     A: dict = {}
 
     def add(T, v):
@@ -236,7 +244,7 @@ def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torc
             t = math.comb(n, L) * (x ** (n - L) * (1 - x) ** L)
             wc = t if wc is None else wc + t
     if wc is not None and n >= 1:
-        add(tuple(range(n)), wc * torch.ones((B, 1), dtype=torch.float32, device=dev))
+        add(tuple(range(n)), wc * torch.ones((B, 1), dtype=rdt, device=dev))
     for L in range(n, 0, -1):
         if L <= b_max and n - L >= 2:
             start(L)
@@ -246,7 +254,7 @@ def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torc
         k = n - L
         q = torch.cat([A[T] for T in Ts], 0)                                          # [len(Ts) * B, S_k]
         ct = torch.as_tensor([cols[T[0]] for T in Ts], dtype=torch.int64, device=dev)
-        W = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(len(Ts) * B, m).to(torch.float32)
+        W = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(len(Ts) * B, m).to(rdt)
         r = _raise_rows(kernel, q, k, W).reshape(len(Ts), B, -1)
         for i, T in enumerate(Ts):
             del A[T]
@@ -254,6 +262,6 @@ def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torc
     if n >= 2:
         add((), _subset_forwards(kernel, Ub, cols, [list(cols)])[0] * (x ** n))
     if () not in A:  # no photon
-        add((), torch.ones((B, 1), dtype=torch.float32, device=dev))
+        add((), torch.ones((B, 1), dtype=rdt, device=dev))
     out = A[()]
     return kernel._out(out, batched)

@@ -60,6 +60,16 @@ def test_plan_errors():
     wsc = ctypes.c_size_t()
     assert L.slos_forward_workspace(h, 2, 3, nat.FLAG_COMPENSATED, ctypes.byref(wsc)) == 0
     assert wsc.value >= 2 * 2 * 8 * (6 + 21) and wsc.value > wsb.value          # hi/lo layers
+    wsf = ctypes.c_size_t()                                                     # complex128 layers
+    assert L.slos_forward_workspace(h, 2, 3, nat.FLAG_F64, ctypes.byref(wsf)) == 0
+    assert wsf.value >= 2 * 16 * (6 + 21) and wsf.value > wsb.value
+    assert L.slos_forward_workspace(h, 2, 3, nat.FLAG_F64 | nat.FLAG_COMPENSATED, ctypes.byref(wsf)) == nat.SLOS_EINVAL
+    assert b"FP64" in L.slos_last_error()
+    assert L.slos_forward(h, None, 1, None, 3, 0, None, 1.0, nat.FLAG_F64, None, 0, None) == nat.SLOS_EINVAL
+    wa = ctypes.c_size_t()
+    assert L.slos_adjoint_workspace_f64(h, 2, 3, ctypes.byref(wa)) == 0 and wa.value >= 2 * 6 * 16
+    assert L.slos_adjoint_workspace_f64(h, 2, 4, ctypes.byref(wa)) == nat.SLOS_EINVAL
+    assert L.slos_loss_descend_f64(h, 3, None, None, 1, 1.0, None) == nat.SLOS_EINVAL
     idx = (ctypes.c_uint64 * 1)(56)
     pos = (ctypes.c_int32 * 3)()
     assert L.slos_unrank(h, 3, idx, 1, pos) == nat.SLOS_ERANGE
@@ -154,5 +164,15 @@ def test_memory_plan_and_preflight():
     Plan.preflight(mp["total"], dev, "fits", free_bytes=lambda: mp["total"])
     with pytest.raises(torch.cuda.OutOfMemoryError, match="needs"):
         Plan.preflight(mp["total"], dev, "too big", free_bytes=lambda: mp["total"] - 1)
+    mf = p.memory_plan(2, flags=nat.FLAG_F64)                                   # complex128: twice the bytes
+    assert mf["output"] == 2 * mp["output"] and mf["unitaries"] == 2 * mp["unitaries"]
+    assert mf["workspace"] == p.workspace_bytes(2, 6, nat.FLAG_F64) > mp["workspace"]
+    from photonic_engines_amd.slos import FockSLOSKernel
+    with pytest.raises(ValueError, match="complex64"):
+        FockSLOSKernel(6, 10, [1] * 6 + [0] * 4, dtype=torch.complex128, compensated=True)
+    with pytest.raises(ValueError, match="dtype"):
+        FockSLOSKernel(6, 10, [1] * 6 + [0] * 4, dtype=torch.complex32)
+    k128 = FockSLOSKernel(6, 10, [1] * 6 + [0] * 4, dtype=torch.complex128)
+    assert k128.rdtype == torch.float64 and k128.memory_plan(2)["output"] == mf["output"]
     big = Plan(12, 64, "fock", list(range(12)))
     assert big.memory_plan(1)["output"] == 4 * math.comb(75, 12)              # ~105 TB of probabilities

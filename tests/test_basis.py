@@ -94,7 +94,11 @@ def test_kernel_constructor_checks():                                           
     with pytest.raises(ValueError):
         FockSLOSKernel(3, 6, [1, 1, 0, 0, 0, 0])
     with pytest.raises(ValueError):
-        FockSLOSKernel(3, 6, occ, dtype=torch.complex128)
+        FockSLOSKernel(3, 6, occ, dtype=torch.float32)
+    with pytest.raises(ValueError):
+        FockSLOSKernel(3, 6, occ, dtype=torch.complex128, compensated=True)
+    k128 = FockSLOSKernel(3, 6, occ, dtype=torch.complex128)                    # kernel.py:117-118
+    assert (k128.cdtype, k128.rdtype) == (torch.complex128, torch.float64)
     with pytest.raises(ValueError):
         FockSLOSKernel(4, 3, [1, 1, 1], mask="unbunched")        # C(3, 4) = 0: empty output space
     k = FockSLOSKernel(3, 6, occ, normalization="immediate")
