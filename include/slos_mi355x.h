@@ -253,8 +253,9 @@ int slos_group_forward(slos_group* group, const void* const* U, const int64_t* b
 int slos_group_tune(slos_group* group, const void* const* U, const int64_t* batch, float* const* out,
                     const double* scale, unsigned flags, void* workspace, size_t workspace_bytes, int reps,
                     slos_stream_t stream);
-/* The launch chosen for layer K: block (1 block kernel, 0 thread-per-state, -1 default), per_wave,
- * grid (0: one work group per item). */
+/* The launch chosen for layer K: block (1 block kernel, 0 thread-per-state, 2 one launch per plan
+ * through the single-plan kernels, -1 default: one launch per plan where a single plan reaches
+ * layer K, else the block kernel where it fits), per_wave, grid (0: one work group per item). */
 int slos_group_config(const slos_group* group, int K, int* block, uint64_t* per_wave, uint32_t* grid);
 
 /* Capture one forward over these buffers as a hipGraph (on an internal stream; nothing is

@@ -973,7 +973,7 @@ size_t simple_lds_bytes(int K, int me, int m) {
 // a group of plans run together (uneven batches)
 struct slos_group {
   struct Cfg {
-    int block = -1;         // 1 block kernel, 0 thread-per-state, -1 default choice
+    int block = -1;         // 1 block kernel, 0 thread-per-state, 2 one single-plan launch per plan, -1 default
     uint64_t per_wave = 0;  // children per wave (block) / states per thread (simple); 0: default
     uint32_t grid = 0;      // work groups of the persistent launch; 0: one per item
   };
@@ -1068,9 +1068,33 @@ void launch_block_group(int maxd, uint32_t grid, size_t sh, hipStream_t st, cons
 #undef SLOS_BLKG
 }
 
-// layer K of every plan with n >= K, in one launch
+// layer K of every plan with n >= K as one launch per plan, through the single-plan path (the
+// 5-position-suffix and group-order kernels where they apply): the same values as the group kernel
+int group_layer_per_plan(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w,
+                         const std::vector<DevTables>& dts, hipStream_t st) {
+  const bool hilo = (a.flags & SLOS_FLAG_COMPENSATED) != 0;
+  for (size_t g = 0; g < G->plans.size(); ++g) {
+    const slos_plan* p = G->plans[g];
+    if (p->n < K) continue;
+    float2* bufs[2] = {reinterpret_cast<float2*>(a.ws + w.buf0[g]), reinterpret_cast<float2*>(a.ws + w.buf1[g])};
+    const float2* par = K == 1 ? reinterpret_cast<const float2*>(a.ws + w.ones[g]) : bufs[(K - 1) & 1];
+    const bool last = K == p->n;
+    void* out = last ? static_cast<void*>(a.out[g]) : static_cast<void*>(bufs[K & 1]);
+    const int rc = launch_layer(p, dts[g], K, p->modes[K - 1], a.U[g], a.B[g], par, p->size(K - 1), out, p->size(K),
+                                last ? 1 : 0, last ? a.scale[g] : 1.f, a.flags & ~SLOS_FLAG_HILO, st, hilo && K >= 2,
+                                hilo && !last);
+    if (rc) return rc;
+  }
+  return SLOS_OK;
+}
+
+// layer K of every plan with n >= K, in one launch (or one per plan: Cfg.block 2, and by default
+// where a single plan reaches layer K)
 int group_layer(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w, const std::vector<DevTables>& dts,
                 const slos_group::Cfg& c, hipStream_t st) {
+  int active = 0;
+  for (size_t g = 0; g < G->plans.size(); ++g) active += G->plans[g]->n >= K;
+  if (c.block == 2 || (c.block == -1 && active == 1)) return group_layer_per_plan(G, K, a, w, dts, st);
   slos::GroupLaunch gl{};
   gl.K = K;
   const bool block = group_can_block(G, dts, K, a.flags) && c.block != 0;
@@ -1972,6 +1996,7 @@ int slos_group_tune(slos_group* G, const void* const* U, const int64_t* batch, f
         for (uint32_t gr : grids) cand.push_back({1, pw, gr});
     for (uint64_t spt : {1, 2, 4})
       for (uint32_t gr : grids) cand.push_back({0, spt, gr});
+    cand.push_back({2, 0, 0});  // one single-plan launch per plan
     float best = 1e30f;
     slos_group::Cfg pick;
     for (const auto& c : cand) {

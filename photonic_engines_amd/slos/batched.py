@@ -3,10 +3,12 @@
 The reference batches one shape at a time: `FockSLOSKernel.probs` takes U as [B, m, m] for a single
 (n, m, input, mask) (photonic_engines/slos/kernel.py:208-215). `UnevenBatch` runs up to 8 such
 kernels of different (n, m, input, mask) together. Layer K of every instance of every kernel with
-n >= K is one persistent launch (slos_group_forward in include/slos_mi355x.h). The forward can be
-captured once as a hipGraph and replayed (`capture` / `replay`). A project-written autotuner
-(`tune`) picks each layer's launch: block or thread-per-state kernel, children per wave or states
-per thread, and persistent grid size, by timing the candidates on the caller's own buffers.
+n >= K is one persistent launch (slos_group_forward in include/slos_mi355x.h); a layer that only
+one kernel reaches runs through that kernel's own single-plan launch (its 5-position-suffix and
+group-order kernels). The forward can be captured once as a hipGraph and replayed (`capture` /
+`replay`). A project-written autotuner (`tune`) picks each layer's launch: block or thread-per-state
+group kernel, children per wave or states per thread, and persistent grid size, or one single-plan
+launch per kernel, by timing the candidates on the caller's own buffers.
 
 Each kernel's output is bit-for-bit the output of its own `FockSLOSKernel.probs(U)` (lossless,
 float32 probabilities over `layers.final_states`).
@@ -128,13 +130,13 @@ class UnevenBatch:
         return self.config()
 
     def config(self) -> list[dict]:
-        """The launch each layer uses: kernel ("block", "thread", "default"), per_wave, grid."""
+        """The launch each layer uses: kernel ("block", "thread", "per_plan", "default"), per_wave, grid."""
         res = []
         for K in range(1, max(k.n for k in self.kernels) + 1):
             blk, pw, gr = ctypes.c_int(), ctypes.c_uint64(), ctypes.c_uint32()
             nat.check(self._lib.slos_group_config(self._h, K, ctypes.byref(blk), ctypes.byref(pw), ctypes.byref(gr)),
                       "slos_group_config")
-            res.append({"layer": K, "kernel": {1: "block", 0: "thread", -1: "default"}[blk.value],
+            res.append({"layer": K, "kernel": {1: "block", 0: "thread", 2: "per_plan", -1: "default"}[blk.value],
                         "per_wave": int(pw.value), "grid": int(gr.value)})
         return res
 

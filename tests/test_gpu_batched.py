@@ -57,8 +57,8 @@ def test_tuned_launches_and_graph_replays():
     b = UnevenBatch.from_specs(SPECS)
     Us = [_us(12, 4, 1), _us(16, 3, 2), _us(20, 2, 3)]
     cfg = b.tune(Us, reps=1)
-    assert len(cfg) == 8 and all(c["kernel"] in ("block", "thread") for c in cfg)
-    assert all(c["kernel"] == "thread" for c in cfg[:4])  # layers <= 4 have no block form
+    assert len(cfg) == 8 and all(c["kernel"] in ("block", "thread", "per_plan") for c in cfg)
+    assert all(c["kernel"] in ("thread", "per_plan") for c in cfg[:4])  # layers <= 4 have no block form
     ref = _singles(b, Us)
     for g, r in zip(b.probs(Us), ref):
         assert torch.equal(g, r)
