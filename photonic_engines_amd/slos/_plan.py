@@ -138,8 +138,11 @@ class Plan:
         out_bytes = 0 if out is not None else B * S * (rdt if out_kind == nat.OUT_PROBS else cdt).itemsize
         self.preflight(out_bytes + ws_bytes, dev, f"SLOS forward (n={self.n}, m={self.m}, batch {B}: "
                        f"output {out_bytes:,} B, workspace {ws_bytes:,} B)")
+        odt = rdt if out_kind == nat.OUT_PROBS else cdt
         if out is None:
-            out = torch.empty((B, S), dtype=rdt if out_kind == nat.OUT_PROBS else cdt, device=dev)
+            out = torch.empty((B, S), dtype=odt, device=dev)
+        elif out.shape != (B, S) or out.dtype != odt or out.device != dev or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous {odt} tensor of shape {(B, S)} on {dev}")
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
         mp = None
         if modes is not None:

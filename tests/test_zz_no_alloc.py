@@ -26,18 +26,19 @@ def _size(h, k):
     return s.value
 
 
-def _forward(h, n, U, out_kind, dtype, noalloc):
+def _forward(h, n, U, out_kind, dtype, noalloc, flags=0):
     L = nat.lib()
     B = U.shape[0]
     out = torch.full((B, _size(h, n)), float("nan"), dtype=dtype, device=DEV)
     st = nat.stream_handle(torch.device(DEV))
     if noalloc:
-        rc = L.slos_forward_noalloc(h, nat.ptr(U), B, None, n, out_kind, nat.ptr(out), 1.0, 0, st)
+        rc = L.slos_forward_noalloc(h, nat.ptr(U), B, None, n, out_kind, nat.ptr(out), 1.0, flags, st)
     else:
         need = ctypes.c_size_t()
-        nat.check(L.slos_forward_workspace(h, B, n, 0, ctypes.byref(need)), "slos_forward_workspace")
+        nat.check(L.slos_forward_workspace(h, B, n, flags, ctypes.byref(need)), "slos_forward_workspace")
         ws = torch.empty(need.value, dtype=torch.uint8, device=DEV)
-        rc = L.slos_forward(h, nat.ptr(U), B, None, n, out_kind, nat.ptr(out), 1.0, 0, nat.ptr(ws), ws.numel(), st)
+        rc = L.slos_forward(h, nat.ptr(U), B, None, n, out_kind, nat.ptr(out), 1.0, flags, nat.ptr(ws), ws.numel(),
+                            st)
     nat.check(rc, "forward")
     torch.cuda.synchronize()
     return out
@@ -62,6 +63,25 @@ def test_noalloc_forward_equals_forward_twice_then_smaller():
         a1b = _forward(big, 8, U1, nat.OUT_PROBS, torch.float32, noalloc=True)     # the scratch keeps stale data
         assert torch.equal(a1b, ref1)
         assert abs(float(ref1.double().sum(1).max()) - 1.0) < 1e-5
+    finally:
+        L.slos_plan_destroy(big)
+        L.slos_plan_destroy(small)
+
+
+def test_noalloc_forward_complex128_equals_forward():
+    """SLOS_FLAG_F64 (complex128 layers, twice the scratch per state): equal to slos_forward with
+    the same flag bit for bit, on two calls in a row, the second smaller."""
+    L = nat.lib()
+    big, small = _plan(7, 18), _plan(5, 12)
+    try:
+        U1 = _unitaries(2, 18, 700).to(torch.complex128)
+        U2 = _unitaries(1, 12, 710).to(torch.complex128)
+        ref1 = _forward(big, 7, U1, nat.OUT_PROBS, torch.float64, noalloc=False, flags=nat.FLAG_F64)
+        ref2 = _forward(small, 5, U2, nat.OUT_AMPS, torch.complex128, noalloc=False, flags=nat.FLAG_F64)
+        a1 = _forward(big, 7, U1, nat.OUT_PROBS, torch.float64, noalloc=True, flags=nat.FLAG_F64)
+        a2 = _forward(small, 5, U2, nat.OUT_AMPS, torch.complex128, noalloc=True, flags=nat.FLAG_F64)
+        assert torch.equal(a1, ref1) and torch.equal(a2, ref2)
+        assert abs(float(ref1.sum(1).max()) - 1.0) < 1e-5
     finally:
         L.slos_plan_destroy(big)
         L.slos_plan_destroy(small)
