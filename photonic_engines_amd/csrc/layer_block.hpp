@@ -262,7 +262,8 @@ __device__ __forceinline__ void sweep(const ChunkCtx<MAXD, SUF>& cx, uint32_t uc
         for (int jj = 0; jj < SUF; ++jj) bk_lo_mac(acc, lc[c][jj], at_bytes(cx.lp + cx.plo, loff[c][jj]));
       }
     }
-    float re = acc.re + acc.cre, im = acc.im + acc.cim;
+    // (plain sums: the correction terms are zero and are not added)
+    float re = COMP ? acc.re + acc.cre : acc.re, im = COMP ? acc.im + acc.cim : acc.im;
     if (KIND == 0 || (KIND == 3 && cx.kind == 0)) {
       if (in[c]) *reinterpret_cast<float2*>(cx.ob + u[c] * 8u) = make_float2(re, im);
       if constexpr (COMP) {
@@ -337,8 +338,8 @@ __device__ __forceinline__ void sweep_pairs(const ChunkCtx<MAXD, SUF>& cx, uint3
       bk_cmac<COMP>(acc, cx.coef[s], c == 0 ? make_float2(sv[s].x, sv[s].y) : make_float2(sv[s].z, sv[s].w));
 #pragma unroll
     for (int jj = 0; jj < SUF; ++jj) bk_cmac<COMP>(acc, lc[c][jj], lv[c][jj]);
-    res[c][0] = acc.re + acc.cre;
-    res[c][1] = acc.im + acc.cim;
+    res[c][0] = COMP ? acc.re + acc.cre : acc.re;
+    res[c][1] = COMP ? acc.im + acc.cim : acc.im;
     if (!(KIND == 0 || (KIND == 3 && cx.kind == 0))) f[c] = child_norm_w(cx, tw[c]);
   }
   if (KIND == 0 || (KIND == 3 && cx.kind == 0)) {
@@ -698,8 +699,8 @@ __device__ __forceinline__ void group_block_pairs(const uint4 (&bw)[8], const fl
   }
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    res[c][0] = accs[c].re + accs[c].cre;
-    res[c][1] = accs[c].im + accs[c].cim;
+    res[c][0] = COMP ? accs[c].re + accs[c].cre : accs[c].re;
+    res[c][1] = COMP ? accs[c].im + accs[c].cim : accs[c].im;
   }
   if constexpr (KIND == 0) {
     const float4 v = make_float4(res[0][0], res[0][1], res[1][0], res[1][1]);
@@ -765,7 +766,7 @@ __device__ __forceinline__ void group_block_one(const uint4 (&bw)[8], const floa
       for (int j = 0; j < 5; ++j) bk_lo_mac(acc, lc[j], at_bytes(lp + plo, off[j]));
     }
   }
-  const float re = acc.re + acc.cre, im = acc.im + acc.cim;
+  const float re = COMP ? acc.re + acc.cre : acc.re, im = COMP ? acc.im + acc.cim : acc.im;
   if constexpr (KIND == 0) {
     if (in) static_cast<float2*>(out)[CB + u] = make_float2(re, im);
     if constexpr (COMP) {

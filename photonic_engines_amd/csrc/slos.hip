@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_layer_simple(LayerArgs a) {
             acc.cim = fmaf(u.y, w.x, acc.cim);
           });
       }
-      float re = acc.re + acc.cre, im = acc.im + acc.cim;
+      float re = COMP ? acc.re + acc.cre : acc.re, im = COMP ? acc.im + acc.cim : acc.im;
       if constexpr (KIND == 0) {
         reinterpret_cast<float2*>(a.out)[(size_t)b * a.S + g] = make_float2(re, im);
         if constexpr (COMP) {
@@ -944,7 +944,7 @@ __global__ __launch_bounds__(256) void k_layer_simple_group(slos::GroupLaunch gl
             acc.cim = fmaf(u.y, v.x, acc.cim);
           });
       }
-      float re = acc.re + acc.cre, im = acc.im + acc.cim;
+      float re = COMP ? acc.re + acc.cre : acc.re, im = COMP ? acc.im + acc.cim : acc.im;
       if (it.kind == 1) {
         int pos[kMaxPhotons];
         sv.unrank(g, K, pos);
