@@ -1,8 +1,9 @@
 // complex128 / float64 forms of the thread-per-state kernels: FockSLOSKernel(dtype=torch.complex128)
 // (photonic_engines/slos/kernel.py:117-118 computes every layer in complex128 and the probabilities
 // in float64). The same pull recurrence, readout, loss descent and adjoint as the complex64 kernels
-// of slos.hip, one thread per state, with the parents and children of pull_index.hpp; every sum in
-// FP64, every state written exactly once, no atomics.
+// of slos.hip, one thread per state, with the parents and children of pull_index.hpp (the forms
+// for at most KM photons, whose per-state arrays stay in registers); every sum in FP64, every state
+// written exactly once, no atomics.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -12,14 +13,19 @@
 
 namespace slos {
 
-// prod_j t_j! of a sorted tuple in FP64 (exact up to 18! per mode, far beyond the float form)
-SLOS_HD double run_factorial_f64(const int* p, int K, int multiset) {
+// prod_j t_j! of a sorted K-tuple (K <= KM) in FP64 (exact up to 18! per mode, far beyond the
+// float form)
+template <int KM>
+SLOS_HD double run_factorial_f64(const int (&p)[KM], int K, int multiset) {
   double f = 1.0, run = 1.0;
-  if (multiset)
-    for (int i = 1; i < K; ++i) {
-      run = (p[i] == p[i - 1]) ? run + 1.0 : 1.0;
-      f *= run;
-    }
+  if (multiset) {
+#pragma unroll
+    for (int i = 1; i < KM; ++i)
+      if (i < K) {
+        run = (p[i] == p[i - 1]) ? run + 1.0 : 1.0;
+        f *= run;
+      }
+  }
   return f;
 }
 
@@ -54,9 +60,9 @@ __host__ __device__ inline size_t space_lds_f64(int K, int me) {
   return ((size_t)(K + 1) * (me + 1) * 8 + (size_t)me * 4 + 15) & ~(size_t)15;
 }
 
-// layer K from layer K-1: KIND 0 raw complex128, 1 float64 probabilities |amp s|^2 prod t!,
-// 2 scaled complex128 amplitudes
-template <int KIND>
+// layer K from layer K-1 (K <= KM): KIND 0 raw complex128, 1 float64 probabilities
+// |amp s|^2 prod t!, 2 scaled complex128 amplitudes
+template <int KIND, int KM>
 __global__ __launch_bounds__(256) void k_layer_f64(LayerArgsF64 a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const SpaceView sv = load_space_f64(smem, a.cg, a.lo, a.me, a.K);
@@ -68,8 +74,10 @@ __global__ __launch_bounds__(256) void k_layer_f64(LayerArgsF64 a) {
     __syncthreads();
     const double2* pb = a.par + (size_t)b * a.S_par;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.S; g += (uint64_t)gridDim.x * blockDim.x) {
+      int p[KM];
+      unrank_fixed<KM>(sv, g, K, p);
       double re = 0.0, im = 0.0;
-      for_each_parent(sv, a.multiset, K, g, [&](uint64_t pr, int mode, int) {
+      parents_fixed<KM>(sv, a.multiset, K, p, [&](uint64_t pr, int mode, int) {
         const double2 u = ucol[mode], v = pb[pr];
         re = fma(u.x, v.x, re);
         re = fma(-u.y, v.y, re);
@@ -79,9 +87,7 @@ __global__ __launch_bounds__(256) void k_layer_f64(LayerArgsF64 a) {
       if constexpr (KIND == 0) {
         static_cast<double2*>(a.out)[(size_t)b * a.S + g] = make_double2(re, im);
       } else {
-        int p[kMaxK];
-        sv.unrank(g, K, p);
-        const double f = run_factorial_f64(p, K, a.multiset);
+        const double f = run_factorial_f64<KM>(p, K, a.multiset);
         re *= a.scale;
         im *= a.scale;
         if constexpr (KIND == 1) {
@@ -108,7 +114,9 @@ __global__ void k_vacuum_out_f64(void* out, int64_t n, int probs, double scale) 
   }
 }
 
-// uniform loss: q_{k-1}(t) = scale * (sum_j q_k(t + e_j) (t_j + 1)) / k (children in ascending mode)
+// uniform loss: q_{k-1}(t) = scale * (sum_j q_k(t + e_j) (t_j + 1)) / k (children in ascending
+// mode; k - 1 <= KM)
+template <int KM>
 __global__ __launch_bounds__(256) void k_descend_f64(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k,
                                                      const double* __restrict__ qk, uint64_t Sk,
                                                      double* __restrict__ qout, uint64_t Sp, int64_t B, double scale) {
@@ -118,14 +126,17 @@ __global__ __launch_bounds__(256) void k_descend_f64(const uint64_t* cg_g, const
   for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
     const double* qb = qk + (size_t)b * Sk;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < Sp; t += (uint64_t)gridDim.x * blockDim.x) {
+      int p[KM];
+      unrank_fixed<KM>(sv, t, k - 1, p);
       double acc = 0.0;
-      for_each_child(sv, multiset, k, t, [&](uint64_t c, int, int cnt) { acc = fma(qb[c], (double)cnt, acc); });
+      children_fixed<KM>(sv, multiset, k, p, [&](uint64_t c, int, int cnt) { acc = fma(qb[c], (double)cnt, acc); });
       qout[(size_t)b * Sp + t] = (acc / (double)k) * scale;
     }
   }
 }
 
-// adjoint of the descent: g_k[c] = sum_{j distinct in c} g_{k-1}[c - e_j] count_j(c) / k
+// adjoint of the descent: g_k[c] = sum_{j distinct in c} g_{k-1}[c - e_j] count_j(c) / k (k <= KM)
+template <int KM>
 __global__ __launch_bounds__(256) void k_ascend_f64(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k,
                                                     const double* __restrict__ gp, uint64_t Sp,
                                                     double* __restrict__ gk, uint64_t Sk, int64_t B) {
@@ -135,14 +146,17 @@ __global__ __launch_bounds__(256) void k_ascend_f64(const uint64_t* cg_g, const 
   for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
     const double* pb = gp + (size_t)b * Sp;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < Sk; g += (uint64_t)gridDim.x * blockDim.x) {
+      int p[KM];
+      unrank_fixed<KM>(sv, g, k, p);
       double acc = 0.0;
-      for_each_parent(sv, multiset, k, g, [&](uint64_t pr, int, int cnt) { acc = fma(pb[pr], (double)cnt, acc); });
+      parents_fixed<KM>(sv, multiset, k, p, [&](uint64_t pr, int, int cnt) { acc = fma(pb[pr], (double)cnt, acc); });
       gk[(size_t)b * Sk + g] = acc / (double)k;
     }
   }
 }
 
-// readout: p = |a s|^2 f  =>  g_a = 2 s^2 f grad_p a
+// readout: p = |a s|^2 f  =>  g_a = 2 s^2 f grad_p a (K <= KM)
+template <int KM>
 __global__ __launch_bounds__(256) void k_readout_adjoint_f64(const uint64_t* cg_g, const int* lo_g, int me, int multiset,
                                                              int K, const double2* __restrict__ a,
                                                              const double* __restrict__ gp, double2* __restrict__ g,
@@ -151,9 +165,9 @@ __global__ __launch_bounds__(256) void k_readout_adjoint_f64(const uint64_t* cg_
   const SpaceView sv = load_space_f64(smem, cg_g, lo_g, me, K);
   __syncthreads();
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < S; c += (uint64_t)gridDim.x * blockDim.x) {
-    int p[kMaxK];
-    sv.unrank(c, K, p);
-    const double f = run_factorial_f64(p, K, multiset);
+    int p[KM];
+    unrank_fixed<KM>(sv, c, K, p);
+    const double f = run_factorial_f64<KM>(p, K, multiset);
     for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
       const double2 v = a[(size_t)b * S + c];
       const double w = 2.0 * s2 * f * gp[(size_t)b * S + c];
@@ -164,8 +178,9 @@ __global__ __launch_bounds__(256) void k_readout_adjoint_f64(const uint64_t* cg_
 
 // g_{k-1}[t] = sum_{children c = t + e_j} conj(U[b, j, col]) g_k[c] (pull over children), and per
 // thread gU_j += conj(psi_{k-1}[t]) g_k[t + e_j] in a private LDS row; each work group then sums its
-// rows in thread order to partial[b][block][j]
+// rows in thread order to partial[b][block][j] (k - 1 <= KM)
 constexpr int kAdjThreadsF64 = 64;
+template <int KM>
 __global__ __launch_bounds__(kAdjThreadsF64) void k_adjoint_layer_f64(const uint64_t* cg_g, const int* lo_g, int me,
                                                                        int multiset, int k, int m, int col,
                                                                        const double2* __restrict__ U,
@@ -186,8 +201,10 @@ __global__ __launch_bounds__(kAdjThreadsF64) void k_adjoint_layer_f64(const uint
   const double2* pb = psi + (size_t)b * Sp;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < Sp; t += (uint64_t)gridDim.x * blockDim.x) {
     const double2 ps = pb[t];
+    int p[KM];
+    unrank_fixed<KM>(sv, t, k - 1, p);
     double gr = 0.0, gi = 0.0;
-    for_each_child(sv, multiset, k, t, [&](uint64_t c, int j, int) {
+    children_fixed<KM>(sv, multiset, k, p, [&](uint64_t c, int j, int) {
       const double2 gv = gb[c];
       const double2 u = ucol[j];
       gr += u.x * gv.x + u.y * gv.y;  // conj(u) * gv

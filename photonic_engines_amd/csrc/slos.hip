@@ -191,7 +191,8 @@ struct LayerArgs {
 };
 
 // KIND 0: intermediate layer (complex64), 1: probabilities (float32), 2: scaled amplitudes
-template <int KIND, bool COMP>
+// (K <= KM: the per-state position arrays stay in registers)
+template <int KIND, bool COMP, int KM>
 __global__ __launch_bounds__(256) void k_layer_simple(LayerArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int stride = a.me + 1;
@@ -211,12 +212,14 @@ __global__ __launch_bounds__(256) void k_layer_simple(LayerArgs a) {
     const float2* pb = a.par + (size_t)b * a.S_par;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.S;
          g += (uint64_t)gridDim.x * blockDim.x) {
+      int p[KM];
+      slos::unrank_fixed<KM>(sv, g, K, p);
       Acc acc;
-      slos::for_each_parent(sv, a.multiset, K, g,
-                            [&](uint64_t pr, int mode, int) { cmac<COMP>(acc, ucol[mode], pb[pr]); });
+      slos::parents_fixed<KM>(sv, a.multiset, K, p,
+                              [&](uint64_t pr, int mode, int) { cmac<COMP>(acc, ucol[mode], pb[pr]); });
       if constexpr (COMP) {
         if (a.par_lo)  // the parents' low-order parts, into the correction terms
-          slos::for_each_parent(sv, a.multiset, K, g, [&](uint64_t pr, int mode, int) {
+          slos::parents_fixed<KM>(sv, a.multiset, K, p, [&](uint64_t pr, int mode, int) {
             const float2 u = ucol[mode], w = pb[a.par_lo + pr];
             acc.cre = fmaf(u.x, w.x, acc.cre);
             acc.cre = fmaf(-u.y, w.y, acc.cre);
@@ -233,9 +236,7 @@ __global__ __launch_bounds__(256) void k_layer_simple(LayerArgs a) {
                 make_float2(acc.cre - (re - acc.re), acc.cim - (im - acc.im));
         }
       } else {
-        int p[kMaxPhotons];
-        sv.unrank(g, K, p);
-        const float f = slos::run_factorial(p, K, a.multiset);
+        const float f = slos::run_factorial_fixed<KM>(p, K, a.multiset);
         re *= a.scale;
         im *= a.scale;
         if constexpr (KIND == 1) {
@@ -380,7 +381,8 @@ __global__ __launch_bounds__(64) void k_spot_ryser(const uint64_t* cg_g, const i
 // ------------------------------------------------------------------------------------------
 // uniform loss: q_{k-1}(t) = sum_j q_k(t + e_j) (t_j + 1) / k  (pull over children, no atomics)
 // ------------------------------------------------------------------------------------------
-// (child ranks: slos::for_each_child in pull_index.hpp)
+// (child ranks: slos::children_fixed in pull_index.hpp; k - 1 <= KM)
+template <int KM>
 __global__ __launch_bounds__(256) void k_descend(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k,
                                                  const float* __restrict__ qk, uint64_t Sk,
                                                  float* __restrict__ qout, uint64_t Sp, int64_t B, float scale) {
@@ -397,8 +399,10 @@ __global__ __launch_bounds__(256) void k_descend(const uint64_t* cg_g, const int
   for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
     const float* qb = qk + (size_t)b * Sk;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < Sp; t += (uint64_t)gridDim.x * blockDim.x) {
+      int p[KM];
+      slos::unrank_fixed<KM>(sv, t, k - 1, p);
       float acc = 0.f;
-      slos::for_each_child(sv, multiset, k, t, [&](uint64_t c, int, int cnt) { acc = fmaf(qb[c], (float)cnt, acc); });
+      slos::children_fixed<KM>(sv, multiset, k, p, [&](uint64_t c, int, int cnt) { acc = fmaf(qb[c], (float)cnt, acc); });
       qout[(size_t)b * Sp + t] = (acc * invk) * scale;
     }
   }
@@ -406,7 +410,8 @@ __global__ __launch_bounds__(256) void k_descend(const uint64_t* cg_g, const int
 
 // probability-level raise (a photon that interferes with nothing, partial distinguishability):
 // q_k(c) = sum_{j distinct in c} w[b][j] q_{k-1}(c - e_j), w = |U[:, col]|^2 (pull over parents,
-// ascending photon position, no atomics)
+// ascending photon position, no atomics; k <= KM)
+template <int KM>
 __global__ __launch_bounds__(256) void k_raise(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k, int m,
                                                const float* __restrict__ w, const float* __restrict__ qp, uint64_t Sp,
                                                float* __restrict__ qout, uint64_t Sk, int64_t B) {
@@ -425,15 +430,18 @@ __global__ __launch_bounds__(256) void k_raise(const uint64_t* cg_g, const int* 
     __syncthreads();
     const float* qb = qp + (size_t)b * Sp;
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < Sk; c += (uint64_t)gridDim.x * blockDim.x) {
+      int p[KM];
+      slos::unrank_fixed<KM>(sv, c, k, p);
       float acc = 0.f;
-      slos::for_each_parent(sv, multiset, k, c, [&](uint64_t pr, int mode, int) { acc = fmaf(wl[mode], qb[pr], acc); });
+      slos::parents_fixed<KM>(sv, multiset, k, p, [&](uint64_t pr, int mode, int) { acc = fmaf(wl[mode], qb[pr], acc); });
       qout[(size_t)b * Sk + c] = acc;
     }
   }
 }
 
 // adjoint of the descent (gradient of q_{k-1} w.r.t. q_k): g_k[c] = sum_{j distinct in c}
-// g_{k-1}[c - e_j] * count_j(c) / k  — the forward pull structure with real weights
+// g_{k-1}[c - e_j] * count_j(c) / k  — the forward pull structure with real weights (k <= KM)
+template <int KM>
 __global__ __launch_bounds__(256) void k_ascend(const uint64_t* cg_g, const int* lo_g, int me, int multiset, int k,
                                                 const float* __restrict__ gp, uint64_t Sp,
                                                 float* __restrict__ gk, uint64_t Sk, int64_t B) {
@@ -451,8 +459,10 @@ __global__ __launch_bounds__(256) void k_ascend(const uint64_t* cg_g, const int*
   for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
     const float* pb = gp + (size_t)b * Sp;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < Sk; g += (uint64_t)gridDim.x * blockDim.x) {
+      int p[KM];
+      slos::unrank_fixed<KM>(sv, g, K, p);
       float acc = 0.f;
-      slos::for_each_parent(sv, multiset, K, g, [&](uint64_t pr, int, int cnt) { acc += pb[pr] * (float)cnt; });
+      slos::parents_fixed<KM>(sv, multiset, K, p, [&](uint64_t pr, int, int cnt) { acc += pb[pr] * (float)cnt; });
       gk[(size_t)b * Sk + g] = acc * invk;
     }
   }
@@ -491,7 +501,9 @@ __global__ __launch_bounds__(256) void k_readout_adjoint(const uint64_t* cg_g, c
 // g_{k-1}[t] = sum_{children c = t + e_j} conj(U[b, j, col]) g_k[c]   (pull over children)
 // and, per thread, gU_j += conj(psi_{k-1}[t]) g_k[t + e_j] in a private LDS row (no atomics);
 // each block then reduces its rows to partial[b][block][j] (FP64 stage in k_reduce_gu)
+// (k - 1 <= KM)
 constexpr int kAdjThreads = 64;
+template <int KM>
 __global__ __launch_bounds__(kAdjThreads) void k_adjoint_layer(const uint64_t* cg_g, const int* lo_g, int me, int multiset,
                                                              int k, int m, int col, const float2* __restrict__ U,
                                                              const float2* __restrict__ psi, const float2* __restrict__ gk,
@@ -516,8 +528,10 @@ __global__ __launch_bounds__(kAdjThreads) void k_adjoint_layer(const uint64_t* c
   const float2* pb = psi + (size_t)b * Sp;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < Sp; t += (uint64_t)gridDim.x * blockDim.x) {
     const float2 ps = pb[t];
+    int p[KM];
+    slos::unrank_fixed<KM>(sv, t, k - 1, p);
     float gr = 0.f, gi = 0.f;
-    slos::for_each_child(sv, multiset, k, t, [&](uint64_t c, int j, int) {
+    slos::children_fixed<KM>(sv, multiset, k, p, [&](uint64_t c, int j, int) {
       const float2 gv = gb[c];
       const float2 u = ucol[j];
       gr += u.x * gv.x + u.y * gv.y;   // conj(u) * gv
@@ -669,6 +683,15 @@ int launch_block(const slos_plan* p, const DevTables& dt, const LayerArgs& a, in
   return SLOS_OK;
 }
 
+// calls f with the smallest photon bound KM (8, 16 or 32) of the thread-per-state kernels that
+// holds `photons` (their per-state arrays are sized by it and stay in registers)
+template <class F>
+void with_km(int photons, F&& f) {
+  if (photons <= 8) f(std::integral_constant<int, 8>{});
+  else if (photons <= 16) f(std::integral_constant<int, 16>{});
+  else f(std::integral_constant<int, 32>{});
+}
+
 // par_hilo / out_hilo (compensated sums only): the parent layer / the complex64 output layer is a
 // hi/lo pair [2][B][S], the low-order parts after the high ones
 int launch_layer(const slos_plan* p, const DevTables& dt, int K, int col, const float2* U, int64_t B,
@@ -688,12 +711,15 @@ int launch_layer(const slos_plan* p, const DevTables& dt, int K, int col, const 
   if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   const uint64_t blocks = std::min<uint64_t>((S + 255) / 256, 1u << 20);
   dim3 grid((unsigned)std::max<uint64_t>(blocks, 1), (unsigned)std::min<int64_t>(B, 65535));
-  if (kind == 0 && !comp) hipLaunchKernelGGL((k_layer_simple<0, false>), grid, dim3(256), sh, st, a);
-  else if (kind == 0) hipLaunchKernelGGL((k_layer_simple<0, true>), grid, dim3(256), sh, st, a);
-  else if (kind == 1 && !comp) hipLaunchKernelGGL((k_layer_simple<1, false>), grid, dim3(256), sh, st, a);
-  else if (kind == 1) hipLaunchKernelGGL((k_layer_simple<1, true>), grid, dim3(256), sh, st, a);
-  else if (!comp) hipLaunchKernelGGL((k_layer_simple<2, false>), grid, dim3(256), sh, st, a);
-  else hipLaunchKernelGGL((k_layer_simple<2, true>), grid, dim3(256), sh, st, a);
+  with_km(K, [&](auto km) {
+    constexpr int KM = decltype(km)::value;
+    if (kind == 0 && !comp) hipLaunchKernelGGL((k_layer_simple<0, false, KM>), grid, dim3(256), sh, st, a);
+    else if (kind == 0) hipLaunchKernelGGL((k_layer_simple<0, true, KM>), grid, dim3(256), sh, st, a);
+    else if (kind == 1 && !comp) hipLaunchKernelGGL((k_layer_simple<1, false, KM>), grid, dim3(256), sh, st, a);
+    else if (kind == 1) hipLaunchKernelGGL((k_layer_simple<1, true, KM>), grid, dim3(256), sh, st, a);
+    else if (!comp) hipLaunchKernelGGL((k_layer_simple<2, false, KM>), grid, dim3(256), sh, st, a);
+    else hipLaunchKernelGGL((k_layer_simple<2, true, KM>), grid, dim3(256), sh, st, a);
+  });
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -711,9 +737,14 @@ int launch_layer_f64(const slos_plan* p, const DevTables& dt, int K, int col, co
   if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   const uint64_t blocks = std::min<uint64_t>((S + 255) / 256, 1u << 20);
   dim3 grid((unsigned)std::max<uint64_t>(blocks, 1), (unsigned)std::min<int64_t>(B, 65535));
-  if (kind == 0) hipLaunchKernelGGL(slos::k_layer_f64<0>, grid, dim3(256), sh, st, a);
-  else if (kind == 1) hipLaunchKernelGGL(slos::k_layer_f64<1>, grid, dim3(256), sh, st, a);
-  else hipLaunchKernelGGL(slos::k_layer_f64<2>, grid, dim3(256), sh, st, a);
+#define SLOS_F64L(KM)                                                                         \
+  if (kind == 0) hipLaunchKernelGGL((slos::k_layer_f64<0, KM>), grid, dim3(256), sh, st, a);  \
+  else if (kind == 1) hipLaunchKernelGGL((slos::k_layer_f64<1, KM>), grid, dim3(256), sh, st, a); \
+  else hipLaunchKernelGGL((slos::k_layer_f64<2, KM>), grid, dim3(256), sh, st, a)
+  if (K <= 8) { SLOS_F64L(8); }
+  else if (K <= 16) { SLOS_F64L(16); }
+  else { SLOS_F64L(32); }
+#undef SLOS_F64L
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -901,7 +932,8 @@ __global__ __launch_bounds__(256) void k_unrank_occ(const uint64_t* cg_g, const 
 // ------------------------------------------------------------------------------------------
 
 // thread-per-state form of a group layer: item = (plan, instance, 256 * spt consecutive states)
-template <bool COMP>
+// (K <= KM)
+template <bool COMP, int KM>
 __global__ __launch_bounds__(256) void k_layer_simple_group(slos::GroupLaunch gl) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int K = gl.K;
@@ -931,12 +963,14 @@ __global__ __launch_bounds__(256) void k_layer_simple_group(slos::GroupLaunch gl
     for (uint32_t i = 0; i < spt; ++i) {
       const uint64_t g = g0 + 256ull * i;
       if (g >= it.S) break;
+      int pos[KM];
+      slos::unrank_fixed<KM>(sv, g, K, pos);
       Acc acc;
-      slos::for_each_parent(sv, it.multiset, K, g,
-                            [&](uint64_t pr, int mode, int) { cmac<COMP>(acc, ucol[mode], pb[pr]); });
+      slos::parents_fixed<KM>(sv, it.multiset, K, pos,
+                              [&](uint64_t pr, int mode, int) { cmac<COMP>(acc, ucol[mode], pb[pr]); });
       if constexpr (COMP) {
         if (it.par_lo)  // the parents' low-order parts, into the correction terms
-          slos::for_each_parent(sv, it.multiset, K, g, [&](uint64_t pr, int mode, int) {
+          slos::parents_fixed<KM>(sv, it.multiset, K, pos, [&](uint64_t pr, int mode, int) {
             const float2 u = ucol[mode], v = pb[it.par_lo + pr];
             acc.cre = fmaf(u.x, v.x, acc.cre);
             acc.cre = fmaf(-u.y, v.y, acc.cre);
@@ -946,9 +980,7 @@ __global__ __launch_bounds__(256) void k_layer_simple_group(slos::GroupLaunch gl
       }
       float re = COMP ? acc.re + acc.cre : acc.re, im = COMP ? acc.im + acc.cim : acc.im;
       if (it.kind == 1) {
-        int pos[kMaxPhotons];
-        sv.unrank(g, K, pos);
-        const float f = slos::run_factorial(pos, K, it.multiset);
+        const float f = slos::run_factorial_fixed<KM>(pos, K, it.multiset);
         re *= it.scale;
         im *= it.scale;
         reinterpret_cast<float*>(it.out)[(size_t)b * it.S + g] = slos::prob_readout(re, im, f);
@@ -1161,8 +1193,12 @@ int group_layer(const slos_group* G, int K, const GroupArgs& a, const GroupWs& w
   const bool comp = a.flags & SLOS_FLAG_COMPENSATED;
   if (block) comp ? launch_block_group<true>(K - slos::kSuf, grid, sh, st, gl)
                   : launch_block_group<false>(K - slos::kSuf, grid, sh, st, gl);
-  else if (comp) hipLaunchKernelGGL((k_layer_simple_group<true>), dim3(grid), dim3(256), sh, st, gl);
-  else hipLaunchKernelGGL((k_layer_simple_group<false>), dim3(grid), dim3(256), sh, st, gl);
+  else
+    with_km(K, [&](auto km) {
+      constexpr int KM = decltype(km)::value;
+      if (comp) hipLaunchKernelGGL((k_layer_simple_group<true, KM>), dim3(grid), dim3(256), sh, st, gl);
+      else hipLaunchKernelGGL((k_layer_simple_group<false, KM>), dim3(grid), dim3(256), sh, st, gl);
+    });
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -1717,8 +1753,11 @@ int slos_adjoint_layer(const slos_plan* p, const void* U, int64_t batch, int k, 
 #undef SLOS_ADJ_E
 #undef SLOS_ADJ
   } else {
-    hipLaunchKernelGGL(k_adjoint_layer, dim3((unsigned)L.nblk, (unsigned)batch), dim3(kAdjThreads), L.lds, st, dt.cg,
-                       dt.lo, p->me, p->multiset, k, p->m, col, Uf, psi, gk, Sk, gout, Sp, part, grad_prev != nullptr);
+    with_km(k - 1, [&](auto km) {
+      hipLaunchKernelGGL((k_adjoint_layer<decltype(km)::value>), dim3((unsigned)L.nblk, (unsigned)batch),
+                         dim3(kAdjThreads), L.lds, st, dt.cg, dt.lo, p->me, p->multiset, k, p->m, col, Uf, psi, gk, Sk,
+                         gout, Sp, part, grad_prev != nullptr);
+    });
   }
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_gu, dim3((unsigned)p->me, (unsigned)batch), dim3(64), 0, st,
@@ -1771,8 +1810,10 @@ int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1,
   const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
   if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 255) / 256, 1u << 20)), (unsigned)batch);
-  hipLaunchKernelGGL(k_descend, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
-                     p->multiset, k, q_k, Sk, q_km1, Sp, batch, (float)scale);
+  with_km(k - 1, [&](auto km) {
+    hipLaunchKernelGGL((k_descend<decltype(km)::value>), grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream),
+                       dt.cg, dt.lo, p->me, p->multiset, k, q_k, Sk, q_km1, Sp, batch, (float)scale);
+  });
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -1789,8 +1830,10 @@ int slos_raise(const slos_plan* p, int k, const float* w, const float* q_km1, fl
   const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4 + (size_t)p->m * 4;
   if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sk + 255) / 256, 1u << 20)), (unsigned)batch);
-  hipLaunchKernelGGL(k_raise, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
-                     p->multiset, k, p->m, w, q_km1, Sp, q_k, Sk, batch);
+  with_km(k, [&](auto km) {
+    hipLaunchKernelGGL((k_raise<decltype(km)::value>), grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream),
+                       dt.cg, dt.lo, p->me, p->multiset, k, p->m, w, q_km1, Sp, q_k, Sk, batch);
+  });
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -1806,8 +1849,10 @@ int slos_loss_ascend(const slos_plan* p, int k, const float* g_km1, float* g_k, 
   const size_t sh = (size_t)(k + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
   if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sk + 255) / 256, 1u << 20)), (unsigned)batch);
-  hipLaunchKernelGGL(k_ascend, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me,
-                     p->multiset, k, g_km1, Sp, g_k, Sk, batch);
+  with_km(k, [&](auto km) {
+    hipLaunchKernelGGL((k_ascend<decltype(km)::value>), grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream),
+                       dt.cg, dt.lo, p->me, p->multiset, k, g_km1, Sp, g_k, Sk, batch);
+  });
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -2114,8 +2159,14 @@ int slos_loss_descend_f64(const slos_plan* p, int k, const double* q_k, double* 
   std::snprintf(name, sizeof name, "slos descent %d", k);
   const RoctxRange range(name);
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sp + 255) / 256, 1u << 20)), (unsigned)batch);
-  hipLaunchKernelGGL(slos::k_descend_f64, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo,
-                     p->me, p->multiset, k, q_k, Sk, q_km1, Sp, batch, scale);
+  const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+#define SLOS_F64D(KM) \
+  hipLaunchKernelGGL((slos::k_descend_f64<KM>), grid, dim3(256), sh, st, dt.cg, dt.lo, p->me, p->multiset, k, q_k, Sk, \
+                     q_km1, Sp, batch, scale)
+  if (k - 1 <= 8) { SLOS_F64D(8); }
+  else if (k - 1 <= 16) { SLOS_F64D(16); }
+  else { SLOS_F64D(32); }
+#undef SLOS_F64D
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -2130,8 +2181,14 @@ int slos_loss_ascend_f64(const slos_plan* p, int k, const double* g_km1, double*
   const size_t sh = slos::space_lds_f64(k, p->me);
   if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((Sk + 255) / 256, 1u << 20)), (unsigned)batch);
-  hipLaunchKernelGGL(slos::k_ascend_f64, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo,
-                     p->me, p->multiset, k, g_km1, Sp, g_k, Sk, batch);
+  const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+#define SLOS_F64A(KM)                                                                                                \
+  hipLaunchKernelGGL((slos::k_ascend_f64<KM>), grid, dim3(256), sh, st, dt.cg, dt.lo, p->me, p->multiset, k, g_km1, Sp, \
+                     g_k, Sk, batch)
+  if (k <= 8) { SLOS_F64A(8); }
+  else if (k <= 16) { SLOS_F64A(16); }
+  else { SLOS_F64A(32); }
+#undef SLOS_F64A
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -2148,9 +2205,14 @@ int slos_readout_adjoint_f64(const slos_plan* p, int64_t batch, const void* amps
   if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((S + 255) / 256, 1u << 20)),
             (unsigned)std::min<int64_t>(batch, 16));
-  hipLaunchKernelGGL(slos::k_readout_adjoint_f64, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg,
-                     dt.lo, p->me, p->multiset, p->n, static_cast<const double2*>(amps), grad_p,
-                     static_cast<double2*>(grad_amps), S, batch, scale * scale);
+  const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+#define SLOS_F64R(KM)                                                                                                 \
+  hipLaunchKernelGGL((slos::k_readout_adjoint_f64<KM>), grid, dim3(256), sh, st, dt.cg, dt.lo, p->me, p->multiset, p->n, \
+                     static_cast<const double2*>(amps), grad_p, static_cast<double2*>(grad_amps), S, batch, scale * scale)
+  if (p->n <= 8) { SLOS_F64R(8); }
+  else if (p->n <= 16) { SLOS_F64R(16); }
+  else { SLOS_F64R(32); }
+#undef SLOS_F64R
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }
@@ -2180,11 +2242,15 @@ int slos_adjoint_layer_f64(const slos_plan* p, const void* U, int64_t batch, int
   if (rc) return rc;
   if (!ws || ws_bytes < (size_t)batch * nblk * p->me * sizeof(double2)) return fail(SLOS_ENOMEM, "workspace too small");
   const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(slos::k_adjoint_layer_f64, dim3((unsigned)nblk, (unsigned)batch), dim3(slos::kAdjThreadsF64), lds,
-                     st, dt.cg, dt.lo, p->me, p->multiset, k, p->m, col, static_cast<const double2*>(U),
-                     static_cast<const double2*>(psi_prev), static_cast<const double2*>(grad_k), p->size(k),
-                     static_cast<double2*>(grad_prev), p->size(k - 1), static_cast<double2*>(ws),
-                     grad_prev != nullptr);
+#define SLOS_F64J(KM)                                                                                              \
+  hipLaunchKernelGGL((slos::k_adjoint_layer_f64<KM>), dim3((unsigned)nblk, (unsigned)batch), dim3(slos::kAdjThreadsF64), \
+                     lds, st, dt.cg, dt.lo, p->me, p->multiset, k, p->m, col, static_cast<const double2*>(U),         \
+                     static_cast<const double2*>(psi_prev), static_cast<const double2*>(grad_k), p->size(k),          \
+                     static_cast<double2*>(grad_prev), p->size(k - 1), static_cast<double2*>(ws), grad_prev != nullptr)
+  if (k - 1 <= 8) { SLOS_F64J(8); }
+  else if (k - 1 <= 16) { SLOS_F64J(16); }
+  else { SLOS_F64J(32); }
+#undef SLOS_F64J
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(slos::k_reduce_gu_f64, dim3((unsigned)p->me, (unsigned)batch), dim3(64), 0, st,
                      static_cast<const double2*>(ws), (int)nblk, p->me, p->m, col, static_cast<double2*>(grad_U),
