@@ -785,80 +785,86 @@ constexpr int kSampleChunk = 64;
 constexpr int kSampleBlock = kSampleChunk * 64;   // 4096 entries
 constexpr int kSampleSeg = 1024;                   // blocks per prefix segment
 
-// One wave per block of 4096 entries: 16-byte coalesced loads into an LDS tile of 64 rows (one per
-// chunk, padded to 65 columns so that lane c's row reads do not collide), then lane c sums chunk c
-// in index order (FP64, from 0.0; entries past S count as 0) and stores it, and every lane sums the
-// 64 chunk sums in order.
-constexpr int kSumWaves = 2;
+// One wave per block of 4096 entries; lane c loads its own chunk c (64 consecutive entries, 16 bytes
+// at a time: the two cache lines of a chunk are read by one lane in consecutive loads) and sums it
+// in index order (FP64, from 0.0; entries past S count as 0), then every lane sums the 64 chunk
+// sums in chunk order. No LDS, so many waves stay resident to cover the stream.
+constexpr int kSumWaves = 4;
 __global__ __launch_bounds__(64 * kSumWaves) void k_sample_block_sums(const float* __restrict__ p, uint64_t S, uint64_t nb,
                                                                       double* __restrict__ bsum,
                                                                       double* __restrict__ csum) {
-  __shared__ float tile[kSumWaves][kSampleChunk * (kSampleChunk + 1)];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint64_t b = (uint64_t)blockIdx.x * kSumWaves + (uint64_t)w;
+  const int lane = threadIdx.x & 63;
+  const uint64_t b = (uint64_t)blockIdx.x * kSumWaves + (uint64_t)(threadIdx.x >> 6);
   if (b >= nb) return;
-  float* t = tile[w];
-  const uint64_t base = b * kSampleBlock;
-#pragma unroll 4
-  for (int j = 0; j < kSampleBlock / 256; ++j) {
-    const int e0 = j * 256 + 4 * lane;  // entry of the block: row e0 / 64, columns e0 % 64 .. + 3
-    const uint64_t i0 = base + (uint64_t)e0;
-    float v[4];
-    if (i0 + 4 <= S) {
-      __builtin_memcpy(v, p + i0, 16);
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) v[e] = i0 + e < S ? p[i0 + e] : 0.f;
-    }
-    float* row = t + (e0 >> 6) * (kSampleChunk + 1) + (e0 & 63);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) row[e] = v[e];
-  }
-  slos::wave_sync();
+  const uint64_t i0 = b * kSampleBlock + (uint64_t)lane * kSampleChunk;
   double cs = 0.0;
-  const float* mine = t + lane * (kSampleChunk + 1);
-  for (int k = 0; k < kSampleChunk; ++k) cs += (double)mine[k];
+  if (i0 + kSampleChunk <= S) {
+    float4 v[kSampleChunk / 4];
+#pragma unroll
+    for (int j = 0; j < kSampleChunk / 4; ++j) __builtin_memcpy(&v[j], p + i0 + 4 * j, 16);
+#pragma unroll
+    for (int j = 0; j < kSampleChunk / 4; ++j) {
+      cs += (double)v[j].x;
+      cs += (double)v[j].y;
+      cs += (double)v[j].z;
+      cs += (double)v[j].w;
+    }
+  } else {
+    for (int k = 0; k < kSampleChunk; ++k) cs += (double)(i0 + k < S ? p[i0 + k] : 0.f);
+  }
   csum[b * 64 + (uint64_t)lane] = cs;
   double tot = 0.0;
   for (int c = 0; c < 64; ++c) tot += __shfl(cs, c, 64);
   if (lane == 0) bsum[b] = tot;
 }
 
-// segment prefix: thread t owns blocks [t*kSampleSeg, (t+1)*kSampleSeg): local inclusive prefix;
-// then the exclusive prefix of the segment totals (thread 0, in order), then cdf = offset + local
-__global__ __launch_bounds__(1024) void k_sample_block_cdf(const double* __restrict__ bsum, uint64_t nb,
-                                                            double* __restrict__ cdf) {
-  __shared__ double tot[1024];
-  const uint64_t t = threadIdx.x;
-  const uint64_t a = t * kSampleSeg;
-  double acc = 0.0;
-  for (uint64_t i = a; i < a + kSampleSeg && i < nb; ++i) {
-    acc += bsum[i];
-    cdf[i] = acc;
-  }
-  tot[t] = acc;
+// Block prefix over segments of kSampleSeg blocks: one work group per segment stages its block sums
+// in LDS (coalesced), one thread takes their inclusive prefix in order (from 0.0) and the group
+// writes it back (coalesced); segtot[seg] = the segment's total.
+__global__ __launch_bounds__(256) void k_sample_seg_prefix(const double* __restrict__ bsum, uint64_t nb,
+                                                           double* __restrict__ cdf, double* __restrict__ segtot) {
+  __shared__ double seg[kSampleSeg];
+  const uint64_t a = (uint64_t)blockIdx.x * kSampleSeg;
+  const int len = (int)(nb - a < (uint64_t)kSampleSeg ? nb - a : (uint64_t)kSampleSeg);
+  for (int i = threadIdx.x; i < len; i += blockDim.x) seg[i] = bsum[a + i];
   __syncthreads();
-  if (t == 0) {
-    double run = 0.0;
-    for (int i = 0; i < 1024; ++i) {
-      const double v = tot[i];
-      tot[i] = run;
-      run += v;
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    for (int i = 0; i < len; ++i) {
+      acc += seg[i];
+      seg[i] = acc;
     }
+    segtot[blockIdx.x] = acc;
   }
   __syncthreads();
-  const double off = tot[t];
-  for (uint64_t i = a; i < a + kSampleSeg && i < nb; ++i) cdf[i] = off + cdf[i];
+  for (int i = threadIdx.x; i < len; i += blockDim.x) cdf[a + i] = seg[i];
 }
 
-// one wave per shot: u = uniform * total; block = first cdf >= u (binary search), then the chunk
-// and the entry by in-order prefix sums; every lane computes the same values (lane 0 stores)
+// exclusive prefix of the segment totals, in order (one thread; at most 1024 segments)
+__global__ __launch_bounds__(64) void k_sample_seg_offsets(double* __restrict__ segtot, int nseg) {
+  if (threadIdx.x != 0) return;
+  double run = 0.0;
+  for (int i = 0; i < nseg; ++i) {
+    const double v = segtot[i];
+    segtot[i] = run;
+    run += v;
+  }
+}
+
+// cdf = segment offset + the segment's own prefix
+__global__ __launch_bounds__(256) void k_sample_add_offsets(double* __restrict__ cdf, uint64_t nb,
+                                                            const double* __restrict__ segoff) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (uint64_t)gridDim.x * blockDim.x)
+    cdf[i] = segoff[i / kSampleSeg] + cdf[i];
+}
+
+// one thread per shot: u = uniform * total; block = first cdf >= u (binary search), then the chunk
+// and the entry by in-order prefix sums of the block pass's chunk sums and of the chunk's entries
 __global__ __launch_bounds__(256) void k_sample_search(const float* __restrict__ p, uint64_t S, uint64_t nb,
                                                        const double* __restrict__ cdf, const double* __restrict__ csum,
                                                        const double* __restrict__ uni, int64_t shots,
                                                        uint64_t* __restrict__ idx) {
-  const int lane = threadIdx.x & 63;
-  const int64_t s = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= shots) return;
   const double total = cdf[nb - 1];
   const double u = uni[s] * total;
@@ -870,11 +876,11 @@ __global__ __launch_bounds__(256) void k_sample_search(const float* __restrict__
   }
   const uint64_t b = lo;
   double r = u - (b > 0 ? cdf[b - 1] : 0.0);
-  const double cs = csum[b * 64 + (uint64_t)lane];  // the block pass's chunk sums
+  const double* cb = csum + b * 64;
   int c = 63;
   double pre = 0.0;
   for (int q = 0; q < 64; ++q) {
-    const double v = __shfl(cs, q, 64);
+    const double v = cb[q];
     if (pre + v >= r) {
       c = q;
       break;
@@ -885,10 +891,8 @@ __global__ __launch_bounds__(256) void k_sample_search(const float* __restrict__
   const uint64_t first = b * kSampleBlock + (uint64_t)c * kSampleChunk;
   int k = 63;
   double acc = 0.0;
-  const uint64_t il = first + (uint64_t)lane;
-  const float pl = il < S ? p[il] : 0.f;  // the chunk's entries, one per lane
   for (int q = 0; q < kSampleChunk; ++q) {
-    acc += (double)__shfl(pl, q, 64);
+    acc += (double)(first + q < S ? p[first + q] : 0.f);
     if (acc >= r) {
       k = q;
       break;
@@ -896,7 +900,7 @@ __global__ __launch_bounds__(256) void k_sample_search(const float* __restrict__
   }
   uint64_t out = first + (uint64_t)k;
   if (out >= S) out = S - 1;
-  if (lane == 0) idx[s] = out;
+  idx[s] = out;
 }
 
 // lex index -> occupation row (int32 [m]) of layer k, one thread per index
@@ -1910,7 +1914,8 @@ int slos_sample_workspace(uint64_t count, size_t* bytes) {
   if (count < 1) return fail(SLOS_EINVAL, "count must be >= 1");
   const uint64_t nb = (count + kSampleBlock - 1) / kSampleBlock;
   if (nb > (uint64_t)kSampleSeg * 1024) return fail(SLOS_ERANGE, "too many categories for the sampler");
-  *bytes = 2 * align256(nb * sizeof(double)) + align256(nb * 64 * sizeof(double));
+  // block sums, the cdf, the chunk sums and the segment totals
+  *bytes = 2 * align256(nb * sizeof(double)) + align256(nb * 64 * sizeof(double)) + align256(1024 * sizeof(double));
   return SLOS_OK;
 }
 
@@ -1931,9 +1936,17 @@ int slos_sample(const float* probs, uint64_t count, const double* uniforms, int6
   hipLaunchKernelGGL(k_sample_block_sums, dim3((unsigned)((nb + kSumWaves - 1) / kSumWaves)), dim3(64 * kSumWaves), 0, st,
                      probs, count, nb, bsum, csum);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_sample_block_cdf, dim3(1), dim3(1024), 0, st, bsum, nb, cdf);
+  double* segtot = reinterpret_cast<double*>(static_cast<unsigned char*>(ws) + 2 * align256(nb * sizeof(double)) +
+                                             align256(nb * 64 * sizeof(double)));
+  const uint64_t nseg = (nb + kSampleSeg - 1) / kSampleSeg;  // <= 1024 (slos_sample_workspace)
+  hipLaunchKernelGGL(k_sample_seg_prefix, dim3((unsigned)nseg), dim3(256), 0, st, bsum, nb, cdf, segtot);
   HIP_TRY(hipGetLastError());
-  const uint64_t g = ((uint64_t)shots + 3) / 4;
+  hipLaunchKernelGGL(k_sample_seg_offsets, dim3(1), dim3(64), 0, st, segtot, (int)nseg);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_sample_add_offsets, dim3((unsigned)std::min<uint64_t>((nb + 255) / 256, 4096)), dim3(256), 0, st,
+                     cdf, nb, segtot);
+  HIP_TRY(hipGetLastError());
+  const uint64_t g = ((uint64_t)shots + 255) / 256;
   if (g > 0x7fffffffull) return fail(SLOS_ERANGE, "too many shots for one launch");
   hipLaunchKernelGGL(k_sample_search, dim3((unsigned)g), dim3(256), 0, st, probs, count, nb, cdf, csum, uniforms, shots,
                      idx);

@@ -3,7 +3,8 @@
 // blocks, group order), the descent's child indices, and the sampling kernels' index logic, each
 // at the sizes the GPU tests use. Test infrastructure; exits non-zero on the first failure.
 //
-// The sampling kernels (k_sample_block_sums, k_sample_block_cdf, k_sample_search, k_unrank_occ in
+// The sampling kernels (k_sample_block_sums, k_sample_seg_prefix, k_sample_seg_offsets,
+// k_sample_add_offsets, k_sample_search, k_unrank_occ in
 // photonic_engines_amd/csrc/slos.hip) are restated here lane by lane, with every global-memory
 // index checked against its buffer's size, at the launch shapes slos_sample and
 // slos_unrank_device use.
@@ -73,36 +74,32 @@ void check_forward(int n, int m) {
     }
 }
 
-// slos_sample's three kernels over p[S] for `shots` uniforms, restated lane by lane
+// slos_sample's kernels over p[S] for `shots` uniforms, restated lane by lane
 std::vector<uint64_t> sample_emulated(const std::vector<float>& pin, const std::vector<double>& uni) {
-  constexpr int kChunk = 64, kBlock = kChunk * 64, kSeg = 1024, kSumWaves = 2;
+  constexpr int kChunk = 64, kBlock = kChunk * 64, kSeg = 1024, kSumWaves = 4;
   const uint64_t S = pin.size();
   const uint64_t nb = (S + kBlock - 1) / kBlock;
   if (nb > (uint64_t)kSeg * 1024) throw std::runtime_error("too many categories");
   Checked<float> p(S, "probs");
   for (uint64_t i = 0; i < S; ++i) p[i] = pin[i];
-  Checked<double> bsum(nb, "bsum"), cdf(nb, "cdf"), csum(nb * 64, "csum");
-  // k_sample_block_sums: grid ceil(nb / kSumWaves) x (64 kSumWaves) threads, one wave per block
+  Checked<double> bsum(nb, "bsum"), cdf(nb, "cdf"), csum(nb * 64, "csum"), segtot(1024, "segtot");
+  // k_sample_block_sums: grid ceil(nb / kSumWaves) x (64 kSumWaves) threads, one wave per block,
+  // lane c sums chunk c
   const uint64_t grid1 = (nb + kSumWaves - 1) / kSumWaves;
   for (uint64_t bx = 0; bx < grid1; ++bx)
     for (int w = 0; w < kSumWaves; ++w) {
       const uint64_t b = bx * kSumWaves + (uint64_t)w;
       if (b >= nb) continue;
-      std::vector<float> tile((size_t)kChunk * (kChunk + 1), 0.f);
-      const uint64_t base = b * kBlock;
-      for (int lane = 0; lane < 64; ++lane)
-        for (int j = 0; j < kBlock / 256; ++j) {
-          const int e0 = j * 256 + 4 * lane;
-          const uint64_t i0 = base + (uint64_t)e0;
-          float v[4];
-          for (int e = 0; e < 4; ++e) v[e] = (i0 + 4 <= S || i0 + e < S) ? p[i0 + e] : 0.f;
-          const size_t row = (size_t)(e0 >> 6) * (kChunk + 1) + (size_t)(e0 & 63);
-          for (int e = 0; e < 4; ++e) tile.at(row + e) = v[e];
-        }
       double cs[64];
       for (int lane = 0; lane < 64; ++lane) {
+        const uint64_t i0 = b * kBlock + (uint64_t)lane * kChunk;
         double acc = 0.0;
-        for (int k = 0; k < kChunk; ++k) acc += (double)tile.at((size_t)lane * (kChunk + 1) + k);
+        if (i0 + kChunk <= S) {
+          for (int j = 0; j < kChunk / 4; ++j)
+            for (int e = 0; e < 4; ++e) acc += (double)p[i0 + 4 * j + e];
+        } else {
+          for (int k = 0; k < kChunk; ++k) acc += (double)(i0 + k < S ? p[i0 + k] : 0.f);
+        }
         cs[lane] = acc;
         csum[b * 64 + (uint64_t)lane] = acc;
       }
@@ -110,33 +107,39 @@ std::vector<uint64_t> sample_emulated(const std::vector<float>& pin, const std::
       for (int c = 0; c < 64; ++c) tot += cs[c];
       bsum[b] = tot;
     }
-  // k_sample_block_cdf: one work group of 1024 threads, thread t owns segment t
-  std::vector<double> segtot(1024, 0.0);
-  for (uint64_t t = 0; t < 1024; ++t) {
+  // k_sample_seg_prefix: one work group per segment of kSeg blocks
+  const uint64_t nseg = (nb + kSeg - 1) / kSeg;
+  for (uint64_t g = 0; g < nseg; ++g) {
+    const uint64_t a = g * kSeg;
+    const int len = (int)(nb - a < (uint64_t)kSeg ? nb - a : (uint64_t)kSeg);
+    std::vector<double> seg((size_t)kSeg, 0.0);
+    for (int i = 0; i < len; ++i) seg.at((size_t)i) = bsum[a + (uint64_t)i];
     double acc = 0.0;
-    for (uint64_t i = t * kSeg; i < t * kSeg + kSeg && i < nb; ++i) {
-      acc += bsum[i];
-      cdf[i] = acc;
+    for (int i = 0; i < len; ++i) {
+      acc += seg.at((size_t)i);
+      seg.at((size_t)i) = acc;
     }
-    segtot[t] = acc;
+    segtot[g] = acc;
+    for (int i = 0; i < len; ++i) cdf[a + (uint64_t)i] = seg.at((size_t)i);
   }
+  // k_sample_seg_offsets: one thread, in order
   double run = 0.0;
-  for (int i = 0; i < 1024; ++i) {
+  for (uint64_t i = 0; i < nseg; ++i) {
     const double v = segtot[i];
     segtot[i] = run;
     run += v;
   }
-  for (uint64_t t = 0; t < 1024; ++t)
-    for (uint64_t i = t * kSeg; i < t * kSeg + kSeg && i < nb; ++i) cdf[i] = segtot[t] + cdf[i];
-  // k_sample_search: ceil(shots / 4) work groups of 4 waves, one wave per shot
+  // k_sample_add_offsets: grid-stride over the blocks
+  for (uint64_t i = 0; i < nb; ++i) cdf[i] = segtot[i / kSeg] + cdf[i];
+  // k_sample_search: ceil(shots / 256) work groups of 256 threads, one thread per shot
   const int64_t shots = (int64_t)uni.size();
   Checked<double> u_d(uni.size(), "uniforms");
   for (size_t i = 0; i < uni.size(); ++i) u_d[i] = uni[i];
   Checked<uint64_t> idx(uni.size(), "idx");
-  const int64_t grid3 = (shots + 3) / 4;
+  const int64_t grid3 = (shots + 255) / 256;
   for (int64_t bx = 0; bx < grid3; ++bx)
-    for (int w = 0; w < 4; ++w) {
-      const int64_t s = bx * 4 + w;
+    for (int t = 0; t < 256; ++t) {
+      const int64_t s = bx * 256 + t;
       if (s >= shots) continue;
       const double total = cdf[nb - 1];
       const double u = u_d[(uint64_t)s] * total;
@@ -148,25 +151,22 @@ std::vector<uint64_t> sample_emulated(const std::vector<float>& pin, const std::
       }
       const uint64_t b = lo;
       double r = u - (b > 0 ? cdf[b - 1] : 0.0);
-      double csl[64];
-      for (int lane = 0; lane < 64; ++lane) csl[lane] = csum[b * 64 + (uint64_t)lane];
       int c = 63;
       double pre = 0.0;
       for (int q = 0; q < 64; ++q) {
-        if (pre + csl[q] >= r) {
+        const double v = csum[b * 64 + (uint64_t)q];
+        if (pre + v >= r) {
           c = q;
           break;
         }
-        pre += csl[q];
+        pre += v;
       }
       r -= pre;
       const uint64_t first = b * kBlock + (uint64_t)c * kChunk;
-      float pl[64];
-      for (int lane = 0; lane < 64; ++lane) pl[lane] = first + lane < S ? p[first + (uint64_t)lane] : 0.f;
       int k = 63;
       double acc = 0.0;
       for (int q = 0; q < kChunk; ++q) {
-        acc += (double)pl[q];
+        acc += (double)(first + (uint64_t)q < S ? p[first + (uint64_t)q] : 0.f);
         if (acc >= r) {
           k = q;
           break;
