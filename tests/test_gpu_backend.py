@@ -44,6 +44,35 @@ def test_fock_input_amplitudes_probabilities_and_samples():
     assert len(s) == 500 and all(sum(t) == 3 for t in s)
 
 
+def test_complex128_backend():
+    """StrongBackend(dtype=torch.complex128) builds complex128 kernels (backends.py:59-77,219 pass the
+    dtype on): probabilities and amplitudes to FP64 rounding against Ryser permanents of the exact
+    U, a lossy block in float64, and samples."""
+    m, occ = 7, [1, 0, 1, 1, 0, 0, 0]
+    U = haar_unitary(m, 41)
+    Un = U.numpy()
+    be = StrongBackend(device=DEV, dtype=torch.complex128)
+    be.set_circuit(Un)
+    be.set_input_state(occ)
+    assert be._kernels[3].cdtype == torch.complex128
+    final = so.to_occupation(so.enumerate_layers(3, m)[3], m).tolist()
+    ref = so.reference_probs(Un, 3, m, occ)
+    for i in (0, 9, 40, len(final) - 1):
+        t = final[i]
+        assert abs(be.probability(t) - ref[i]) < 1e-13
+        assert abs(be.prob_amplitude(t) - _amp(Un, occ, t)) < 1e-13
+    assert abs(sum(be.all_prob()) - 1) < 1e-12
+    lossy = StrongBackend(device=DEV, dtype=torch.complex128, loss=0.6)
+    lossy.set_circuit(Un)
+    lossy.set_input_state(occ)
+    lossy._ensure()
+    ld = lossy._lossy[3]
+    assert all(q.dtype == torch.float64 for q in ld.probs)
+    assert abs(sum(float(ld.mass(k)) for k in range(4)) - 1) < 1e-12
+    s = be.samples(300, generator=torch.Generator(device=DEV).manual_seed(2))
+    assert len(s) == 300 and all(sum(t) == 3 for t in s)
+
+
 def test_rank_lookup_at_the_frontier():
     n, m = 11, 32
     be = StrongBackend(device=DEV)
