@@ -460,8 +460,13 @@ def main() -> None:
     def last_layer_ms(flags: int, reps: int = 5) -> float:
         """The last-layer launch alone (the same kernel probs(U) ends with, on the same inputs):
         layer n-1 computed once, then `reps` launches of layer n bracketed by HIP events on the
-        stream they run on; the mean duration."""
+        stream they run on; the mean duration. Compensated sums carry hi/lo layers, as the
+        compensated forward does (the last layer then reads the parents' low-order parts too)."""
+        if flags & nat.FLAG_COMPENSATED:
+            flags |= nat.FLAG_HILO
         psi = torch.ones((1, 1), dtype=torch.complex64, device=dev)
+        if flags & nat.FLAG_HILO:
+            psi = torch.stack([psi, torch.zeros_like(psi)])
         for k in range(1, n):
             psi = plan.layer_step(U, k, modes[k - 1], psi, nat.OUT_RAW, flags=flags)
         q = plan.layer_step(U, n, modes[n - 1], psi, nat.OUT_PROBS, scale=1.0, flags=flags)
