@@ -322,19 +322,9 @@ def bench_next_rows(kern, U: torch.Tensor) -> dict:
     return out
 
 
-def full_fp64_accuracy(p: torch.Tensor, n: int, m: int, occ, U128: torch.Tensor, dev: str) -> dict:
-    """Every output of the complex64 step `p` against the engine's complex128 forward (FP64
-    throughout, pinned to the oracle and to Ryser permanents in the GPU tests) of the exact U:
-    max|dp|, TVD, the max relative error over outputs above 1e-12, and the complex128 forward's
-    own time and FP64 mass."""
-    from photonic_engines_amd.slos import FockSLOSKernel
-    k = FockSLOSKernel(n, m, occ, device=dev, dtype=torch.complex128)
-    Uz = U128.to(dev)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    p64 = k.probs(Uz)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+def compare_every_output(p: torch.Tensor, p64: torch.Tensor) -> dict:
+    """max|dp|, TVD and the max relative error over the outputs above 1e-12 of a float32
+    distribution against a float64 one of the same size, on the GPU in chunks."""
     mx = tvd = rel = 0.0
     chunk = 1 << 28
     for i in range(0, p64.numel(), chunk):
@@ -346,9 +336,30 @@ def full_fp64_accuracy(p: torch.Tensor, n: int, m: int, occ, U128: torch.Tensor,
         if bool(big.any()):
             rel = max(rel, float((d[big] / b[big]).max()))
         del a, b, d, big
-    res = {"outputs": int(p64.numel()), "max_abs_dp": mx, "tvd": 0.5 * tvd, "max_rel_above_1e-12": rel,
-           "oracle": "the engine's complex128 forward of the same U (FP64)", "complex128_ms": dt * 1e3,
-           "complex128_mass_minus_1": float(k.total_mass(p64)) - 1.0}
+    return {"outputs": int(p64.numel()), "max_abs_dp": mx, "tvd": 0.5 * tvd, "max_rel_above_1e-12": rel}
+
+
+def fp64_forward(n: int, m: int, occ, U128: torch.Tensor, dev: str):
+    """The engine's complex128 forward (FP64 throughout, pinned to the oracle and to Ryser
+    permanents in the GPU tests) of U128: (float64 probabilities, seconds, FP64 mass - 1)."""
+    from photonic_engines_amd.slos import FockSLOSKernel
+    k = FockSLOSKernel(n, m, occ, device=dev, dtype=torch.complex128)
+    Uz = U128.to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p64 = k.probs(Uz)
+    torch.cuda.synchronize()
+    return p64, time.perf_counter() - t0, float(k.total_mass(p64)) - 1.0
+
+
+def full_fp64_accuracy(p: torch.Tensor, n: int, m: int, occ, U128: torch.Tensor, dev: str) -> dict:
+    """Every output of the complex64 step `p` against the engine's complex128 forward of the exact
+    U (the complex64 step sees U rounded to complex64): max|dp|, TVD, the max relative error over
+    outputs above 1e-12, and the complex128 forward's own time and FP64 mass."""
+    p64, dt, mass = fp64_forward(n, m, occ, U128, dev)
+    res = compare_every_output(p, p64)
+    res.update({"oracle": "the engine's complex128 forward of the same U (FP64)", "complex128_ms": dt * 1e3,
+                "complex128_mass_minus_1": mass})
     del p64
     torch.cuda.empty_cache()
     return res
@@ -575,6 +586,20 @@ def main() -> None:
         out["variants"] = {"compensated": {"ms_per_step": dtc * 1e3, "value": S_n / dtc, "kernel_ms": last_c,
                                            "max_abs_dp": float(np.abs(gc - ref).max()),
                                            "max_rel": float((np.abs(gc - ref) / np.maximum(ref, 1e-300)).max())}}
+        # the arithmetic alone: both arms against FP64 arithmetic on the complex64 U they are given
+        # (the complex128 forward of that U upcast), every output; the FP64 forward's time beside
+        p64, dt64, _ = fp64_forward(n, m, occ, U[0].to(torch.complex128), str(dev))
+        pc = kc.probs(U2)
+        cmp_c = compare_every_output(pc, p64)
+        del pc
+        pp = kern.probs(U2)
+        cmp_p = compare_every_output(pp, p64)
+        del pp, p64
+        torch.cuda.empty_cache()
+        out["variants"]["arithmetic_vs_fp64_on_the_same_complex64_u"] = {
+            "compensated": cmp_c, "plain": cmp_p, "fp64_ms_per_step": dt64 * 1e3,
+            "compensated_ms_per_step": dtc * 1e3, "plain_ms_per_step": dt * 1e3,
+            "compensated_speedup_vs_fp64": dt64 / dtc}
     torch.cuda.empty_cache()
     if not args.no_lossy:
         out.setdefault("variants", {})["lossy"] = bench_lossy(kern, U[0])
