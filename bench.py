@@ -86,6 +86,12 @@ def bench_uneven(reps: int = 5) -> dict:
     Us = [torch.stack([haar_unitary(m, 100_000 * (g + 1) + i) for i in range(c)]).to(torch.complex64).to(dev)
           for g, (n, m, c) in enumerate(UNEVEN)]
     total = sum(k.output_size() * c for k, (_, _, c) in zip(ks, UNEVEN))
+    # compulsory HBM bytes of the whole workload: every layer of every instance read once and written
+    # once (complex64), the last written as float32
+    alg = sum(bytes_alg(n, m) * c for n, m, c in UNEVEN)
+
+    def rate(t: float) -> dict:
+        return {"ms": t * 1e3, "probs_per_s": total / t, "alg_GBps": alg / t / 1e9, "frac": alg / t / 1e9 / PEAK_HBM_GBS}
 
     def clock(fn):
         fn()
@@ -97,19 +103,19 @@ def bench_uneven(reps: int = 5) -> dict:
         return (time.perf_counter() - t0) / reps
 
     res = {"workload": "512 Haar unitaries: 171 x (4,12), 171 x (6,16), 170 x (8,20), photons in modes 0..n-1, "
-                       "complex64, float32 probabilities", "outputs": total}
+                       "complex64, float32 probabilities", "outputs": total, "alg_bytes": alg}
     t = clock(lambda: [k.probs(U[i]) for k, U in zip(ks, Us) for i in range(U.shape[0])])
-    res["single_u_loop"] = {"ms": t * 1e3, "probs_per_s": total / t}
+    res["single_u_loop"] = rate(t)
     t = clock(lambda: [k.probs(U) for k, U in zip(ks, Us)])
-    res["batched_per_shape"] = {"ms": t * 1e3, "probs_per_s": total / t}
+    res["batched_per_shape"] = rate(t)
     batch = UnevenBatch(ks)
     t = clock(lambda: batch.probs(Us))
-    res["group_untuned"] = {"ms": t * 1e3, "probs_per_s": total / t}
+    res["group_untuned"] = rate(t)
     cfg = batch.tune(Us, reps=2)
     batch.capture(Us)
     t = clock(lambda: batch.replay())
-    res["group_tuned_graph"] = {"ms": t * 1e3, "probs_per_s": total / t,
-                                "tuned": [f"L{c['layer']}:{c['kernel']}/{c['per_wave']}/{c['grid']}" for c in cfg]}
+    res["group_tuned_graph"] = dict(rate(t), tuned=[f"L{c['layer']}:{c['kernel']}/{c['per_wave']}/{c['grid']}"
+                                                    for c in cfg])
     outs = batch.replay()
     torch.cuda.synchronize()
     rng = np.random.default_rng(512)
