@@ -252,13 +252,30 @@ def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torc
         if not Ts:
             continue
         k = n - L
-        q = torch.cat([A[T] for T in Ts], 0)                                          # [len(Ts) * B, S_k]
-        ct = torch.as_tensor([cols[T[0]] for T in Ts], dtype=torch.int64, device=dev)
-        W = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(len(Ts) * B, m).to(rdt)
-        r = _raise_rows(kernel, q, k, W).reshape(len(Ts), B, -1)
-        for i, T in enumerate(Ts):
+        # every raised row goes to A[T minus its smallest photon]. The rows are raised in an order
+        # that makes each target's j-th contribution (its sources in ascending order of that
+        # photon, Ts being sorted) part of one contiguous block: targets by descending source
+        # count, block j = the j-th source of every target that has one. A target's contributions
+        # are then summed source by source with contiguous prefix additions.
+        srcs: dict = {}
+        for T in Ts:
+            srcs.setdefault(T[1:], []).append(T)
+        order = sorted(srcs, key=lambda t: -len(srcs[t]))                  # stable: ties in set order
+        cnt = [sum(1 for t in order if len(srcs[t]) > j) for j in range(len(srcs[order[0]]))]
+        rows = [srcs[t][j] for j in range(len(cnt)) for t in order[:cnt[j]]]
+        q = torch.cat([A[T] for T in rows], 0)                                        # [len(Ts) * B, S_k]
+        ct = torch.as_tensor([cols[T[0]] for T in rows], dtype=torch.int64, device=dev)
+        W = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(len(rows) * B, m).to(rdt)
+        r = _raise_rows(kernel, q, k, W).reshape(len(rows), B, -1)
+        for T in Ts:
             del A[T]
-            add(T[1:], r[i])
+        acc = r[:cnt[0]].clone()
+        o = cnt[0]
+        for c in cnt[1:]:
+            acc[:c] += r[o:o + c]
+            o += c
+        for ti, t in enumerate(order):
+            add(t, acc[ti])
     if n >= 2:
         add((), _subset_forwards(kernel, Ub, cols, [list(cols)])[0] * (x ** n))
     if () not in A:  # no photon
