@@ -44,6 +44,9 @@ class UnevenBatch:
                 raise TypeError("UnevenBatch takes FockSLOSKernel objects")
             if k.normalization != "deferred":
                 raise ValueError("UnevenBatch runs deferred normalisation only")
+            if k.cdtype != torch.complex64:
+                raise ValueError("UnevenBatch runs complex64 kernels (the group launch); a complex128 kernel "
+                                 "computes through its own probs(U)")
         self.kernels = kernels
         comp = kernels[0].compensated if compensated is None else bool(compensated)
         self.flags = nat.FLAG_COMPENSATED if comp else 0

@@ -174,5 +174,8 @@ def test_memory_plan_and_preflight():
         FockSLOSKernel(6, 10, [1] * 6 + [0] * 4, dtype=torch.complex32)
     k128 = FockSLOSKernel(6, 10, [1] * 6 + [0] * 4, dtype=torch.complex128)
     assert k128.rdtype == torch.float64 and k128.memory_plan(2)["output"] == mf["output"]
+    from photonic_engines_amd.slos import UnevenBatch
+    with pytest.raises(ValueError, match="complex64"):
+        UnevenBatch([k128])
     big = Plan(12, 64, "fock", list(range(12)))
     assert big.memory_plan(1)["output"] == 4 * math.comb(75, 12)              # ~105 TB of probabilities
