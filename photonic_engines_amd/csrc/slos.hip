@@ -1664,7 +1664,7 @@ static int adjoint_blocks(uint64_t Sp) { return (int)std::max<uint64_t>(1, std::
 
 namespace {
 
-// parents per XCD chunk of the block descent and adjoint (see slos::xcd_chunk_index)
+// parents per XCD chunk of the block adjoint (see slos::xcd_chunk_index)
 constexpr uint64_t kDescChunkParents = 1ull << 18;
 
 // How one adjoint layer runs: the block kernel (multiset spaces whose parent layer has a 1..8
@@ -1787,12 +1787,15 @@ int slos_loss_descend(const slos_plan* p, int k, const float* q_k, float* q_km1,
     const int maxd = dp <= 4 ? 4 : dp <= 6 ? 6 : 8;
     const size_t shb = slos::desc_dyn_lds(k, p->me, maxd);
     if (shb <= lds_limit()) {
+      // parents per wave: 1024 on layers of 2^27 parents or more, 512 below, fewer on small layers;
+      // XCD chunks of 2^16 parents (measured at n = 11, m = 32: the k = 11, 10, 9 steps 34.4, 9.0,
+      // 2.5 ms with 4096 per wave and chunks of 2^18, 32.6, 8.1, 2.0 ms so)
       uint64_t pw = (Sp / 16384 + 63) & ~63ull;
-      pw = std::min<uint64_t>(std::max<uint64_t>(pw, 256), 4096);
+      pw = std::min<uint64_t>(std::max<uint64_t>(pw, 256), Sp >= (1ull << 27) ? 1024 : 512);
       const uint64_t wgs = ((Sp + pw - 1) / pw + slos::kWaves - 1) / slos::kWaves;
       if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
       const dim3 grid((unsigned)wgs, (unsigned)batch);
-      const uint32_t xc = (uint32_t)std::max<uint64_t>(1, kDescChunkParents / ((uint64_t)slos::kWaves * pw));
+      const uint32_t xc = (uint32_t)std::max<uint64_t>(1, (1ull << 16) / ((uint64_t)slos::kWaves * pw));
       const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
       const uint64_t S_pre = p->size(dp);
 #define SLOS_DESC(D, E)                                                                                                  \
