@@ -371,6 +371,50 @@ def full_fp64_accuracy(p: torch.Tensor, n: int, m: int, occ, U128: torch.Tensor,
     return res
 
 
+def bench_gradient_frontier(steps: int = 2) -> dict:
+    """n=11, m=32 (4.28e9 outputs): one gradient step of L = -mean log p over 1,024 outputs drawn
+    from the exact distribution (probs(U) under autograd, L.backward() through the adjoint
+    kernels: every other layer kept, the last layer's amplitudes recomputed); time and peak
+    device memory. Skipped when the device has less than 130 GB free."""
+    from photonic_engines_amd.slos import FockSLOSKernel, haar_unitary
+    from photonic_engines_amd.slos.sampling import draw_indices
+    dev = torch.device("cuda", 0)
+    torch.cuda.empty_cache()
+    free = torch.cuda.mem_get_info(dev)[0]
+    if free < 130e9:
+        return {"skipped": f"needs about 130 GB free, {free / 1e9:.0f} GB are"}
+    n, m = 11, 32
+    k = FockSLOSKernel(n, m, [1] * n + [0] * (m - n), device="cuda")
+    U0 = haar_unitary(m, 1132).to(torch.complex64).cuda()
+    with torch.no_grad():
+        p = k.probs(U0)
+        idx = draw_indices(p, 1024, torch.Generator(device="cuda").manual_seed(8))
+    del p
+    torch.cuda.empty_cache()
+
+    def step():
+        U = U0.clone().requires_grad_(True)
+        loss = -torch.log(k.probs(U)[idx]).mean()
+        loss.backward()
+        return U.grad
+
+    g = step()
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    res = {"workload": "n=11 m=32, first 11 modes, L = -mean log p over 1,024 sampled outputs",
+           "states": int(k.output_size()), "ms_per_step": dt * 1e3,
+           "peak_GB": torch.cuda.max_memory_allocated(dev) / 1e9,
+           "grad_nonzero_columns": int((g.abs().sum(0) > 0).sum()), "grad_finite": bool(torch.isfinite(g).all())}
+    del g, k
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_stored_edge(threads: int, reps: int = 3) -> dict:
     """The reference's own algorithm on the CPU, where its edge tables fit: the torch-CPU
     restatement of the stored-edge mode-slab forward (oracle/stored_edge.py) at n=8, m=24 (7.9M
@@ -613,6 +657,7 @@ def main() -> None:
     if not args.no_gradient:
         out.setdefault("variants", {})["gradient_n6_m16"] = bench_gradient()
         out["variants"]["gradient_n8_m24"] = bench_adjoint()
+        out["variants"]["gradient_n11_m32"] = bench_gradient_frontier()
         torch.cuda.empty_cache()
     if not args.no_next_rows:
         out.setdefault("variants", {})["next_rows"] = bench_next_rows(kern, U[0])

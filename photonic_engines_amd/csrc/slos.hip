@@ -1686,9 +1686,10 @@ int adjoint_launch(const slos_plan* p, int k, AdjLaunch* L) {
     const int maxd = dp <= 4 ? 4 : dp <= 6 ? 6 : 8;
     const size_t shb = slos::desc_dyn_lds(k, p->me, maxd);
     if (shb + sizeof(double) * 2 * slos::kWaves * 32 <= lds_limit()) {
-      // parents per wave: at least 2048, and at most about 2^16 work groups (FP64 partials per mode)
+      // parents per wave: at most about 2^16 work groups (FP64 partials per mode), at least 256
+      // (at n = 8, m = 24 the top layer took 0.46 ms with at least 2048 a wave, 0.25 ms so)
       uint64_t pw = (Sp / ((uint64_t)slos::kWaves << 16) + 63) & ~63ull;
-      pw = std::max<uint64_t>(pw, 2048);
+      pw = std::max<uint64_t>(pw, 256);
       const uint64_t wgs = ((Sp + pw - 1) / pw + slos::kWaves - 1) / slos::kWaves;
       if (wgs > 0x7fffffffull) return fail(SLOS_ERANGE, "layer too large for one launch");
       L->block = true;
