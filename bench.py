@@ -26,8 +26,9 @@ group launch per layer, and the autotuned group forward replayed as a hipGraph; 
 built on the path (`next_rows`): device sampling of 10^6 patterns from the n=11, m=32
 distribution, the x-model and a 4-component coherent superposition at n=8, m=24, grouped
 probabilities, and rank lookups of the strong-simulation backend at n=11, m=32; and
-(`published_shapes`) probs(U) on every shape the reference publishes an H100 time for, timed the
-way its product bench times it, with that time and its source file beside ours.
+(`published_shapes`) probs(U) on every shape the reference publishes a time for (its torch kernel
+on an H100, its C engine on 48 CPU threads), timed the way its product bench times it, with that
+time and its source file beside ours.
 """
 from __future__ import annotations
 
@@ -467,24 +468,38 @@ def cpu_model() -> str:
     return "unknown"
 
 
-# The reference's own published probs(U) times (one H100; `ms` column of each file, median of the
-# timed calls after one warm-up): (tag, n, m, input occupation, dtype label, H100 ms, source).
+# The reference's own published times for this computation: (tag, n, m, input occupation, dtype
+# label, reference ms, hardware, source). The H100 rows are FockSLOSKernel.probs(U) (`ms` column,
+# median of the timed calls after one warm-up); the CPU rows are its C engine on one 48-thread
+# rank (`total_s`, basis build included).
+H100 = "1x NVIDIA H100 PCIe, FockSLOSKernel.probs"
+CPU48 = "48-core CPU node, the reference's C engine slos_dist.c v2, 1 rank x 48 threads"
 PUBLISHED_SHAPES = [
-    ("n11_m22", 11, 22, [1] * 11 + [0] * 11, "complex64", 98.908,
+    ("n11_m22", 11, 22, [1] * 11 + [0] * 11, "complex64", 98.908, H100,
      "results/keep/sbs_20260919_product_slos_r2_n11.csv:2"),
-    ("n4_m8", 4, 8, [1] * 4 + [0] * 4, "complex64", 1.99, "results/keep/sbs_20260919_product_slos_r2.csv:2"),
-    ("n6_m12", 6, 12, [1] * 6 + [0] * 6, "complex64", 3.739, "results/keep/sbs_20260919_product_slos_r2.csv:4"),
-    ("n8_m16", 8, 16, [1] * 8 + [0] * 8, "complex64", 5.139, "results/keep/sbs_20260919_product_slos_r2.csv:6"),
-    ("n10_m20", 10, 20, [1] * 10 + [0] * 10, "complex64", 19.357, "results/keep/sbs_20260919_product_slos_r2.csv:8"),
-    ("n4_m12", 4, 12, [1] * 4 + [0] * 8, "unrecorded", 1.965, "results/keep/sbs_20260919_product_slos_r3_c128.csv:2"),
-    ("n6_m18", 6, 18, [1] * 6 + [0] * 12, "unrecorded", 4.226, "results/keep/sbs_20260919_product_slos_r3_c128.csv:4"),
-    ("n8_m24", 8, 24, [1] * 8 + [0] * 16, "unrecorded", 12.335, "results/keep/sbs_20260919_product_slos_r3_c128.csv:6"),
-    ("n6_m12_alternating", 6, 12, [1, 0] * 6, "complex64", 0.29,
+    ("n4_m8", 4, 8, [1] * 4 + [0] * 4, "complex64", 1.99, H100, "results/keep/sbs_20260919_product_slos_r2.csv:2"),
+    ("n6_m12", 6, 12, [1] * 6 + [0] * 6, "complex64", 3.739, H100, "results/keep/sbs_20260919_product_slos_r2.csv:4"),
+    ("n8_m16", 8, 16, [1] * 8 + [0] * 8, "complex64", 5.139, H100, "results/keep/sbs_20260919_product_slos_r2.csv:6"),
+    ("n10_m20", 10, 20, [1] * 10 + [0] * 10, "complex64", 19.357, H100,
+     "results/keep/sbs_20260919_product_slos_r2.csv:8"),
+    ("n4_m12", 4, 12, [1] * 4 + [0] * 8, "unrecorded", 1.965, H100,
+     "results/keep/sbs_20260919_product_slos_r3_c128.csv:2"),
+    ("n6_m18", 6, 18, [1] * 6 + [0] * 12, "unrecorded", 4.226, H100,
+     "results/keep/sbs_20260919_product_slos_r3_c128.csv:4"),
+    ("n8_m24", 8, 24, [1] * 8 + [0] * 16, "unrecorded", 12.335, H100,
+     "results/keep/sbs_20260919_product_slos_r3_c128.csv:6"),
+    ("n6_m12_alternating", 6, 12, [1, 0] * 6, "complex64", 0.29, H100,
      "results/keep/bench_emitter_slos_h100_20260909_modeslab.csv:2"),
-    ("n8_m16_alternating", 8, 16, [1, 0] * 8, "complex64", 0.40,
+    ("n8_m16_alternating", 8, 16, [1, 0] * 8, "complex64", 0.40, H100,
      "results/keep/bench_emitter_slos_h100_20260909_modeslab.csv:3"),
-    ("n10_m20_alternating", 10, 20, [1, 0] * 10, "complex64", 10.57,
+    ("n10_m20_alternating", 10, 20, [1, 0] * 10, "complex64", 10.57, H100,
      "results/keep/bench_emitter_slos_h100_20260909_modeslab.csv:4"),
+    ("n11_m22_c_engine", 11, 22, [1] * 11 + [0] * 11, "complex128", 1918.0, CPU48,
+     "results/keep/cpu48_scaling_v2.csv:8"),
+    ("n12_m24_c_engine", 12, 24, [1] * 12 + [0] * 12, "complex128", 11284.0, CPU48,
+     "results/keep/cpu48_scaling_v2.csv:14"),
+    ("n13_m26_c_engine", 13, 26, [1] * 13 + [0] * 13, "complex128", 73087.0, CPU48,
+     "results/keep/cpu48_scaling_v2_n13.csv:2"),
 ]
 
 
@@ -492,8 +507,8 @@ def bench_published_shapes(repeats: int = 7) -> dict:
     """FockSLOSKernel.probs(U) on every shape the reference publishes a time for, timed as the
     reference's product bench times it (bench/bench_product_slos.py:45-58: one warm-up call, then
     the median of `repeats` calls, each closed by a device synchronisation). The U is this build's
-    seeded Haar unitary, seed 100 n + m. Where the reference's file does not record the dtype, the
-    complex128 forward is timed as well."""
+    seeded Haar unitary, seed 100 n + m. Rows whose reference dtype is not complex64 (complex128,
+    or not recorded) are timed in complex128 as well."""
     from photonic_engines_amd.slos import FockSLOSKernel, haar_unitary
 
     def median_ms(fn):
@@ -508,21 +523,21 @@ def bench_published_shapes(repeats: int = 7) -> dict:
         return sorted(ts)[len(ts) // 2]
 
     out = {}
-    for tag, n, m, occ, dt_label, ref_ms, src in PUBLISHED_SHAPES:
+    for tag, n, m, occ, dt_label, ref_ms, hw, src in PUBLISHED_SHAPES:
         U128 = haar_unitary(m, 100 * n + m)
-        rec = {"n": n, "m": m, "input": occ, "reference_h100_ms": ref_ms, "reference_dtype": dt_label,
-               "reference_source": src}
+        rec = {"n": n, "m": m, "input": occ, "reference_ms": ref_ms, "reference_dtype": dt_label,
+               "reference_hardware": hw, "reference_source": src}
         dtypes = [torch.complex64] if dt_label == "complex64" else [torch.complex64, torch.complex128]
         for cd in dtypes:
             k = FockSLOSKernel(n, m, occ, device="cuda", dtype=cd)
             Ud = U128.to(cd).cuda()
             ms = median_ms(lambda: k.probs(Ud))
             name = "complex64" if cd == torch.complex64 else "complex128"
-            rec[name] = {"ms": ms, "probs_per_s": k.output_size() / (ms * 1e-3), "speedup_vs_reference_h100": ref_ms / ms}
+            rec[name] = {"ms": ms, "probs_per_s": k.output_size() / (ms * 1e-3), "speedup_vs_reference": ref_ms / ms}
             rec["states"] = k.output_size()
             del k, Ud
+            torch.cuda.empty_cache()
         out[tag] = rec
-        torch.cuda.empty_cache()
     return out
 
 
@@ -545,7 +560,7 @@ def main() -> None:
     ap.add_argument("--no-next-rows", action="store_true",
                     help="skip the sampling / x-model / superposition / grouped / backend timings")
     ap.add_argument("--no-published-shapes", action="store_true",
-                    help="skip probs(U) on the shapes the reference publishes H100 times for")
+                    help="skip probs(U) on the shapes the reference publishes times for")
     ap.add_argument("--copy-calibration-gb", type=float, default=0.0,
                     help="also run one plain device copy of this many GB (HBM counter calibration)")
     args = ap.parse_args()

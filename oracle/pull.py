@@ -41,7 +41,16 @@ def lib():
     return _lib
 
 
-def _modes(occ):
+def _modes(occ, n=None, m=None, U=None):
+    """Input modes of the photons, one per photon in ascending mode order. With n and m given, the
+    occupation must have m entries summing to n and U (if given) must be [m, m]: the C routines
+    index U by these modes and would read outside it otherwise."""
+    if m is not None and len(occ) != m:
+        raise ValueError(f"occupation has {len(occ)} modes, expected m = {m}")
+    if n is not None and sum(int(c) for c in occ) != n:
+        raise ValueError(f"occupation holds {sum(int(c) for c in occ)} photons, expected n = {n}")
+    if U is not None and np.shape(U) != (m, m):
+        raise ValueError(f"U must be [{m}, {m}], got {np.shape(U)}")
     return [j for j, c in enumerate(occ) for _ in range(int(c))]
 
 
@@ -73,7 +82,7 @@ def unrank(n, m, mask, k, idx) -> np.ndarray:
 
 def forward_probs(n, m, occ, U, mask="fock", prec=128, threads=0) -> np.ndarray:
     """Lossless probabilities of the pull recurrence in complex128 (prec=128) or complex64 (64)."""
-    modes = np.asarray(_modes(occ), dtype=np.int32)
+    modes = np.asarray(_modes(occ, n, m, U), dtype=np.int32)
     Ud = np.ascontiguousarray(np.asarray(U, dtype=np.complex128)).view(np.float64)
     in_norm = 1.0 / math.sqrt(math.prod(math.factorial(int(c)) for c in occ))
     out = np.zeros(layer_size(n, m, mask, n), dtype=np.float64)
@@ -86,7 +95,7 @@ def forward_probs(n, m, occ, U, mask="fock", prec=128, threads=0) -> np.ndarray:
 
 def forward_amps(n, m, occ, U, mask="fock", threads=0) -> np.ndarray:
     """complex128 raw (deferred-normalisation) amplitudes times the input norm."""
-    modes = np.asarray(_modes(occ), dtype=np.int32)
+    modes = np.asarray(_modes(occ, n, m, U), dtype=np.int32)
     Ud = np.ascontiguousarray(np.asarray(U, dtype=np.complex128)).view(np.float64)
     in_norm = 1.0 / math.sqrt(math.prod(math.factorial(int(c)) for c in occ))
     out = np.zeros(layer_size(n, m, mask, n), dtype=np.complex128)
@@ -106,7 +115,7 @@ def permanent(A) -> complex:
 
 def spot_probs(n, m, occ, U, idx, mask="fock", threads=0) -> np.ndarray:
     """Ryser (complex128) probabilities of the outputs with lex indices idx."""
-    modes = np.asarray(_modes(occ), dtype=np.int32)
+    modes = np.asarray(_modes(occ, n, m, U), dtype=np.int32)
     Ud = np.ascontiguousarray(np.asarray(U, dtype=np.complex128)).view(np.float64)
     s_fact = float(math.prod(math.factorial(int(c)) for c in occ))
     idx = np.ascontiguousarray(idx, dtype=np.uint64)
@@ -121,7 +130,7 @@ def spot_probs(n, m, occ, U, idx, mask="fock", threads=0) -> np.ndarray:
 def forward_sample(n, m, occ, U, last_count, mask="fock", prec=64, threads=0):
     """Timed bounded sample of the pull recurrence: layers 1..n-1 complete plus the first
     `last_count` states of layer n. Returns (prefix_seconds, last_seconds, states_done, checksum)."""
-    modes = np.asarray(_modes(occ), dtype=np.int32)
+    modes = np.asarray(_modes(occ, n, m, U), dtype=np.int32)
     Ud = np.ascontiguousarray(np.asarray(U, dtype=np.complex128)).view(np.float64)
     t = np.zeros(2)
     cs = np.zeros(1)

@@ -37,7 +37,7 @@ FWD = [(4, 6, "fock", None), (5, 7, "fock", None), (6, 6, "fock", [3, 0, 2, 0, 1
        (7, 9, "fock", [1, 0, 2, 1, 0, 3, 0, 0, 0]), (8, 10, "fock", None), (6, 8, "fock", [6, 0, 0, 0, 0, 0, 0, 0]),
        (7, 12, "fock", None), (5, 20, "sparse_1in4", None), (9, 5, "fock", [2, 2, 2, 2, 1]), (10, 5, "fock", [3, 0, 4, 1, 2]),
        (11, 4, "fock", [5, 0, 3, 3]), (12, 4, "fock", [3, 3, 3, 3]), (13, 3, "fock", [5, 4, 4]), (9, 12, "unbunched", None),
-       (10, 20, "dual_rail", None), (5, 40, "unbunched", None), (7, 20, "fock", None)]
+       (10, 20, "dual_rail", None), (5, 40, "unbunched", None), (7, 20, "fock", None), (13, 9, "fock", [2, 2, 2, 2, 1, 1, 1, 1, 1])]
 
 
 @pytest.mark.parametrize("suf", [4, 5, 6])
@@ -51,6 +51,7 @@ def test_block_kernel_decomposition(emu, n, m, mask, occ, suf):
     if suf == 6 and not (n >= 7 and mask in ("fock", "tail_empty")):
         pytest.skip("no group-order last layer")
     occ = occ or occupation_list(n, m, mask)
+    assert len(occ) == m and sum(occ) == n                 # the emulator indexes U by these modes
     U = haar_unitary(m, n + 7 * m).numpy()
     ref = pull.forward_probs(n, m, occ, U, mask)
     modes = np.array([j for j, c in enumerate(occ) for _ in range(c)], dtype=np.int32)

@@ -45,6 +45,8 @@ CASES = [(3, 6, "fock", None), (3, 6, "unbunched", None), (3, 6, "dual_rail", No
          (9, 5, "fock", [2, 2, 2, 2, 1]), (10, 5, "fock", [3, 0, 4, 1, 2]), (11, 4, "fock", [5, 0, 3, 3]),
          (12, 4, "fock", [3, 3, 3, 3]), (13, 3, "fock", [5, 4, 4]), (9, 12, "unbunched", None),
          (10, 20, "dual_rail", None),
+         # 5-position suffixes with 7 and 8 distinct prefix modes (layers 12 and 13)
+         (13, 9, "fock", [2, 2, 2, 2, 1, 1, 1, 1, 1]),
          # the block kernel's widest mode count (6-bit positions in the tuple words)
          (5, 64, "fock", None), (6, 64, "unbunched", None)]
 
