@@ -24,6 +24,16 @@ def _dtype_flags(cdtype: torch.dtype, flags: int) -> int:
     return flags & ~nat.FLAG_F64
 
 
+# Batch rows one launch takes: the batch is the launch grid's y dimension. Larger batches run as
+# consecutive launches over row ranges (every kernel treats its rows independently, so each row's
+# result is the same as in a single launch); an empty batch launches nothing.
+MAX_ROWS = 65535
+
+
+def _row_ranges(B: int) -> list[tuple[int, int]]:
+    return [(i, min(B, i + MAX_ROWS)) for i in range(0, B, MAX_ROWS)]
+
+
 def _out_dtypes(cdtype: torch.dtype) -> tuple[torch.dtype, torch.dtype]:
     """(probability dtype, amplitude dtype) of a forward over `cdtype` data."""
     return (torch.float64, torch.complex128) if cdtype == torch.complex128 else (torch.float32, torch.complex64)
@@ -133,7 +143,7 @@ class Plan:
             raise ValueError("U must be a contiguous complex64 or complex128 [B, m, m] tensor")
         flags = _dtype_flags(U.dtype, flags)
         S = self.sizes[layer]
-        ws_bytes = self.workspace_bytes(B, layer, flags)
+        ws_bytes = self.workspace_bytes(max(1, min(B, MAX_ROWS)), layer, flags)
         rdt, cdt = _out_dtypes(U.dtype)
         out_bytes = 0 if out is not None else B * S * (rdt if out_kind == nat.OUT_PROBS else cdt).itemsize
         self.preflight(out_bytes + ws_bytes, dev, f"SLOS forward (n={self.n}, m={self.m}, batch {B}: "
@@ -147,9 +157,10 @@ class Plan:
         mp = None
         if modes is not None:
             mp = (ctypes.c_int32 * layer)(*[int(i) for i in modes])
-        nat.check(self._lib.slos_forward(self._h, nat.ptr(U), B, mp, layer, out_kind, nat.ptr(out),
-                                         float(scale), int(flags), nat.ptr(ws), ws.numel(),
-                                         nat.stream_handle(dev)), "slos_forward")
+        for i, j in _row_ranges(B):
+            nat.check(self._lib.slos_forward(self._h, nat.ptr(U[i:j]), j - i, mp, layer, out_kind, nat.ptr(out[i:j]),
+                                             float(scale), int(flags), nat.ptr(ws), ws.numel(),
+                                             nat.stream_handle(dev)), "slos_forward")
         return out
 
     def layer_step(self, U: torch.Tensor, k: int, col: int, psi_prev: torch.Tensor, out_kind: int = nat.OUT_RAW,
@@ -172,9 +183,22 @@ class Plan:
             out = torch.empty(shape, dtype=dtype, device=U.device)
         elif out.shape != shape or out.dtype != dtype or out.device != U.device or not out.is_contiguous():
             raise ValueError(f"out must be a contiguous {dtype} tensor of shape {shape} on {U.device}")
-        nat.check(self._lib.slos_layer_step(self._h, nat.ptr(U), B, k, int(col), nat.ptr(psi_prev), out_kind,
-                                            nat.ptr(out), float(scale), int(flags), nat.stream_handle(U.device)),
-                  "slos_layer_step")
+        hilo_in = bool(flags & nat.FLAG_HILO)
+        hilo_out = len(shape) == 3
+        ranges = _row_ranges(B)
+        whole = len(ranges) == 1
+        for i, j in ranges:
+            if whole:
+                pin, pout = psi_prev, out
+            else:
+                # a hi/lo pair [2, B, S] is passed as one contiguous block: its row ranges are copies
+                pin = psi_prev[:, i:j].contiguous() if hilo_in else psi_prev[i:j]
+                pout = torch.empty((2, j - i, S), dtype=dtype, device=U.device) if hilo_out else out[i:j]
+            nat.check(self._lib.slos_layer_step(self._h, nat.ptr(U[i:j]), j - i, k, int(col), nat.ptr(pin), out_kind,
+                                                nat.ptr(pout), float(scale), int(flags),
+                                                nat.stream_handle(U.device)), "slos_layer_step")
+            if not whole and hilo_out:
+                out[:, i:j].copy_(pout)
         return out
 
     def readout_adjoint(self, amps: torch.Tensor, grad_p: torch.Tensor, scale: float) -> torch.Tensor:
@@ -182,8 +206,9 @@ class Plan:
         f64 = amps.dtype == torch.complex128
         fn = self._lib.slos_readout_adjoint_f64 if f64 else self._lib.slos_readout_adjoint
         grad_p = grad_p.to(torch.float64 if f64 else torch.float32).contiguous()
-        nat.check(fn(self._h, amps.shape[0], nat.ptr(amps), nat.ptr(grad_p), float(scale), nat.ptr(g),
-                     nat.stream_handle(amps.device)), "slos_readout_adjoint")
+        for i, j in _row_ranges(amps.shape[0]):
+            nat.check(fn(self._h, j - i, nat.ptr(amps[i:j]), nat.ptr(grad_p[i:j]), float(scale), nat.ptr(g[i:j]),
+                         nat.stream_handle(amps.device)), "slos_readout_adjoint")
         return g
 
     def adjoint_layer(self, U: torch.Tensor, k: int, col: int, psi_prev: torch.Tensor, grad_k: torch.Tensor,
@@ -195,27 +220,29 @@ class Plan:
             raise ValueError(f"psi_prev, grad_k and grad_U must be {U.dtype} like U")
         wsb = ctypes.c_size_t()
         wsfn = self._lib.slos_adjoint_workspace_f64 if f64 else self._lib.slos_adjoint_workspace
-        nat.check(wsfn(self._h, B, k, ctypes.byref(wsb)), "slos_adjoint_workspace")
+        nat.check(wsfn(self._h, max(1, min(B, MAX_ROWS)), k, ctypes.byref(wsb)), "slos_adjoint_workspace")
         ws = torch.empty(max(int(wsb.value), 1), dtype=torch.uint8, device=U.device)
         gprev = torch.empty((B, self.sizes[k - 1]), dtype=U.dtype, device=U.device) if want_prev else None
         fn = self._lib.slos_adjoint_layer_f64 if f64 else self._lib.slos_adjoint_layer
-        nat.check(fn(self._h, nat.ptr(U), B, k, int(col), nat.ptr(psi_prev), nat.ptr(grad_k),
-                     nat.ptr(gprev) if want_prev else None, nat.ptr(grad_U), 1, nat.ptr(ws), ws.numel(),
-                     nat.stream_handle(U.device)), "slos_adjoint_layer")
+        for i, j in _row_ranges(B):
+            nat.check(fn(self._h, nat.ptr(U[i:j]), j - i, k, int(col), nat.ptr(psi_prev[i:j]), nat.ptr(grad_k[i:j]),
+                         nat.ptr(gprev[i:j]) if want_prev else None, nat.ptr(grad_U[i:j]), 1, nat.ptr(ws), ws.numel(),
+                         nat.stream_handle(U.device)), "slos_adjoint_layer")
         return gprev
 
     def loss_descend(self, k: int, q: torch.Tensor, flags: int = 0, scale: float = 1.0) -> torch.Tensor:
         """q_{k-1} = scale * descent of q_k ([B, S_k] float32, or float64 in FP64, contiguous on the GPU)."""
         out = torch.empty((q.shape[0], self.sizes[k - 1]), dtype=q.dtype, device=q.device)
-        if q.dtype == torch.float64:
-            rc = self._lib.slos_loss_descend_f64(self._h, k, nat.ptr(q), nat.ptr(out), q.shape[0], float(scale),
-                                                 nat.stream_handle(q.device))
-        elif q.dtype == torch.float32:
-            rc = self._lib.slos_loss_descend(self._h, k, nat.ptr(q), nat.ptr(out), q.shape[0], flags, float(scale),
-                                             nat.stream_handle(q.device))
-        else:
+        if q.dtype not in (torch.float32, torch.float64):
             raise ValueError(f"q must be float32 or float64, got {q.dtype}")
-        nat.check(rc, "slos_loss_descend")
+        for i, j in _row_ranges(q.shape[0]):
+            if q.dtype == torch.float64:
+                rc = self._lib.slos_loss_descend_f64(self._h, k, nat.ptr(q[i:j]), nat.ptr(out[i:j]), j - i,
+                                                     float(scale), nat.stream_handle(q.device))
+            else:
+                rc = self._lib.slos_loss_descend(self._h, k, nat.ptr(q[i:j]), nat.ptr(out[i:j]), j - i, flags,
+                                                 float(scale), nat.stream_handle(q.device))
+            nat.check(rc, "slos_loss_descend")
         return out
 
     def loss_ascend(self, k: int, g: torch.Tensor) -> torch.Tensor:
@@ -224,15 +251,17 @@ class Plan:
         fn = self._lib.slos_loss_ascend_f64 if g.dtype == torch.float64 else self._lib.slos_loss_ascend
         if g.dtype not in (torch.float32, torch.float64):
             raise ValueError(f"g must be float32 or float64, got {g.dtype}")
-        nat.check(fn(self._h, k, nat.ptr(g), nat.ptr(out), g.shape[0], nat.stream_handle(g.device)),
-                  "slos_loss_ascend")
+        for i, j in _row_ranges(g.shape[0]):
+            nat.check(fn(self._h, k, nat.ptr(g[i:j]), nat.ptr(out[i:j]), j - i, nat.stream_handle(g.device)),
+                      "slos_loss_ascend")
         return out
 
     def raise_layer(self, k: int, w: torch.Tensor, q: torch.Tensor) -> torch.Tensor:
         """q_k(c) = sum_{j distinct in c} w[b, j] q_{k-1}(c - e_j): w [B, m], q [B, S_{k-1}] float32 (GPU)."""
         out = torch.empty((q.shape[0], self.sizes[k]), dtype=torch.float32, device=q.device)
-        nat.check(self._lib.slos_raise(self._h, k, nat.ptr(w), nat.ptr(q), nat.ptr(out), q.shape[0],
-                                       nat.stream_handle(q.device)), "slos_raise")
+        for i, j in _row_ranges(q.shape[0]):
+            nat.check(self._lib.slos_raise(self._h, k, nat.ptr(w[i:j]), nat.ptr(q[i:j]), nat.ptr(out[i:j]), j - i,
+                                           nat.stream_handle(q.device)), "slos_raise")
         return out
 
     def total_mass(self, probs: torch.Tensor) -> torch.Tensor:
@@ -243,10 +272,11 @@ class Plan:
         B, S = probs.shape
         out = torch.empty(B, dtype=torch.float64, device=probs.device)
         wsb = ctypes.c_size_t()
-        nat.check(self._lib.slos_mass_workspace(B, S, ctypes.byref(wsb)), "slos_mass_workspace")
-        ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=probs.device)
-        nat.check(self._lib.slos_total_mass(nat.ptr(probs), B, S, nat.ptr(out), nat.ptr(ws), ws.numel(),
-                                            nat.stream_handle(probs.device)), "slos_total_mass")
+        nat.check(self._lib.slos_mass_workspace(max(1, min(B, MAX_ROWS)), S, ctypes.byref(wsb)), "slos_mass_workspace")
+        ws = torch.empty(max(int(wsb.value), 1), dtype=torch.uint8, device=probs.device)
+        for i, j in _row_ranges(B):
+            nat.check(self._lib.slos_total_mass(nat.ptr(probs[i:j]), j - i, S, nat.ptr(out[i:j]), nat.ptr(ws),
+                                                ws.numel(), nat.stream_handle(probs.device)), "slos_total_mass")
         return out
 
     def spot_probs(self, U128: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:

@@ -29,7 +29,7 @@ import torch
 from .. import _native as nat
 from ..imperfections import LossyDistribution  # noqa: F401  (re-exported like the reference)
 from ..masks import Mask
-from ._plan import Plan
+from ._plan import MAX_ROWS, Plan, _row_ranges
 from .basis import RankedLayers, get_layers
 
 
@@ -311,11 +311,12 @@ class GroupedSLOSKernel:
         B = q.shape[0]
         n_chunks = chunks.shape[0]
         out = torch.empty((B, self.n_groups), dtype=q.dtype, device=q.device)
-        ws = torch.empty(max(B * n_chunks, 1), dtype=torch.float64, device=q.device)
+        ws = torch.empty(max(min(B, MAX_ROWS) * n_chunks, 1), dtype=torch.float64, device=q.device)
         fn = nat.lib().slos_group_sums_f64 if q.dtype == torch.float64 else nat.lib().slos_group_sums
-        nat.check(fn(nat.ptr(q), B, q.shape[1], nat.ptr(order), nat.ptr(chunks), n_chunks, nat.ptr(first),
-                     self.n_groups, nat.ptr(out), nat.ptr(ws), ws.numel() * 8, nat.stream_handle(q.device)),
-                  "slos_group_sums")
+        for i, j in _row_ranges(B):
+            nat.check(fn(nat.ptr(q[i:j]), j - i, q.shape[1], nat.ptr(order), nat.ptr(chunks), n_chunks, nat.ptr(first),
+                         self.n_groups, nat.ptr(out[i:j]), nat.ptr(ws), ws.numel() * 8, nat.stream_handle(q.device)),
+                      "slos_group_sums")
         return out if batched else out.squeeze(0)
 
 
@@ -402,7 +403,7 @@ class SuperpositionSLOSKernel:
         else:
             raw = self._plan.forward(Up, modes=list(range(self.n)), layer=self.n, out_kind=nat.OUT_AMPS, scale=1.0)
         coef = (w * torch.as_tensor(self._norms, dtype=self.rdtype, device=dev)).to(cdt)
-        amp = (raw.reshape(K, B, -1) * coef.view(K, 1, 1)).sum(0)
+        amp = (raw.reshape(K, B, raw.shape[-1]) * coef.view(K, 1, 1)).sum(0)
         amp = amp if batched else amp.squeeze(0)
         return amp.to(self.device) if self.device.type != "cuda" else amp
 

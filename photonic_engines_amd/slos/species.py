@@ -174,7 +174,7 @@ def _subset_forwards(kernel, Ub: torch.Tensor, cols: Sequence[int], goods: Seque
         norms.append(1.0 / math.prod(math.factorial(v) for v in counts.values()))
     if any(v != 1.0 for v in norms):                                                  # 1 / prod s_i!
         q = q * torch.as_tensor(norms, dtype=q.dtype, device=dev).repeat_interleave(B).unsqueeze(1)
-    return q.reshape(C, B, -1)
+    return q.reshape(C, B, q.shape[-1])
 
 
 def _raise_rows(kernel, q: torch.Tensor, k: int, W: torch.Tensor) -> torch.Tensor:
@@ -218,7 +218,7 @@ def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torc
     B, m = Ub.shape[0], Ub.shape[-1]
     dev = Ub.device
     b_max = n if order is None else max(0, min(int(order), n))
-    per = max(1, (2 << 30) // (rdt.itemsize * kernel.layers.size(n) * B))
+    per = max(1, (2 << 30) // (rdt.itemsize * kernel.layers.size(n) * max(B, 1)))
     A: dict = {}
 
     def add(T, v):
@@ -266,7 +266,8 @@ def x_mixture(kernel, U: torch.Tensor, indist, order: int | None = None) -> torc
         q = torch.cat([A[T] for T in rows], 0)                                        # [len(Ts) * B, S_k]
         ct = torch.as_tensor([cols[T[0]] for T in rows], dtype=torch.int64, device=dev)
         W = (Ub[:, :, ct].abs() ** 2).permute(2, 0, 1).reshape(len(rows) * B, m).to(rdt)
-        r = _raise_rows(kernel, q, k, W).reshape(len(rows), B, -1)
+        r = _raise_rows(kernel, q, k, W)
+        r = r.reshape(len(rows), B, r.shape[-1])
         for T in Ts:
             del A[T]
         acc = r[:cnt[0]].clone()
