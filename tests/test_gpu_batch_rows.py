@@ -65,6 +65,24 @@ def test_loss_and_gradient_beyond_one_launch():
     assert torch.count_nonzero(g_all[:, :, [0, 2]]).item() > 0
 
 
+def test_compensated_gradient_beyond_one_launch():
+    """compensated=True under autograd steps through hi/lo layer pairs [2, B, S], whose row ranges are
+    passed as copies: the probabilities equal the plain compensated forward's, bit for bit, and the
+    gradient rows of the second range equal those of the same rows run alone."""
+    n, m, B = 3, 4, MAX_ROWS + 2
+    occ = [1, 1, 0, 1]
+    U = _haar_batch(B, m, 5).to(torch.complex64).to(DEV)
+    k = FockSLOSKernel(n, m, occ, device=DEV, compensated=True)
+    a = k.probs(U)
+    Ug = U.clone().requires_grad_(True)
+    b = k.probs(Ug)
+    assert torch.equal(a, b.detach())
+    b[:, 3].sum().backward()
+    Ut = U[MAX_ROWS:].clone().requires_grad_(True)
+    k.probs(Ut)[:, 3].sum().backward()
+    assert torch.equal(Ug.grad[MAX_ROWS:], Ut.grad)
+
+
 def test_xmodel_rows_beyond_one_launch():
     """n=3 at x=0.7: the forwards of the one-photon-distinguishable subsets run as 3 B rows and the
     raises as more; with B = 30,000 both exceed one launch."""
