@@ -165,13 +165,24 @@ __global__ __launch_bounds__(256) void k_readout_adjoint_f64(const uint64_t* cg_
   const SpaceView sv = load_space_f64(smem, cg_g, lo_g, me, K);
   __syncthreads();
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < S; c += (uint64_t)gridDim.x * blockDim.x) {
-    int p[KM];
-    unrank_fixed<KM>(sv, c, K, p);
-    const double f = run_factorial_f64<KM>(p, K, multiset);
+    bool have_f = false;
+    double f = 0.0;
     for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
-      const double2 v = a[(size_t)b * S + c];
-      const double w = 2.0 * s2 * f * gp[(size_t)b * S + c];
-      g[(size_t)b * S + c] = make_double2(w * v.x, w * v.y);
+      // zero grad_p: zero gradient, without the amplitude or the state's factor
+      const double gv = gp[(size_t)b * S + c];
+      double2 r = make_double2(0.0, 0.0);
+      if (gv != 0.0) {
+        if (!have_f) {
+          int p[KM];
+          unrank_fixed<KM>(sv, c, K, p);
+          f = run_factorial_f64<KM>(p, K, multiset);
+          have_f = true;
+        }
+        const double2 v = a[(size_t)b * S + c];
+        const double w = 2.0 * s2 * f * gv;
+        r = make_double2(w * v.x, w * v.y);
+      }
+      g[(size_t)b * S + c] = r;
     }
   }
 }

@@ -115,15 +115,19 @@ def test_c4_gradient_matches_finite_differences():
 @pytest.mark.parametrize("n,m,mask,occ", [(3, 6, "fock", [1, 0, 1, 0, 1, 0]), (3, 4, "fock", [2, 0, 1, 0]),
                                           (4, 8, "dual_rail", None), (3, 7, "tail_empty", None),
                                           (5, 7, "fock", [1, 1, 1, 1, 1, 0, 0]), (4, 9, "unbunched", None)])
-def test_gradient_matches_restatement(n, m, mask, occ):
+@pytest.mark.parametrize("sparse", [False, True])
+def test_gradient_matches_restatement(n, m, mask, occ, sparse):
     """Weighted-sum loss over every state, batched: adjoint kernels vs complex128 autograd of the
-    oracle's stored-edge recurrence written in torch (test_kernel.py:131-146 shape)."""
+    oracle's stored-edge recurrence written in torch (test_kernel.py:131-146 shape). `sparse`: the
+    weights of six states in seven are zero (the readout adjoint's zero-gradient entries)."""
     occ = occ or occupation_list(n, m, mask)
     Ub = torch.stack([haar_unitary(m, s) for s in (6, 7)])
     k = FockSLOSKernel(n, m, occ, mask=mask, device=DEV)
     U = Ub.to(torch.complex64).to(DEV).requires_grad_(True)
     p = k.probs(U)
     w = torch.linspace(0.5, 1.5, p.shape[-1], device=DEV)
+    if sparse:
+        w[torch.arange(p.shape[-1], device=DEV) % 7 != 3] = 0.0
     (p * w).sum().backward()
     L = so.enumerate_layers(n, m, mask)
     modes = [j for j, c in enumerate(occ) for _ in range(c)]
