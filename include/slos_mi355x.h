@@ -152,6 +152,14 @@ int slos_layer_step(const slos_plan* plan, const void* U, int64_t batch, int k, 
 int slos_readout_adjoint(const slos_plan* plan, int64_t batch, const void* amps, const float* grad_p,
                          double scale, void* grad_amps, slos_stream_t stream);
 
+/* The same adjoint from layer n-1 instead of layer n, for a grad_p that is zero almost everywhere (a
+ * loss over sampled outputs): where grad_p is nonzero the state's raw amplitude is pulled from
+ * psi_prev (complex64 [batch][S_{n-1}], layer n-1 of U, complex64 [batch][m][m]); grad_amps is zero
+ * elsewhere. One pass over grad_p and grad_amps in place of recomputing layer n. No counterpart of
+ * its own in the reference (whose autograd forms the dense product). */
+int slos_readout_adjoint_pull(const slos_plan* plan, const void* U, int64_t batch, const void* psi_prev,
+                              const float* grad_p, double scale, void* grad_amps, slos_stream_t stream);
+
 /* Adjoint of one layer (the backward of _CheckpointedLayer, kernel.py:63-80, written as a pull
  * over children instead of scatter_add_): grad_prev[t] = sum_j conj(U[b,j,col]) grad_k[t + e_j]
  * (skipped if grad_prev is NULL) and grad_U[b][j][col] (+)= sum_t conj(psi_prev[t]) grad_k[t + e_j]

@@ -40,6 +40,7 @@ SIGNATURES = {
     "slos_layer_step": (_c.c_int, [_vp, _vp, _c.c_int64, _c.c_int, _c.c_int, _vp, _c.c_int, _vp, _c.c_double,
                                    _c.c_uint, _vp]),
     "slos_readout_adjoint": (_c.c_int, [_vp, _c.c_int64, _vp, _vp, _c.c_double, _vp, _vp]),
+    "slos_readout_adjoint_pull": (_c.c_int, [_vp, _vp, _c.c_int64, _vp, _vp, _c.c_double, _vp, _vp]),
     "slos_adjoint_workspace": (_c.c_int, [_vp, _c.c_int64, _c.c_int, _c.POINTER(_c.c_size_t)]),
     "slos_adjoint_layer": (_c.c_int, [_vp, _vp, _c.c_int64, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int,
                                       _vp, _c.c_size_t, _vp]),

@@ -511,6 +511,51 @@ __global__ __launch_bounds__(256) void k_readout_adjoint(const uint64_t* cg_g, c
   }
 }
 
+// The same adjoint from layer K - 1 instead of layer K: where grad_p is nonzero the state's raw
+// amplitude is pulled from its parents in `par` (the parent sum of k_layer_simple, plain sums);
+// elsewhere the gradient is zero. For a grad_p that is zero almost everywhere this replaces the
+// recompute of layer K and the read of its amplitudes by one pass that reads grad_p and writes g.
+template <int KM>
+__global__ __launch_bounds__(256) void k_readout_adjoint_pull(const uint64_t* cg_g, const int* lo_g, int me, int multiset,
+                                                              int K, int m, int col, const float2* __restrict__ U,
+                                                              const float2* __restrict__ par, uint64_t S_par,
+                                                              const float* __restrict__ gp, float2* __restrict__ g,
+                                                              uint64_t S, int64_t B, float s2) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int stride = me + 1;
+  const int ncg = (K + 1) * stride;
+  uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
+  int* lo = reinterpret_cast<int*>(cg + ncg);
+  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = cg_g[i];
+  for (int i = threadIdx.x; i < me; i += blockDim.x) lo[i] = lo_g[i];
+  __syncthreads();
+  const slos::SpaceView sv{cg, lo, me, stride};
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < S; c += (uint64_t)gridDim.x * blockDim.x) {
+    bool have_p = false;
+    int p[KM];
+    float f = 0.f;
+    for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
+      const float gv = gp[(size_t)b * S + c];
+      float2 r = make_float2(0.f, 0.f);
+      if (gv != 0.f) {
+        if (!have_p) {
+          slos::unrank_fixed<KM>(sv, c, K, p);
+          f = slos::run_factorial_fixed<KM>(p, K, multiset);
+          have_p = true;
+        }
+        const float2* ub = U + (size_t)b * m * m;
+        const float2* pb = par + (size_t)b * S_par;
+        Acc acc;
+        slos::parents_fixed<KM>(sv, multiset, K, p,
+                                [&](uint64_t pr, int mode, int) { cmac<false>(acc, ub[(size_t)mode * m + col], pb[pr]); });
+        const float w = 2.f * s2 * f * gv;
+        r = make_float2(w * acc.re, w * acc.im);
+      }
+      g[(size_t)b * S + c] = r;
+    }
+  }
+}
+
 // g_{k-1}[t] = sum_{children c = t + e_j} conj(U[b, j, col]) g_k[c]   (pull over children)
 // and, per thread, gU_j += conj(psi_{k-1}[t]) g_k[t + e_j] in a private LDS row (no atomics);
 // each block then reduces its rows to partial[b][block][j] (FP64 stage in k_reduce_gu)
@@ -1667,6 +1712,30 @@ int slos_readout_adjoint(const slos_plan* p, int64_t batch, const void* amps, co
   hipLaunchKernelGGL(k_readout_adjoint, grid, dim3(256), sh, reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo,
                      p->me, p->multiset, p->n, static_cast<const float2*>(amps), grad_p,
                      static_cast<float2*>(grad_amps), S, batch, (float)(scale * scale));
+  HIP_TRY(hipGetLastError());
+  return SLOS_OK;
+}
+
+int slos_readout_adjoint_pull(const slos_plan* p, const void* U, int64_t batch, const void* psi_prev,
+                              const float* grad_p, double scale, void* grad_amps, slos_stream_t stream) {
+  if (!p || !U || !psi_prev || !grad_p || !grad_amps) return fail(SLOS_EINVAL, "NULL argument");
+  if (batch < 1 || batch > 65535) return fail(SLOS_EINVAL, "batch must be in [1, 65535]");
+  if (p->n < 1) return fail(SLOS_EINVAL, "no photon layer");
+  DevTables dt;
+  int rc = device_tables(p, &dt);
+  if (rc) return rc;
+  const int K = p->n;
+  const uint64_t S = p->size(K);
+  const size_t sh = (size_t)(K + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
+  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
+  dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((S + 255) / 256, 1u << 20)),
+            (unsigned)std::min<int64_t>(batch, 16));
+  with_km(K, [&](auto km) {
+    hipLaunchKernelGGL((k_readout_adjoint_pull<decltype(km)::value>), grid, dim3(256), sh,
+                       reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me, p->multiset, K, p->m,
+                       p->modes[K - 1], static_cast<const float2*>(U), static_cast<const float2*>(psi_prev),
+                       p->size(K - 1), grad_p, static_cast<float2*>(grad_amps), S, batch, (float)(scale * scale));
+  });
   HIP_TRY(hipGetLastError());
   return SLOS_OK;
 }

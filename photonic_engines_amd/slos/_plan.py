@@ -211,6 +211,19 @@ class Plan:
                          nat.stream_handle(amps.device)), "slos_readout_adjoint")
         return g
 
+    def readout_adjoint_pull(self, U: torch.Tensor, psi_prev: torch.Tensor, grad_p: torch.Tensor,
+                             scale: float) -> torch.Tensor:
+        """The readout adjoint from layer n-1 (psi_prev [B, S_{n-1}] complex64, of U): zero where
+        grad_p is zero, elsewhere with the state's amplitude pulled from its parents."""
+        B = U.shape[0]
+        grad_p = grad_p.to(torch.float32).contiguous()
+        g = torch.empty((B, self.sizes[self.n]), dtype=torch.complex64, device=U.device)
+        for i, j in _row_ranges(B):
+            nat.check(self._lib.slos_readout_adjoint_pull(self._h, nat.ptr(U[i:j]), j - i, nat.ptr(psi_prev[i:j]),
+                                                          nat.ptr(grad_p[i:j]), float(scale), nat.ptr(g[i:j]),
+                                                          nat.stream_handle(U.device)), "slos_readout_adjoint_pull")
+        return g
+
     def adjoint_layer(self, U: torch.Tensor, k: int, col: int, psi_prev: torch.Tensor, grad_k: torch.Tensor,
                       grad_U: torch.Tensor, want_prev: bool = True) -> torch.Tensor | None:
         """grad of layer k-1 (if want_prev) and grad_U[:, :, col] += the layer's contribution."""
