@@ -521,15 +521,9 @@ __global__ __launch_bounds__(256) void k_readout_adjoint_pull(const uint64_t* cg
                                                               const float2* __restrict__ par, uint64_t S_par,
                                                               const float* __restrict__ gp, float2* __restrict__ g,
                                                               uint64_t S, int64_t B, float s2) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int stride = me + 1;
-  const int ncg = (K + 1) * stride;
-  uint64_t* cg = reinterpret_cast<uint64_t*>(smem);
-  int* lo = reinterpret_cast<int*>(cg + ncg);
-  for (int i = threadIdx.x; i < ncg; i += blockDim.x) cg[i] = cg_g[i];
-  for (int i = threadIdx.x; i < me; i += blockDim.x) lo[i] = lo_g[i];
-  __syncthreads();
-  const slos::SpaceView sv{cg, lo, me, stride};
+  // the counting tables are read from global memory: only the rare nonzero entries unrank, so
+  // staging them in LDS per work group would cost more than it saves
+  const slos::SpaceView sv{cg_g, lo_g, me, me + 1};
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < S; c += (uint64_t)gridDim.x * blockDim.x) {
     bool have_p = false;
     int p[KM];
@@ -1726,12 +1720,10 @@ int slos_readout_adjoint_pull(const slos_plan* p, const void* U, int64_t batch, 
   if (rc) return rc;
   const int K = p->n;
   const uint64_t S = p->size(K);
-  const size_t sh = (size_t)(K + 1) * (p->me + 1) * 8 + (size_t)p->me * 4;
-  if (sh > lds_limit()) return fail(SLOS_ERANGE, "counting table does not fit LDS");
   dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((S + 255) / 256, 1u << 20)),
             (unsigned)std::min<int64_t>(batch, 16));
   with_km(K, [&](auto km) {
-    hipLaunchKernelGGL((k_readout_adjoint_pull<decltype(km)::value>), grid, dim3(256), sh,
+    hipLaunchKernelGGL((k_readout_adjoint_pull<decltype(km)::value>), grid, dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), dt.cg, dt.lo, p->me, p->multiset, K, p->m,
                        p->modes[K - 1], static_cast<const float2*>(U), static_cast<const float2*>(psi_prev),
                        p->size(K - 1), grad_p, static_cast<float2*>(grad_amps), S, batch, (float)(scale * scale));
