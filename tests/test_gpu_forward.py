@@ -127,6 +127,25 @@ def test_n9_m22_full_against_complex128():
     assert abs(float(k.total_mass(p)) - 1) < 1e-5
 
 
+def test_n11_m22_full_against_complex128():
+    """n=11, m=22, the shape the reference publishes its own probs(U) time for (129,024,480
+    outputs, results/keep/sbs_20260919_product_slos_r2_n11.csv:2): every output against the
+    complex128 pull oracle, relative to its own size, plus the TVD and the FP64 mass."""
+    n, m = 11, 22
+    U = haar_unitary(m, 11022)
+    occ = first_modes(n, m)
+    k = FockSLOSKernel(n, m, occ, device=DEV)
+    p = k.probs(U.to(torch.complex64).to(DEV))
+    assert p.shape == (129_024_480,)
+    ref = pull.forward_probs(n, m, occ, U.numpy())
+    pc = p.double().cpu().numpy()
+    d = np.abs(pc - ref)
+    assert d.max() <= 1e-9
+    assert np.all(d <= 1e-3 * ref + 1e-12)
+    assert 0.5 * d.sum() < 1e-5
+    assert abs(float(k.total_mass(p)) - 1) < 1e-5
+
+
 def test_batched_equals_looped_and_deterministic():                            # test_kernel.py:64-71
     n, m = 5, 10
     occ = occupation_list(n, m)
