@@ -127,16 +127,18 @@ def test_n9_m22_full_against_complex128():
     assert abs(float(k.total_mass(p)) - 1) < 1e-5
 
 
-def test_n11_m22_full_against_complex128():
-    """n=11, m=22, the shape the reference publishes its own probs(U) time for (129,024,480
-    outputs, results/keep/sbs_20260919_product_slos_r2_n11.csv:2): every output against the
-    complex128 pull oracle, relative to its own size, plus the TVD and the FP64 mass."""
-    n, m = 11, 22
-    U = haar_unitary(m, 11022)
+@pytest.mark.parametrize("n,m,S", [(11, 22, 129_024_480), (12, 24, 834_451_800)])
+def test_published_shapes_full_against_complex128(n, m, S):
+    """The shapes the reference publishes its own times for: n=11, m=22 (its probs(U),
+    results/keep/sbs_20260919_product_slos_r2_n11.csv:2) and n=12, m=24 (its C engine,
+    results/keep/cpu48_scaling_v2.csv:14; layer 12 runs the 5-position-suffix block kernel with 7
+    stream slots): every output against the complex128 pull oracle, relative to its own size,
+    plus the TVD and the FP64 mass."""
+    U = haar_unitary(m, 1000 * n + m)
     occ = first_modes(n, m)
     k = FockSLOSKernel(n, m, occ, device=DEV)
     p = k.probs(U.to(torch.complex64).to(DEV))
-    assert p.shape == (129_024_480,)
+    assert p.shape == (S,)
     ref = pull.forward_probs(n, m, occ, U.numpy())
     pc = p.double().cpu().numpy()
     d = np.abs(pc - ref)
