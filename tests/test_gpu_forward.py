@@ -127,15 +127,17 @@ def test_n9_m22_full_against_complex128():
     assert abs(float(k.total_mass(p)) - 1) < 1e-5
 
 
-@pytest.mark.parametrize("n,m,S", [(11, 22, 129_024_480), (12, 24, 834_451_800)])
-def test_published_shapes_full_against_complex128(n, m, S):
+@pytest.mark.parametrize("n,m,S,alternate", [(11, 22, 129_024_480, False), (12, 24, 834_451_800, False),
+                                             (10, 20, 20_030_010, True)])
+def test_published_shapes_full_against_complex128(n, m, S, alternate):
     """The shapes the reference publishes its own times for: n=11, m=22 (its probs(U),
-    results/keep/sbs_20260919_product_slos_r2_n11.csv:2) and n=12, m=24 (its C engine,
+    results/keep/sbs_20260919_product_slos_r2_n11.csv:2), n=12, m=24 (its C engine,
     results/keep/cpu48_scaling_v2.csv:14; layer 12 runs the 5-position-suffix block kernel with 7
-    stream slots): every output against the complex128 pull oracle, relative to its own size,
-    plus the TVD and the FP64 mass."""
+    stream slots) and n=10, m=20 with the photons in alternate modes
+    (results/keep/bench_emitter_slos_h100_20260909_modeslab.csv:4): every output against the
+    complex128 pull oracle, relative to its own size, plus the TVD and the FP64 mass."""
     U = haar_unitary(m, 1000 * n + m)
-    occ = first_modes(n, m)
+    occ = [1, 0] * n if alternate else first_modes(n, m)
     k = FockSLOSKernel(n, m, occ, device=DEV)
     p = k.probs(U.to(torch.complex64).to(DEV))
     assert p.shape == (S,)
