@@ -87,6 +87,21 @@ def test_complex128_c2_full():
     assert (p64.double() - p).abs().max().item() <= 1e-6
 
 
+def test_complex128_n11_m22_full():
+    """n=11, m=22 (129,024,480 outputs, the shape the reference publishes its probs(U) time for,
+    results/keep/sbs_20260919_product_slos_r2_n11.csv:2): every output of the complex128 engine
+    against the complex128 pull oracle, the TVD and the FP64 mass."""
+    n, m = 11, 22
+    U = haar_unitary(m, 1000 * n + m)
+    occ = first_modes(n, m)
+    p = FockSLOSKernel(n, m, occ, device=DEV, dtype=C128).probs(U.to(DEV))
+    ref = pull.forward_probs(n, m, occ, U.numpy())
+    pc = p.cpu().numpy()
+    _close(pc, ref, what="n=11, m=22")
+    assert 0.5 * np.abs(pc - ref).sum() < 1e-12
+    assert abs(float(pc.sum()) - 1) < 1e-12
+
+
 def test_complex128_batched_vacuum_and_errors():
     n, m = 5, 10
     occ = occupation_list(n, m)
