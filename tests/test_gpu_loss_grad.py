@@ -35,6 +35,23 @@ def test_thinned_distribution_matches_oracle(occ, eta):                         
     assert k.probs(U.to(torch.complex64), loss=1.0).shape == (k.output_size(),)
 
 
+def test_lossy_n11_m22_block_descent_against_fp64():
+    """n=11, m=22 (129M outputs; the descents k = 11..6 run on the block structure) at eta = 0.9:
+    every block of the complex64 engine against the complex128 engine's FP64 thread-per-state
+    descents of the same U, and the FP64 block masses against the binomial law."""
+    n, m, eta = 11, 22, 0.9
+    U = haar_unitary(m, 1000 * n + m)
+    occ = first_modes(n, m)
+    ld = FockSLOSKernel(n, m, occ, device=DEV).probs(U.to(torch.complex64).to(DEV), loss=eta)
+    ld64 = FockSLOSKernel(n, m, occ, device=DEV, dtype=torch.complex128).probs(U.to(DEV), loss=eta)
+    pk = binomial_pmf(n, eta)
+    for j in range(n + 1):
+        d = (ld.probs[j].double() - ld64.probs[j]).abs()
+        assert float(d.max()) <= 1e-7, j
+        assert float(d.sum()) <= 1e-5, j
+        assert abs(float(ld.probs[j].double().sum()) - float(pk[j])) < 1e-5, j
+
+
 @pytest.mark.parametrize("mask,n,m", [("unbunched", 3, 7), ("dual_rail", 3, 6), ("tail_empty", 3, 7),
                                       ("sparse_1in4", 2, 12)])
 def test_thinning_masks(mask, n, m):
