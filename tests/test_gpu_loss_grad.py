@@ -190,6 +190,30 @@ def test_sparse_readout_adjoint_equals_dense(monkeypatch):
     assert torch.count_nonzero(g_sparse[:, :, n:]) == 0
 
 
+def test_gradient_n11_m22_against_fp64():
+    """n=11, m=22 (129M outputs): L = -mean log p over 1,024 outputs, the complex64 gradient (the
+    readout adjoint pulled from layer 10, block-structured adjoint layers) against the complex128
+    engine's gradient of the same loss (FP64 thread-per-state adjoint kernels)."""
+    n, m = 11, 22
+    U = haar_unitary(m, 1000 * n + m)
+    occ = first_modes(n, m)
+    k = FockSLOSKernel(n, m, occ, device=DEV)
+    S = k.output_size()
+    idx = torch.as_tensor(np.random.default_rng(1122).choice(S, 1024, replace=False), device=DEV)
+
+    def grad(kern, Ud):
+        Ug = Ud.clone().requires_grad_(True)
+        loss = -torch.log(kern.probs(Ug)[idx]).mean()
+        loss.backward()
+        return Ug.grad
+
+    g32 = grad(k, U.to(torch.complex64).to(DEV)).to(torch.complex128)
+    g64 = grad(FockSLOSKernel(n, m, occ, device=DEV, dtype=torch.complex128), U.to(DEV))
+    scale = float(g64.abs().max())
+    assert float((g32 - g64).abs().max()) <= 1e-3 * scale
+    assert torch.count_nonzero(g64[:, n:]) == 0 and torch.count_nonzero(g64[:, :n]) > 0
+
+
 def test_compensated_probs_equal_with_and_without_grad():
     """FockSLOSKernel(compensated=True): every layer is compensated whether or not U requires a
     gradient (the autograd forward runs the same launches as the plain forward)."""
