@@ -110,3 +110,22 @@ def test_raise_as_complex_layer_step_equals_raise_kernel(n, m, mask, occ):
         b = plan.layer_step(Uw, kk, 0, torch.complex(q, torch.zeros_like(q)), nat.OUT_RAW)
         assert torch.all(b.imag == 0)
         assert torch.allclose(b.real, a, rtol=2e-6, atol=0), kk
+
+
+@pytest.mark.parametrize("order", [None, 2])
+def test_x_model_n10_m20_against_fp64(order):
+    """n=10, m=20 (20,030,010 outputs, every subset of the 10 photons, or the subsets with at most
+    2 distinguishable photons): the complex64 x-model against the complex128 engine's FP64 x-model
+    of the same U, and the FP64 mass."""
+    n, m, x = 10, 20, 0.8
+    U = haar_unitary(m, 1020)
+    occ = [1] * n + [0] * (m - n)
+    p = FockSLOSKernel(n, m, occ, device=DEV).probs(U.to(torch.complex64).to(DEV), indist=x, order=order)
+    p64 = FockSLOSKernel(n, m, occ, device=DEV, dtype=torch.complex128).probs(U.to(DEV), indist=x, order=order)
+    assert p.shape == (20_030_010,)
+    assert float((p.double() - p64).abs().max()) <= TOL
+    mass = float(p64.sum())
+    if order is None:
+        assert abs(mass - 1) < 1e-9
+    else:
+        assert abs(mass - (1 - truncation_bound(n, x, order))) < 1e-9
