@@ -130,3 +130,18 @@ def test_grouped_sums_large_groups_empty_groups_and_reproducible():
     pg = g.probs(Ug)
     pg[0].backward()
     assert Ug.grad is not None and float(Ug.grad.abs().sum()) > 0
+
+
+def test_superposition_n9_m18_against_fp64():
+    """n=9, m=18 (3,124,550 outputs), K = 3 components with complex weights: the complex64 kernel
+    against the complex128 kernel's FP64 forward of the same U and weights, and the FP64 mass
+    (sum |w_k|^2 for these mutually orthogonal Fock inputs)."""
+    n, m = 9, 18
+    occs = [[1] * 9 + [0] * 9, [0] * 9 + [1] * 9, [2, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 0, 0]]
+    w = torch.tensor([0.6, 0.48j, 0.64], dtype=torch.complex128)
+    U = haar_unitary(m, 918)
+    p = SuperpositionSLOSKernel(n, m, occs, device=DEV).probs(U.to(torch.complex64).to(DEV), weights=w)
+    p64 = SuperpositionSLOSKernel(n, m, occs, device=DEV, dtype=torch.complex128).probs(U.to(DEV), weights=w)
+    assert p.shape == (3_124_550,)
+    assert float((p.double() - p64).abs().max()) <= 1e-6
+    assert abs(float(p64.sum()) - float((w.abs() ** 2).sum())) < 1e-9
